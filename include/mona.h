@@ -1,0 +1,277 @@
+/*
+ * libmona -- MI355X (gfx950) implementation of Monalisa's non-Cartesian
+ * reconstruction hot path behind a C ABI.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for the reference's native (C++ MEX) inner
+ *     functions.  Same names, same argument meaning and layout, host
+ *     pointers, synchronous, outputs zero-filled by the callee, no status
+ *     (the reference returns void); on a failure they print the reason to
+ *     stderr and abort, and mona_last_error() holds the message.
+ *
+ *  2. A device-resident handle API (mona_*) for the operators and solvers:
+ *     device pointers, an explicit hipStream_t (0 = null stream), an int
+ *     status (MONA_OK or an error code; no exceptions cross the ABI).
+ *
+ * Complex data on the device API is interleaved float2 (re, im).  Layouts:
+ *   image  x   [prod(n_u)]            x fastest (the reference's column order)
+ *   k-space y  [nPt][nCh]             coil index fastest
+ *   grid       [prod(N_u)][nCh]       coil index fastest
+ *   coil maps  [prod(n_u)][nCh]       coil index fastest
+ */
+#ifndef MONA_H
+#define MONA_H
+
+#include <hip/hip_runtime.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MONA_OK 0
+#define MONA_ERR_ARG 1       /* invalid argument / failed structural check */
+#define MONA_ERR_HIP 2       /* HIP runtime error */
+#define MONA_ERR_OOM 3       /* device or host allocation failed */
+#define MONA_ERR_STATE 4     /* object not in the required state */
+#define MONA_ERR_INTERNAL 5
+
+#define MONA_KERNEL_GAUSS 0
+#define MONA_KERNEL_KAISER 1
+
+/* Message of the last failure in this process ("" if none). */
+const char* mona_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int mona_device_count(void);
+/* Library version string. */
+const char* mona_version(void);
+
+/* ======================================================================
+ * 1. Drop-in entry points (host pointers, synchronous, void)
+ * ====================================================================== */
+
+/* W = S*V, complex, column layout: v_re/v_im are n_vec planes of r_size
+ * floats, w_re/w_im are n_vec planes of l_size floats.  S is a cpp_prepared
+ * one-block bmSparseMat: r_jump running column deltas (first absolute,
+ * 0-based), r_nJump entries per stored row, l_jump running output-row
+ * deltas or NULL when rows are not squeezed.
+ * Replaces src/sparseMat/cpp/bmSparseMat_cC_oBlock_omp.h:9-14 (library
+ * symbol) and the inner function of
+ * src/sparseMat/mex/bmSparseMat_vec/bmSparseMat_cC_oBlock_omp/bmSparseMat_cC_oBlock_omp_mex.cpp:106-117. */
+void bmSparseMat_cC_oBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                               int l_size, int* l_jump, int l_nJump,
+                               float* v_real, float* v_imag, int n_vec,
+                               float* w_real, float* w_imag);
+
+/* Serial variant, same semantics.
+ * Replaces .../bmSparseMat_cC_oBlock/bmSparseMat_cC_oBlock_mex.cpp:82-193. */
+void bmSparseMat_cC_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                           int l_size, int* l_jump, int l_nJump,
+                           float* v_real, float* v_imag, int n_vec,
+                           float* w_real, float* w_imag);
+
+/* Variant that requires exactly 2*n_vec threads in the reference (planes are
+ * skipped when fewer are granted); here every plane is always computed.
+ * Replaces .../bmSparseMat_cC_oBlock_2_omp/bmSparseMat_cC_oBlock_2_omp_mex.cpp:83-201. */
+void bmSparseMat_cC_oBlock_2_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                                 int l_size, int* l_jump, int l_nJump,
+                                 float* v_real, float* v_imag, int n_vec,
+                                 float* w_real, float* w_imag);
+
+/* Complex row layout: v is [r_size][n_vec] (vector index fastest), w is
+ * [l_size][n_vec].
+ * Replaces .../bmSparseMat_cR_oBlock/bmSparseMat_cR_oBlock_mex.cpp:80-223. */
+void bmSparseMat_cR_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                           int l_size, int* l_jump, int l_nJump,
+                           float* v_real, float* v_imag, int n_vec,
+                           float* w_real, float* w_imag);
+
+/* Real column layout.
+ * Replaces .../bmSparseMat_rC_oBlock/bmSparseMat_rC_oBlock_mex.cpp:77-174
+ * and .../bmSparseMat_rC_oBlock_omp/bmSparseMat_rC_oBlock_omp_mex.cpp:79-152. */
+void bmSparseMat_rC_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                           int l_size, int* l_jump, int l_nJump,
+                           float* v, int n_vec, float* w);
+
+/* Real row layout.
+ * Replaces .../bmSparseMat_rR_oBlock/bmSparseMat_rR_oBlock_mex.cpp:77-194. */
+void bmSparseMat_rR_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                           int l_size, int* l_jump, int l_nJump,
+                           float* v, int n_vec, float* w);
+
+/* Multi-block row-layout products: the block partition only changes how
+ * the reference splits the work over threads; entries are still a single
+ * running delta sequence, except that the first entry of every block (at
+ * m_block_start[b]) and the first output row of every block (at
+ * l_block_start[b]) are absolute 0-based.
+ * Replace .../bmSparseMat_cR_nBlock_omp/bmSparseMat_cR_nBlock_omp_mex.cpp:97-483
+ * and .../bmSparseMat_rR_nBlock_omp/bmSparseMat_rR_nBlock_omp_mex.cpp:97-436. */
+void bmSparseMat_cR_nBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                               int l_size, int* l_jump, int l_nJump,
+                               int nBlock, int* block_length, int* l_block_start,
+                               long long* m_block_start,
+                               float* v_real, float* v_imag, int n_vec,
+                               float* w_real, float* w_imag);
+void bmSparseMat_rR_nBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,
+                               int l_size, int* l_jump, int l_nJump,
+                               int nBlock, int* block_length, int* l_block_start,
+                               long long* m_block_start,
+                               float* v, int n_vec, float* w);
+
+/* Row-id expansion: ind[sum(r_nJump)] = 0-based row of each entry.
+ * Replaces .../bmSparseMat_r_nJump2index/bmSparseMat_r_nJump2index_mex.cpp:70-89. */
+void bmSparseMat_r_nJump2index_mex(int l_size, int* r_nJump, int* ind);
+
+/* Normalised n2u gridding, Gaussian weights computed on the fly, clipped
+ * box.  a_* are [nPt][nCh], b_* are [Nu][nCh] (coil fastest); t* are
+ * 1-based grid coordinates.
+ * Replace src/gridding123/mex/bmGridder_n2u_leight{1,2,3}/bmGridder_n2u_leight{1,2,3}_mex.cpp
+ * (leight3: :116-358). */
+void bmGridder_n2u_leight1_mex(float* aReal, float* aImag, float* bReal, float* bImag,
+                               float* tx, float* Dn, int nCh, int nPt, int Nx,
+                               int nWin, float kernelParam_1, float kernelParam_2);
+void bmGridder_n2u_leight2_mex(float* aReal, float* aImag, float* bReal, float* bImag,
+                               float* tx, float* ty, float* Dn, int nCh, int nPt,
+                               int Nx, int Ny, int nWin, float kernelParam_1,
+                               float kernelParam_2);
+void bmGridder_n2u_leight3_mex(float* aReal, float* aImag, float* bReal, float* bImag,
+                               float* tx, float* ty, float* tz, float* Dn, int nCh,
+                               int nPt, int Nx, int Ny, int Nz, int nWin,
+                               float kernelParam_1, float kernelParam_2);
+
+/* Normalised n2u with precomputed weights w[nPt][nNb] and running index
+ * jumps nJump[nPt*nNb] (first absolute, 0-based).
+ * Replaces src/gridding123/mex/bmGridder_n2u/bmGridder_n2u_mex.cpp:72-201. */
+void bmGridder_n2u_mex(float* aReal, float* aImag, float* bReal, float* bImag, float* w,
+                       int* nJump, int nNb, int nCh, int nPt, int Nu_tot);
+
+/* Number of in-box (point, neighbour) pairs with d^2 < (nWin/2)^2.
+ * Replace src/gridding123/mex/bmGriddingMatrix_secret_length{1,2,3}/..._mex.cpp
+ * (secret_length3: :70-222). */
+void bmGriddingMatrix_secret_length1_mex(float* tx, int nCh, int nPt, int Nx, int nWin,
+                                         long long* secret_length);
+void bmGriddingMatrix_secret_length2_mex(float* tx, float* ty, int nCh, int nPt, int Nx,
+                                         int Ny, int nWin, long long* secret_length);
+void bmGriddingMatrix_secret_length3_mex(float* tx, float* ty, float* tz, int nCh, int nPt,
+                                         int Nx, int Ny, int Nz, int nWin,
+                                         long long* secret_length);
+
+/* ======================================================================
+ * 2. Device-resident handle API
+ * ====================================================================== */
+
+typedef struct mona_sparse_s* mona_sparse;
+typedef struct mona_plan_s* mona_plan;
+
+/* --- bmSparseMat on the device ---------------------------------------- */
+
+/* Uploads a cpp_prepared one-block matrix given in the reference encoding
+ * (host arrays) and decodes it on the device.  Applies the rules of
+ * bmSparseMat.check (src/sparseMat/m/bmSparseMat.m:192-393). */
+int mona_sparse_create(int r_size, const int* r_jump, const int* r_nJump,
+                       const float* m_val, int l_size, const int* l_jump, int l_nJump,
+                       long long nnz, mona_sparse* out);
+int mona_sparse_destroy(mona_sparse S);
+/* nnz, l_nJump, r_size, l_size, squeezed flag (any pointer may be NULL). */
+int mona_sparse_info(mona_sparse S, long long* nnz, int* l_nJump, int* r_size,
+                     int* l_size, int* squeezed);
+/* Copies the reference encoding back to host arrays (l_jump ignored when
+ * not squeezed; any pointer may be NULL). */
+int mona_sparse_export(mona_sparse S, int* r_jump, int* r_nJump, float* m_val,
+                       int* l_jump);
+/* w = S v on the device, complex interleaved, vector index fastest:
+ * v [r_size][n_vec] float2, w [l_size][n_vec] float2. */
+int mona_sparse_apply(mona_sparse S, const void* v, void* w, int n_vec,
+                      hipStream_t stream);
+
+/* --- trajectory -> gridding matrices ----------------------------------
+ * bmTraj2SparseMat (src/gridding123/m/bmTraj2SparseMat.m:12-314):
+ * t [nPt][imDim] (physical k, point-major), ve [nPt] (may be NULL when
+ * only Gu/Gut are requested), N_u[imDim], dK_u[imDim].  kernelParam has
+ * 2 (gauss: sigma, K_max) or 3 (kaiser: tau, alpha, K_max) entries.
+ * which: bit 0 = Gn, bit 1 = Gu and Gut.  Outputs not requested may be NULL. */
+int mona_traj2sparsemat(int imDim, int nPt, const double* t, const double* ve,
+                        const int* N_u, const double* dK_u, int kernelType, int nWin,
+                        const double* kernelParam, int nParam, int which,
+                        mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut);
+
+/* --- reconstruction plan -----------------------------------------------
+ * Geometry + gridding kernel + coil maps; owns KFC/KFC_conj (bmKF.m:6-35,
+ * bmKF_conj.m:6-38), the oversampled-grid workspace and nFrames gridding
+ * matrix pairs (Gu, Gut), which it references but does not own. */
+int mona_plan_create(int imDim, const int* N_u, const int* n_u, const double* dK_u,
+                     int nCh, int kernelType, int nWin, const double* kernelParam,
+                     int nParam, mona_plan* out);
+int mona_plan_destroy(mona_plan P);
+/* Coil maps C [prod(n_u)][nCh] complex (device pointer).  Computes the
+ * deapodisation K (bmK.m:6-156) and KFC, KFC_conj on the device. */
+int mona_plan_set_coils(mona_plan P, const void* C, hipStream_t stream);
+/* Frame f uses Gu (nPt x prod(N_u)) and Gut (its transpose). */
+int mona_plan_set_frame(mona_plan P, int frame, mona_sparse Gu, mona_sparse Gut);
+/* Deapodisation K cropped to n_u [prod(n_u)] float (device out). */
+int mona_plan_get_K(mona_plan P, float* K, hipStream_t stream);
+/* KFC and KFC_conj [prod(n_u)][nCh] float2 (device out, either may be NULL). */
+int mona_plan_get_KFC(mona_plan P, void* KFC, void* KFC_conj, hipStream_t stream);
+
+/* Forward model y = Gu * cfft(zerofill(x .* KFC))
+ * (bmShanna_MATLAB.m:6-56).  x [prod(n_u)], y [nPt][nCh]. */
+int mona_shanna(mona_plan P, int frame, const void* x, void* y, hipStream_t stream);
+/* Adjoint x = sum_c KFC_conj .* crop(cifft(Gut * y))  (C_flag = 1) or the
+ * per-coil images [prod(n_u)][nCh] (C_flag = 0)  (bmNakatsha_MATLAB.m:6-60). */
+int mona_nakatsha(mona_plan P, int frame, const void* y, void* x, int C_flag,
+                  hipStream_t stream);
+
+/* Centred n-D DFT over the first imDim axes of a [prod(N)][nBatch] float2
+ * array, in place: fftshift(fft(ifftshift(.))) per axis (inverse: ifft,
+ * scaled 1/N per axis).  Follows bmShanna_MATLAB.m:46-50 /
+ * bmNakatsha_MATLAB.m:36-40. */
+int mona_cfft(int imDim, const int* N, int nBatch, void* data, int inverse,
+              hipStream_t stream);
+
+/* CG-SENSE (src/fourier123/solver_function/imDim/nonCartesian/bmSensa.m:6-96).
+ * x [prod(n_u)] in/out (initial guess), y [nPt][nCh], ve [nPt] (volume
+ * elements; HY = min(ve, ve_max); ve_max <= 0 means max(ve)).
+ * residual (host, may be NULL) receives nIter values of <r, HY r>.
+ * Vectors FP32, inner products and CG scalars FP64. */
+int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve,
+               float ve_max, int nIter, int nCGD, double* residual, hipStream_t stream);
+
+/* Spatial-TV ADMM (src/fourier123/solver_function/imDim/nonCartesian/bmSteva.m:6-138).
+ * z, u [imDim][prod(n_u)] float2 in/out (z_init/u_init flags: 0 = compute
+ * z = grad(x), u = 0).  delta/rho: 2 values each (linear schedule start,
+ * end over nIter).  hist (host, may be NULL): nIter x {residual, TV}. */
+int mona_steva(mona_plan P, int frame, void* x, void* z, void* u, int zu_given,
+               const void* y, const float* ve, float ve_max, const float* delta,
+               const float* rho, int nIter, int nCGD, double* hist,
+               hipStream_t stream);
+
+/* Temporal-TV ADMM over the plan's nFr frames with identity deformations
+ * (bmTevaMorphosia_chain.m:6-294, regul_mode 'normal', Tu = []).
+ * x, z, u: nFr images each, contiguous [nFr][prod(n_u)] float2.
+ * y: frame f at y_off[f] float2 elements into y ([nPt_f][nCh]); ve likewise
+ * at ve_off[f] (floats).  hist (host, may be NULL): nIter x {objective,
+ * dafi, regul}. */
+int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
+                    const void* y, const long long* y_off, const float* ve,
+                    const long long* ve_off, float ve_max, const float* delta,
+                    const float* rho, int nIter, int nCGD, double* hist,
+                    hipStream_t stream);
+
+/* Gridded reconstruction (bmNasha.m:6-91): n2u with Gn, centred ifft times
+ * prod(N_u)*prod(dK_u), times K, crop to n_u; with C != NULL the coil
+ * combine sum(conj(C) x)/sum|C|^2 (bmCoilSense_pinv.m:6-12), output
+ * [prod(n_u)]; without C the per-coil images [prod(n_u)][nCh]. */
+int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x,
+               hipStream_t stream);
+
+/* Kernel timing: when enabled, every gridding launch (forward Gu product,
+ * adjoint Gut product) records HIP events on its stream; the accumulated
+ * launch count and total milliseconds per kind are read with
+ * mona_timing_read (kind 0 = u2n, 1 = n2u, 2 = fft pass, 3 = elementwise). */
+int mona_timing_enable(int on);
+int mona_timing_read(int kind, long long* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MONA_H */

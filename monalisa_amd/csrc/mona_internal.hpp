@@ -1,0 +1,157 @@
+// Internal helpers shared by the HIP translation units of libmona.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mona.h"
+
+namespace mona {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define MONA_HIP(expr)                                                              \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess)                                                       \
+            throw ::mona::Error{MONA_ERR_HIP, std::string(#expr) + ": " +          \
+                                                  hipGetErrorString(_e)};           \
+    } while (0)
+
+#define MONA_CHECK(cond, code, msg)                                                 \
+    do {                                                                            \
+        if (!(cond)) throw ::mona::Error{(code), (msg)};                            \
+    } while (0)
+
+// Runs f(); converts exceptions to a status code and records the message.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return MONA_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return MONA_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return MONA_ERR_INTERNAL;
+    }
+}
+
+// The reference's void entry points have no status; a failure there prints
+// the reason and aborts the process rather than returning garbage.
+void fatal(const char* where, int code);
+
+// ---------------------------------------------------------------- memory
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error{MONA_ERR_OOM, "hipMalloc of " + std::to_string(n) + " bytes failed"};
+        }
+        bytes = n;
+    }
+    void ensure(size_t n) { if (n > bytes) alloc(n); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-process context for the synchronous host-pointer entry points:
+// one stream and grow-only staging buffers, guarded by a mutex.
+struct HostCallCtx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DevBuf slot[12];
+    void init();
+};
+HostCallCtx& host_ctx();
+
+// ---------------------------------------------------------------- launch
+inline unsigned blocks_for(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > 0x7fffffff) b = 0x7fffffff;
+    return (unsigned)b;
+}
+
+// Device scans (scan.hip).  All run on `s`, need no host synchronisation.
+void scan_i32_to_i32_inclusive(const int* in, int* out, int64_t n, hipStream_t s);
+void scan_i32_to_i64_exclusive(const int* in, int64_t* out, int64_t n, hipStream_t s);  // out has n+1
+void scan_i32_to_i64_inclusive(const int* in, int64_t* out, int64_t n, hipStream_t s);
+void fill_f32(float* p, float v, int64_t n, hipStream_t s);
+void fill_i32(int* p, int v, int64_t n, hipStream_t s);
+
+// ---------------------------------------------------------------- sparse
+// Device-resident, decoded form of a cpp_prepared one-block bmSparseMat.
+struct SparseDev {
+    int r_size = 0, l_size = 0, l_nJump = 0;
+    int64_t nnz = 0;
+    bool squeezed = false;
+    int max_row = 0;          // longest stored row
+    DevBuf col;               // int32[nnz]   absolute 0-based column
+    DevBuf val;               // float[nnz]
+    DevBuf rowptr;            // int64[l_nJump+1]
+    DevBuf rowidx;            // int32[l_nJump] output row (only if squeezed)
+    // host copy of the reference encoding (kept for export; decoded onto the
+    // device on first use)
+    std::vector<int> h_r_jump, h_r_nJump, h_l_jump;
+    std::vector<float> h_m_val;
+    bool on_device = false;
+    std::mutex mu;
+};
+
+// Validates a jump-encoded matrix like bmSparseMat.check and keeps the host
+// copy; no device work.
+void sparse_set_host(SparseDev& S, int r_size, const int* r_jump, const int* r_nJump,
+                     const float* m_val, int l_size, const int* l_jump, int l_nJump, int64_t nnz);
+void sparse_set_host(SparseDev& S, int r_size, std::vector<int>&& r_jump, std::vector<int>&& r_nJump,
+                     std::vector<float>&& m_val, int l_size, std::vector<int>&& l_jump);
+// Uploads and decodes on the current device (idempotent, thread-safe).
+void sparse_upload(SparseDev& S, hipStream_t s);
+
+// w = S v for complex vectors stored interleaved (float2) with `n_vec`
+// vectors, entry (row, k) at base[row * n_vec + k] (vector index fastest).
+// Rows of w not stored in S are set to 0.  Sequential row sums in stored
+// order, no FMA: bit-identical to the reference CPU kernel.
+void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
+                      const float* yscale, hipStream_t s);
+
+}  // namespace mona
+
+struct mona_sparse_s {
+    mona::SparseDev S;
+};

@@ -1,0 +1,651 @@
+// Reconstruction plan, deapodisation, the forward/adjoint operators and the
+// CG-SENSE / ADMM-TV solvers.
+//
+//   forward  y = Gu * cfft(zerofill(x .* KFC))          bmShanna_MATLAB.m:6-56
+//   adjoint  x = sum_c KFC_conj .* crop(cifft(Gut * y))  bmNakatsha_MATLAB.m:6-60
+//
+// All vectors stay in HBM; the CG scalars live in a small device struct and
+// are computed in FP64 by single-workgroup kernels, so a solver iteration
+// never waits on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fft.hpp"
+#include "mona_internal.hpp"
+#include "recon.hpp"
+
+namespace mona {
+
+// ================================================================ timing
+namespace {
+struct TimingRec {
+    int kind;
+    hipEvent_t a, b;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<TimingRec> g_trecs;
+double g_tms[4] = {0, 0, 0, 0};
+long long g_tn[4] = {0, 0, 0, 0};
+
+void timing_flush_locked() {
+    for (auto& r : g_trecs) {
+        float ms = 0;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            g_tms[r.kind] += ms;
+            g_tn[r.kind] += 1;
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_trecs.clear();
+}
+}  // namespace
+
+TimedRegion::TimedRegion(int kind, hipStream_t s) : kind_(kind), s_(s) {
+    if (!g_timing) return;
+    if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) return;
+    (void)hipEventRecord(a_, s_);
+    on_ = true;
+}
+TimedRegion::~TimedRegion() {
+    if (!on_) return;
+    (void)hipEventRecord(b_, s_);
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_trecs.push_back({kind_, a_, b_});
+    if (g_trecs.size() > 4096) timing_flush_locked();
+}
+
+// ================================================================ kernels
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Deapodisation on the image grid n_u (bmK.m:29-151 followed by the crop of
+// bmKF.m:16): K(k) = | sum_x w(|x|) cos(2 pi sum_d x_d k_d / N_d) | / sum_x w
+// over the window points x (half-integer spacing, per-axis box |x_d| <= h),
+// then min(1/K, K_max).  The maximum of |Re DFT| over the grid is its value
+// at k = 0, sum_x w, because every w >= 0.
+__global__ void deapod_kernel(float* K, int imDim, int n0, int n1, int n2, int N0, int N1, int N2,
+                              const double* wpts, int npts, double wsum, double Kmax) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)n0 * n1 * n2;
+    if (i >= tot) return;
+    int ix = (int)(i % n0);
+    int iy = (int)((i / n0) % n1);
+    int iz = (int)(i / ((int64_t)n0 * n1));
+    double k0 = ix - n0 / 2, k1 = iy - n1 / 2, k2 = iz - n2 / 2;
+    double acc = 0.0;
+    for (int p = 0; p < npts; p++) {
+        const double* q = wpts + 4 * p;
+        double ph = q[0] * k0 / N0;
+        if (imDim > 1) ph += q[1] * k1 / N1;
+        if (imDim > 2) ph += q[2] * k2 / N2;
+        acc += q[3] * cospi(2.0 * ph);
+    }
+    double v = fabs(acc) / wsum;
+    double r = 1.0 / v;
+    K[i] = (float)(r < Kmax ? r : Kmax);
+}
+
+// KFC = (K .* F) .* C and KFC_conj = (K .* F_conj) .* conj(C) in single
+// precision (bmKF.m:33 with F = single(1/prod(N_u)/prod(dK_u)); bmKF_conj.m:36).
+__global__ void kfc_kernel(const float* K, const float2* C, float2* KFC, float2* KFCc, int64_t nu,
+                           int nCh, float F, float Fc) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nu * nCh) return;
+    int64_t i = t / nCh;
+    float2 c = C[t];
+    float kf = K[i] * F, kfc = K[i] * Fc;
+    KFC[t] = make_float2(kf * c.x, kf * c.y);
+    KFCc[t] = make_float2(kfc * c.x, kfc * -c.y);
+}
+
+// Forward prologue: grid[g][c] = x[i] * KFC[i][c] inside the centred n_u
+// window (bmImZeroFill.m:41-51), 0 elsewhere.
+__global__ void prologue_kernel(const float2* __restrict__ x, const float2* __restrict__ KFC,
+                                float2* __restrict__ grid, Geom g) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = g.Nu * g.nCh;
+    if (t >= tot) return;
+    int c = (int)(t % g.nCh);
+    int64_t cell = t / g.nCh;
+    int gx = (int)(cell % g.N[0]);
+    int gy = (int)((cell / g.N[0]) % g.N[1]);
+    int gz = (int)(cell / ((int64_t)g.N[0] * g.N[1]));
+    int ix = gx - g.off[0], iy = gy - g.off[1], iz = gz - g.off[2];
+    float2 v = make_float2(0.f, 0.f);
+    if (ix >= 0 && ix < g.n[0] && iy >= 0 && iy < g.n[1] && iz >= 0 && iz < g.n[2]) {
+        int64_t i = ix + (int64_t)g.n[0] * (iy + (int64_t)g.n[1] * iz);
+        v = cmulf(x[i], KFC[i * g.nCh + c]);
+    }
+    grid[t] = v;
+}
+
+// Adjoint epilogue: x[i] = scale * sum_c KFC_conj[i][c] * grid[crop(i)][c]
+// (bmImCrope, then .*KFC_conj and sum over coils, bmNakatsha_MATLAB.m:46-55);
+// with combine == 0 the per-coil products are written instead.
+__global__ void epilogue_kernel(const float2* __restrict__ grid, const float2* __restrict__ KFCc,
+                                float2* __restrict__ x, Geom g, int combine, float scale) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nu) return;
+    int ix = (int)(i % g.n[0]);
+    int iy = (int)((i / g.n[0]) % g.n[1]);
+    int iz = (int)(i / ((int64_t)g.n[0] * g.n[1]));
+    int64_t cell = (ix + g.off[0]) + (int64_t)g.N[0] * ((iy + g.off[1]) + (int64_t)g.N[1] * (iz + g.off[2]));
+    const float2* gp = grid + cell * g.nCh;
+    const float2* kp = KFCc + i * g.nCh;
+    if (combine) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int c = 0; c < g.nCh; c++) {
+            float2 p = cmulf(gp[c], kp[c]);
+            acc.x += p.x;
+            acc.y += p.y;
+        }
+        x[i] = make_float2(acc.x * scale, acc.y * scale);
+    } else {
+        for (int c = 0; c < g.nCh; c++) {
+            float2 p = cmulf(gp[c], kp[c]);
+            x[i * g.nCh + c] = make_float2(p.x * scale, p.y * scale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- vectors
+__global__ void k_sub(const float2* a, const float2* b, float2* out, int64_t n) {  // out = a - b
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { float2 x = a[i], y = b[i]; out[i] = make_float2(x.x - y.x, x.y - y.y); }
+}
+__global__ void k_copy(const float2* a, float2* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i];
+}
+// y += coef * x where coef = sign * scalar (device double, rounded to float)
+__global__ void k_axpy_dev(float2* y, const float2* x, const double* scal, float sign, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float a = sign * (float)(*scal);
+        float2 xv = x[i], yv = y[i];
+        yv.x = yv.x + a * xv.x;
+        yv.y = yv.y + a * xv.y;
+        y[i] = yv;
+    }
+}
+// p = d + b * p
+__global__ void k_xpby_dev(float2* p, const float2* d, const double* scal, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float b = (float)(*scal);
+        float2 pv = p[i], dv = d[i];
+        p[i] = make_float2(dv.x + b * pv.x, dv.y + b * pv.y);
+    }
+}
+__global__ void k_hy(const float* ve, float vmax, float* hy, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hy[i] = fminf(ve[i], vmax);
+}
+
+// FP64 weighted squared norm sum_i w(i) |a_i|^2 (w = per-point weight for
+// [n/nvec][nvec] data, or a constant).  Fixed launch shape and fixed
+// reduction tree: deterministic.
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
+
+__global__ void __launch_bounds__(kRedThreads)
+k_sqnorm_partial(const float2* a, int64_t n, const float* w, int nvec, double cw, double* partial) {
+    __shared__ double sh[kRedThreads];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedThreads) {
+        float2 v = a[i];
+        double m = (double)v.x * v.x + (double)v.y * v.y;
+        if (w) m *= (double)w[i / nvec];
+        s += m;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] * cw;
+}
+
+// out[slot] (= or +=) sum of partials
+__global__ void __launch_bounds__(kRedThreads)
+k_reduce_final(const double* partial, int np, double* out, int accumulate) {
+    __shared__ double sh[kRedThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += kRedThreads) s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = accumulate ? (*out + sh[0]) : sh[0];
+}
+
+// CG scalar updates (bmSensa.m:54-75):  at the top of an iteration the
+// sticky `done` flag is raised when sqn < eps; a = done ? 0 : sqn / den.
+__global__ void k_cg_alpha(CgScalars* s, double eps) {
+    if (!s->done && s->sqn < eps) s->done = 1;
+    s->a = s->done ? 0.0 : s->sqn / s->den;
+}
+__global__ void k_cg_beta(CgScalars* s) {
+    s->b = s->done ? 0.0 : s->sqn_new / s->sqn;
+    if (!s->done) s->sqn = s->sqn_new;
+}
+__global__ void k_cg_reset(CgScalars* s) {
+    s->done = 0;
+    s->a = s->b = 0.0;
+}
+__global__ void k_store(double* dst, const double* src) { *dst = *src; }
+
+// ================================================================ host side
+
+void Plan::launch_check() { MONA_HIP(hipGetLastError()); }
+
+Geom Plan::geom() const {
+    Geom g{};
+    for (int d = 0; d < 3; d++) {
+        g.N[d] = N[d];
+        g.n[d] = n[d];
+        g.off[d] = N[d] / 2 - n[d] / 2;
+    }
+    g.Nu = Nu;
+    g.nu = nu;
+    g.nCh = nCh;
+    return g;
+}
+
+void Plan::init(int imDim_, const int* N_, const int* n_, const double* dK_, int nCh_, int ktype,
+                int nWin_, const double* kp, int nParam) {
+    MONA_CHECK(imDim_ >= 1 && imDim_ <= 3, MONA_ERR_ARG, "imDim must be 1, 2 or 3");
+    MONA_CHECK(nCh_ >= 1, MONA_ERR_ARG, "nCh must be >= 1");
+    MONA_CHECK((ktype == MONA_KERNEL_GAUSS && nParam == 2) || (ktype == MONA_KERNEL_KAISER && nParam == 3),
+               MONA_ERR_ARG, "Wrong list of gridding kernel parameters.");
+    MONA_CHECK(nWin_ >= 1 && nWin_ <= 9, MONA_ERR_ARG, "nWin must be in [1, 9]");
+    imDim = imDim_;
+    nCh = nCh_;
+    ktype_ = ktype;
+    nWin = nWin_;
+    for (int i = 0; i < 3; i++) kparam[i] = i < nParam ? (double)(float)kp[i] : 0.0;
+    Nu = 1;
+    nu = 1;
+    HX = 1.0;
+    for (int d = 0; d < 3; d++) {
+        if (d < imDim) {
+            N[d] = N_[d];
+            n[d] = n_ ? n_[d] : N_[d];
+            dK[d] = (double)(float)dK_[d];
+            MONA_CHECK(N[d] % 2 == 0, MONA_ERR_ARG, "N_u must have all components even for the Fourier transform. ");
+            MONA_CHECK(n[d] % 2 == 0, MONA_ERR_ARG, "n_u must have all components even for the Fourier transform. ");
+            MONA_CHECK(n[d] >= 2 && n[d] <= N[d], MONA_ERR_ARG, "n_u must be in [2, N_u]");
+            MONA_CHECK(fft_size_supported(N[d]), MONA_ERR_ARG, "N_u component is not a supported FFT length");
+            MONA_CHECK(dK[d] > 0, MONA_ERR_ARG, "dK_u must be > 0");
+            // HX = prod(dX_u), dX_u = single((1/single(dK_u))/single(N_u))  (bmSensa.m:26-27)
+            float dx = (1.0f / (float)dK[d]) / (float)N[d];
+            HX = (double)((float)HX * dx);
+        } else {
+            N[d] = 1;
+            n[d] = 1;
+            dK[d] = 1.0;
+        }
+        Nu *= N[d];
+        nu *= n[d];
+    }
+}
+
+void Plan::ensure_grid() {
+    grid.ensure((size_t)Nu * nCh * sizeof(float2));
+}
+
+static inline unsigned nb256(int64_t n) { return blocks_for(n, 256); }
+
+void Plan::set_coils(const float2* C, hipStream_t s) {
+    // window points of bmK (half-integer grid, per-axis box |x| <= nWin/2)
+    int h = nWin / 2;
+    std::vector<double> pts;
+    double wsum = 0.0;
+    double I0a = 1.0;
+    auto i0 = [](double x) {
+        double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+        for (int k = 1; k < 500; k++) {
+            term *= q / ((double)k * k);
+            sum += term;
+            if (term < sum * 1e-17) break;
+        }
+        return sum;
+    };
+    if (ktype_ == MONA_KERNEL_KAISER) I0a = i0(kparam[1]);
+    auto weight = [&](double d) {
+        if (ktype_ == MONA_KERNEL_GAUSS) {
+            double r = d / kparam[0];
+            return std::exp(-0.5 * r * r) / (std::sqrt(2.0 * M_PI) * kparam[0]);
+        }
+        double r = d / kparam[0];
+        double w = 1.0 - r * r;
+        if (w < 0) w = 0;
+        return i0(kparam[1] * std::sqrt(w)) / I0a;
+    };
+    int span = 4 * h + 1;  // x in {-h, -h+1/2, ..., h}
+    int sy = imDim > 1 ? span : 1, sz = imDim > 2 ? span : 1;
+    for (int c = 0; c < sz; c++)
+        for (int b = 0; b < sy; b++)
+            for (int a = 0; a < span; a++) {
+                double x = -h + 0.5 * a, y = imDim > 1 ? -h + 0.5 * b : 0.0, z = imDim > 2 ? -h + 0.5 * c : 0.0;
+                if (imDim == 1 && !(x >= h && x <= h)) continue;  // bmK.m:88 (1-D mask x < h | x > h)
+                double w = weight(std::sqrt(x * x + y * y + z * z));
+                pts.insert(pts.end(), {x, y, z, w});
+                wsum += w;
+            }
+    int npts = (int)(pts.size() / 4);
+    DevBuf dpts(pts.size() * sizeof(double));
+    MONA_HIP(hipMemcpyAsync(dpts.p, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    K.ensure((size_t)nu * sizeof(float));
+    deapod_kernel<<<nb256(nu), 256, 0, s>>>(K.as<float>(), imDim, n[0], n[1], n[2], N[0], N[1], N[2],
+                                            dpts.as<double>(), npts, wsum, kparam[imDim == 0 ? 0 : (ktype_ == MONA_KERNEL_GAUSS ? 1 : 2)]);
+    launch_check();
+    float F = 1.0f, Fc = 1.0f;
+    {
+        float pN = 1.0f, pdK = 1.0f;
+        for (int d = 0; d < imDim; d++) { pN *= (float)N[d]; pdK *= (float)dK[d]; }
+        F = (float)(1.0 / (double)pN / (double)pdK);
+        Fc = (float)(1.0 / (double)pdK);
+    }
+    KFC.ensure((size_t)nu * nCh * sizeof(float2));
+    KFCc.ensure((size_t)nu * nCh * sizeof(float2));
+    kfc_kernel<<<nb256(nu * nCh), 256, 0, s>>>(K.as<float>(), C, KFC.as<float2>(), KFCc.as<float2>(), nu,
+                                                nCh, F, Fc);
+    launch_check();
+    MONA_HIP(hipStreamSynchronize(s));  // dpts goes out of scope
+    coils_set = true;
+}
+
+const Frame& Plan::frame(int f) const {
+    MONA_CHECK(f >= 0 && f < (int)frames.size() && frames[f].Gu && frames[f].Gut, MONA_ERR_STATE,
+               "frame " + std::to_string(f) + " has no gridding matrices");
+    return frames[f];
+}
+
+void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
+    MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
+    const Frame& fr = frame(f);
+    ensure_grid();
+    Geom g = geom();
+    float2* gr = grid.as<float2>();
+    {
+        TimedRegion t(3, s);
+        prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
+        launch_check();
+    }
+    {
+        TimedRegion t(2, s);
+        cfft_nd(gr, imDim, N, nCh, false, s);
+    }
+    {
+        TimedRegion t(0, s);
+        spmv_c2_rowmajor(*fr.Gu, gr, y, nCh, nullptr, s);
+    }
+}
+
+void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float* hy, float scale,
+                   hipStream_t s) {
+    MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
+    const Frame& fr = frame(f);
+    ensure_grid();
+    Geom g = geom();
+    float2* gr = grid.as<float2>();
+    {
+        TimedRegion t(1, s);
+        spmv_c2_rowmajor(*fr.Gut, y, gr, nCh, hy, s);
+    }
+    {
+        TimedRegion t(2, s);
+        cfft_nd(gr, imDim, N, nCh, true, s);
+    }
+    {
+        TimedRegion t(3, s);
+        epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
+        launch_check();
+    }
+}
+
+// ---------------------------------------------------------------- vector helpers
+void vsub(const float2* a, const float2* b, float2* out, int64_t n, hipStream_t s) {
+    k_sub<<<nb256(n), 256, 0, s>>>(a, b, out, n);
+    MONA_HIP(hipGetLastError());
+}
+void vcopy(const float2* a, float2* out, int64_t n, hipStream_t s) {
+    k_copy<<<nb256(n), 256, 0, s>>>(a, out, n);
+    MONA_HIP(hipGetLastError());
+}
+void vaxpy(float2* y, const float2* x, const double* a, float sign, int64_t n, hipStream_t s) {
+    k_axpy_dev<<<nb256(n), 256, 0, s>>>(y, x, a, sign, n);
+    MONA_HIP(hipGetLastError());
+}
+void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s) {
+    k_xpby_dev<<<nb256(n), 256, 0, s>>>(p, d, b, n);
+    MONA_HIP(hipGetLastError());
+}
+void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
+            double* partial, hipStream_t s) {
+    int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+    k_sqnorm_partial<<<nb, kRedThreads, 0, s>>>(a, n, w, nvec, cw, partial);
+    MONA_HIP(hipGetLastError());
+    k_reduce_final<<<1, kRedThreads, 0, s>>>(partial, nb, out, acc ? 1 : 0);
+    MONA_HIP(hipGetLastError());
+}
+void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s) {
+    k_hy<<<nb256(n), 256, 0, s>>>(ve, vmax, hy, n);
+    MONA_HIP(hipGetLastError());
+}
+void cg_alpha(CgScalars* sc, hipStream_t s) {
+    k_cg_alpha<<<1, 1, 0, s>>>(sc, 10.0 * 1.1920928955078125e-07);  // 10*eps('single')
+    MONA_HIP(hipGetLastError());
+}
+void cg_beta(CgScalars* sc, hipStream_t s) {
+    k_cg_beta<<<1, 1, 0, s>>>(sc);
+    MONA_HIP(hipGetLastError());
+}
+void cg_reset(CgScalars* sc, hipStream_t s) {
+    k_cg_reset<<<1, 1, 0, s>>>(sc);
+    MONA_HIP(hipGetLastError());
+}
+void dstore(double* dst, const double* src, hipStream_t s) {
+    k_store<<<1, 1, 0, s>>>(dst, src);
+    MONA_HIP(hipGetLastError());
+}
+
+// max of a float array on the host side (ve_max default = max(ve))
+float device_max(const float* v, int64_t n, hipStream_t s) {
+    std::vector<float> h(n);
+    MONA_HIP(hipMemcpyAsync(h.data(), v, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    MONA_HIP(hipStreamSynchronize(s));
+    float m = -INFINITY;
+    for (float x : h) m = std::max(m, x);
+    return m;
+}
+
+// ================================================================ CG-SENSE
+void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
+                 double* residual, hipStream_t s) {
+    MONA_CHECK(nIter >= 0 && nCGD >= 0, MONA_ERR_ARG, "nIter and nCGD must be >= 0");
+    const Frame& fr = frame(f);
+    int64_t P = fr.Gu->l_size;
+    int64_t ny = P * nCh;
+    MONA_CHECK(fr.Gut->r_size == P, MONA_ERR_ARG, "Gut does not match Gu");
+    if (!(ve_max > 0)) ve_max = device_max(ve, P, s);
+    Work& w = work;
+    w.hy.ensure((size_t)P * sizeof(float));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.i1.ensure((size_t)nu * sizeof(float2));
+    w.i2.ensure((size_t)nu * sizeof(float2));
+    w.sc.ensure(sizeof(CgScalars));
+    w.partial.ensure(kRedBlocks * sizeof(double));
+    w.hist.ensure((size_t)std::max(nIter, 1) * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2* res = w.k1.as<float2>();
+    float2* Mp = w.k2.as<float2>();
+    float2* dag = w.i1.as<float2>();
+    float2* p = w.i2.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    compute_hy(ve, ve_max, hy, P, s);
+    const float invHX = (float)(1.0 / HX);
+
+    for (int it = 0; it < nIter; it++) {
+        cg_reset(sc, s);
+        forward(f, x, Mp, s);
+        vsub(y, Mp, res, ny, s);                             // res = y - M x
+        adjoint(f, res, dag, true, hy, invHX, s);            // dag = (1/HX) M^H (HY .* res)
+        sqnorm(dag, nu, nullptr, 1, HX, &sc->sqn, false, part, s);
+        vcopy(dag, p, nu, s);
+        for (int i = 1; i <= nCGD; i++) {
+            forward(f, p, Mp, s);                            // Mp = M p
+            sqnorm(Mp, ny, hy, nCh, 1.0, &sc->den, false, part, s);
+            cg_alpha(sc, s);                                 // done check, a
+            vaxpy(x, p, &sc->a, 1.0f, nu, s);                // x += a p
+            if (i == nCGD) break;
+            vaxpy(res, Mp, &sc->a, -1.0f, ny, s);            // res -= a Mp
+            adjoint(f, res, dag, true, hy, invHX, s);
+            sqnorm(dag, nu, nullptr, 1, HX, &sc->sqn_new, false, part, s);
+            cg_beta(sc, s);
+            vxpby(p, dag, &sc->b, nu, s);                    // p = dag + b p
+        }
+        // monitoring residual <r, HY r>, r = y - M x   (bmSensa.m:82-84)
+        forward(f, x, Mp, s);
+        vsub(y, Mp, res, ny, s);
+        sqnorm(res, ny, hy, nCh, 1.0, w.hist.as<double>() + it, false, part, s);
+    }
+    if (residual && nIter > 0) {
+        MONA_HIP(hipMemcpyAsync(residual, w.hist.p, nIter * sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace mona
+
+using namespace mona;
+
+struct mona_plan_s {
+    Plan P;
+};
+
+extern "C" {
+
+int mona_plan_create(int imDim, const int* N_u, const int* n_u, const double* dK_u, int nCh,
+                     int kernelType, int nWin, const double* kernelParam, int nParam, mona_plan* out) {
+    return guarded([&] {
+        MONA_CHECK(out && N_u && dK_u && kernelParam, MONA_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        auto* h = new mona_plan_s;
+        try {
+            h->P.init(imDim, N_u, n_u, dK_u, nCh, kernelType, nWin, kernelParam, nParam);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int mona_plan_destroy(mona_plan P) {
+    return guarded([&] { delete P; });
+}
+
+int mona_plan_set_coils(mona_plan P, const void* C, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && C, MONA_ERR_ARG, "NULL argument");
+        P->P.set_coils(static_cast<const float2*>(C), stream);
+    });
+}
+
+int mona_plan_set_frame(mona_plan P, int frame, mona_sparse Gu, mona_sparse Gut) {
+    return guarded([&] {
+        MONA_CHECK(P && Gu && Gut && frame >= 0, MONA_ERR_ARG, "bad argument");
+        const SparseDev& u = Gu->S;
+        const SparseDev& ut = Gut->S;
+        MONA_CHECK(u.r_size == P->P.Nu, MONA_ERR_ARG, "Gu has r_size != prod(N_u)");
+        MONA_CHECK(ut.l_size == P->P.Nu, MONA_ERR_ARG, "Gut has l_size != prod(N_u)");
+        MONA_CHECK(ut.r_size == u.l_size, MONA_ERR_ARG, "Gut is not the transpose shape of Gu");
+        HostCallCtx& C = host_ctx();
+        {
+            std::lock_guard<std::mutex> lk(C.mu);
+            C.init();
+        }
+        sparse_upload(Gu->S, C.stream);
+        sparse_upload(Gut->S, C.stream);
+        if ((int)P->P.frames.size() <= frame) P->P.frames.resize(frame + 1);
+        P->P.frames[frame].Gu = &Gu->S;
+        P->P.frames[frame].Gut = &Gut->S;
+    });
+}
+
+int mona_plan_get_K(mona_plan P, float* K, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && K && P->P.coils_set, MONA_ERR_STATE, "coil maps not set");
+        MONA_HIP(hipMemcpyAsync(K, P->P.K.p, P->P.nu * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    });
+}
+
+int mona_plan_get_KFC(mona_plan P, void* KFC, void* KFCc, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && P->P.coils_set, MONA_ERR_STATE, "coil maps not set");
+        size_t b = (size_t)P->P.nu * P->P.nCh * sizeof(float2);
+        if (KFC) MONA_HIP(hipMemcpyAsync(KFC, P->P.KFC.p, b, hipMemcpyDeviceToDevice, stream));
+        if (KFCc) MONA_HIP(hipMemcpyAsync(KFCc, P->P.KFCc.p, b, hipMemcpyDeviceToDevice, stream));
+    });
+}
+
+int mona_shanna(mona_plan P, int frame, const void* x, void* y, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
+        P->P.forward(frame, static_cast<const float2*>(x), static_cast<float2*>(y), stream);
+    });
+}
+
+int mona_nakatsha(mona_plan P, int frame, const void* y, void* x, int C_flag, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
+        P->P.adjoint(frame, static_cast<const float2*>(y), static_cast<float2*>(x), C_flag != 0, nullptr,
+                     1.0f, stream);
+    });
+}
+
+int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, float ve_max, int nIter,
+               int nCGD, double* residual, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
+        P->P.sensa(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nIter, nCGD,
+                   residual, stream);
+    });
+}
+
+int mona_timing_enable(int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        if (!on) timing_flush_locked();
+        g_timing = on != 0;
+        if (on) {
+            for (int k = 0; k < 4; k++) { g_tms[k] = 0; g_tn[k] = 0; }
+        }
+    });
+}
+
+int mona_timing_read(int kind, long long* launches, double* total_ms) {
+    return guarded([&] {
+        MONA_CHECK(kind >= 0 && kind < 4, MONA_ERR_ARG, "kind must be 0..3");
+        std::lock_guard<std::mutex> lk(g_tmu);
+        timing_flush_locked();
+        if (launches) *launches = g_tn[kind];
+        if (total_ms) *total_ms = g_tms[kind];
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Reconstruction plan and operator declarations (recon.hip, admm.hip).
+#pragma once
+#include <vector>
+
+#include "mona_internal.hpp"
+
+namespace mona {
+
+// Geometry passed by value to the prologue / epilogue kernels.
+struct Geom {
+    int N[3], n[3], off[3];
+    int64_t Nu, nu;
+    int nCh;
+};
+
+// Device-resident CG scalars (FP64).
+struct CgScalars {
+    double sqn, sqn_new, den, a, b;
+    double den2;  // second term of the ADMM denominator
+    int done, pad;
+};
+
+struct Frame {
+    const SparseDev* Gu = nullptr;
+    const SparseDev* Gut = nullptr;
+};
+
+// Records HIP events around a region when kernel timing is enabled.
+class TimedRegion {
+  public:
+    TimedRegion(int kind, hipStream_t s);
+    ~TimedRegion();
+  private:
+    int kind_;
+    hipStream_t s_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+    bool on_ = false;
+};
+
+struct Work {
+    DevBuf hy, k1, k2, i1, i2, sc, partial, hist;
+    DevBuf z1, z2, z3, z4, z5;  // ADMM image-space buffers
+};
+
+struct Plan {
+    int imDim = 0, nCh = 0, nWin = 3, ktype_ = 0;
+    int N[3] = {1, 1, 1}, n[3] = {1, 1, 1};
+    double dK[3] = {1, 1, 1};
+    double kparam[3] = {0, 0, 0};
+    int64_t Nu = 1, nu = 1;
+    double HX = 1.0;
+    bool coils_set = false;
+    DevBuf K, KFC, KFCc, grid;
+    std::vector<Frame> frames;
+    Work work;
+
+    void init(int imDim, const int* N, const int* n, const double* dK, int nCh, int ktype, int nWin,
+              const double* kp, int nParam);
+    void set_coils(const float2* C, hipStream_t s);
+    Geom geom() const;
+    const Frame& frame(int f) const;
+    void ensure_grid();
+    void launch_check();
+    float kmax() const { return (float)kparam[ktype_ == MONA_KERNEL_GAUSS ? 1 : 2]; }
+
+    // y = M x  (x [nu], y [P][nCh])
+    void forward(int f, const float2* x, float2* y, hipStream_t s);
+    // x = scale * sum_c KFC_conj .* crop(cifft(Gut (hy .* y)))   (hy may be NULL)
+    void adjoint(int f, const float2* y, float2* x, bool combine, const float* hy, float scale,
+                 hipStream_t s);
+    void sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
+               double* residual, hipStream_t s);
+};
+
+// vector helpers (recon.hip)
+void vsub(const float2* a, const float2* b, float2* out, int64_t n, hipStream_t s);
+void vcopy(const float2* a, float2* out, int64_t n, hipStream_t s);
+void vaxpy(float2* y, const float2* x, const double* a, float sign, int64_t n, hipStream_t s);
+void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s);
+void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
+            double* partial, hipStream_t s);
+void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s);
+void cg_alpha(CgScalars* sc, hipStream_t s);
+void cg_beta(CgScalars* sc, hipStream_t s);
+void cg_reset(CgScalars* sc, hipStream_t s);
+void dstore(double* dst, const double* src, hipStream_t s);
+float device_max(const float* v, int64_t n, hipStream_t s);
+
+}  // namespace mona

@@ -1,0 +1,616 @@
+// bmSparseMat on the device: decode of the jump encoding, the products
+// W = S V in every layout the reference's mat-vec MEX family supports, and the
+// drop-in host-pointer entry points.
+//
+// Each output element is one work-item that walks its row's entries in
+// stored order and accumulates `acc = acc + m * v` with a separately rounded
+// multiply and add (FMA contraction is off for this file): the same float32
+// operation sequence as the reference CPU kernels
+// (src/sparseMat/cpp/bmSparseMat_cC_oBlock_omp.cpp:99-110), hence
+// bit-identical results with no atomics.
+#include <algorithm>
+#include <cstring>
+
+#include "mona_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mona {
+
+// -------------------------------------------------------------- context
+static thread_local std::string g_err;
+static std::mutex g_err_mu;
+static std::string g_err_global;
+
+void set_error(const std::string& msg) {
+    g_err = msg;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err_global = msg;
+}
+const char* last_error() {
+    if (!g_err.empty()) return g_err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err_global.c_str();
+}
+
+void fatal(const char* where, int code) {
+    std::fprintf(stderr, "libmona: %s failed (status %d): %s\n", where, code, last_error());
+    std::fflush(stderr);
+    std::abort();
+}
+
+HostCallCtx& host_ctx() {
+    static HostCallCtx ctx;
+    return ctx;
+}
+
+void HostCallCtx::init() {
+    if (!stream) {
+        int n = 0;
+        MONA_CHECK(hipGetDeviceCount(&n) == hipSuccess && n > 0, MONA_ERR_HIP,
+                   "no HIP device is visible");
+        MONA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    }
+}
+
+// -------------------------------------------------------------- kernels
+
+// Validates decoded columns / rows are in range; sets *bad on violation.
+__global__ void check_range_i32(const int* a, int64_t n, int lo, int hi, int* bad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        if (a[i] < lo || a[i] >= hi) { *bad = 1; return; }
+}
+
+// Strictly increasing output rows (squeezed matrices).
+__global__ void check_increasing_i32(const int* a, int64_t n, int* bad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        if (a[i] <= a[i - 1]) { *bad = 1; return; }
+}
+
+// Multi-block decode: col[k] = P[k] - P[block start of k - 1] where P is the
+// inclusive int64 prefix of r_jump; the first entry of each block is absolute.
+__global__ void rebase_blocks(const int64_t* P, int64_t nnz, const int64_t* mstart, int nBlock,
+                              int* col) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; k < nnz; k += stride) {
+        int lo = 0, hi = nBlock - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mstart[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        int64_t base = mstart[lo] > 0 ? P[mstart[lo] - 1] : 0;
+        col[k] = (int)(P[k] - base);
+    }
+}
+
+// Generic real-plane product.  Plane p = (vector k, component c): v element
+// (col) at vc[c] + k*v_vs + col*v_cs; w element (row) at wc[c] + k*w_vs +
+// row*w_rs.  ROW_FAST: consecutive work-items take consecutive rows (column
+// layouts); otherwise consecutive planes (row layouts).
+template <bool ROW_FAST>
+__global__ void __launch_bounds__(256)
+spmv_planes(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+            const float* __restrict__ val, const int* __restrict__ rowidx, int nrows, int nplanes,
+            int ncomp, const float* v0, const float* v1, int64_t v_vs, int64_t v_cs, float* w0,
+            float* w1, int64_t w_vs, int64_t w_rs) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)nrows * nplanes;
+    if (t >= total) return;
+    int i, p;
+    if (ROW_FAST) { p = (int)(t / nrows); i = (int)(t - (int64_t)p * nrows); }
+    else { i = (int)(t / nplanes); p = (int)(t - (int64_t)i * nplanes); }
+    int c = p % ncomp, k = p / ncomp;
+    const float* vb = (c ? v1 : v0) + (int64_t)k * v_vs;
+    float acc = 0.0f;
+    for (int64_t j = rowptr[i], e = rowptr[i + 1]; j < e; j++) {
+        float prod = val[j] * vb[(int64_t)col[j] * v_cs];
+        acc = acc + prod;
+    }
+    int row = rowidx ? rowidx[i] : i;
+    (c ? w1 : w0)[(int64_t)k * w_vs + (int64_t)row * w_rs] = acc;
+}
+
+// Complex interleaved rows, vector index fastest: v[col][n_vec] float2.
+// Optional per-column scale s[col] (a k-space weight HY applied to v before
+// the product, rounded exactly as a separate elementwise product would be).
+template <bool SCALE>
+__global__ void __launch_bounds__(256)
+spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+        const float* __restrict__ val, const int* __restrict__ rowidx, int nrows, int n_vec,
+        const float2* __restrict__ v, float2* __restrict__ w, const float* __restrict__ scale) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)nrows * n_vec;
+    if (t >= total) return;
+    int i = (int)(t / n_vec), k = (int)(t - (int64_t)i * n_vec);
+    float ar = 0.0f, ai = 0.0f;
+    for (int64_t j = rowptr[i], e = rowptr[i + 1]; j < e; j++) {
+        int cj = col[j];
+        float m = val[j];
+        float2 x = v[(int64_t)cj * n_vec + k];
+        if (SCALE) { float s = scale[cj]; x.x = s * x.x; x.y = s * x.y; }
+        float pr = m * x.x, pi = m * x.y;
+        ar = ar + pr;
+        ai = ai + pi;
+    }
+    int row = rowidx ? rowidx[i] : i;
+    w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
+}
+
+__global__ void expand_rows(const int64_t* rowptr, int nrows, int* ind) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int64_t j = rowptr[i], e = rowptr[i + 1]; j < e; j++) ind[j] = i;
+}
+
+__global__ void max_row_len(const int64_t* rowptr, int nrows, int* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    atomicMax(out, (int)(rowptr[i + 1] - rowptr[i]));
+}
+
+// -------------------------------------------------------------- decode
+
+static int64_t checked_nnz(const int* r_nJump, int l_nJump) {
+    int64_t s = 0;
+    for (int i = 0; i < l_nJump; i++) {
+        MONA_CHECK(r_nJump[i] >= 0, MONA_ERR_ARG, "r_nJump has a negative entry (check 2.6)");
+        s += r_nJump[i];
+    }
+    return s;
+}
+
+// Decodes into S (device arrays).  Inputs are device pointers.
+static void decode_device(SparseDev& S, const int* d_r_jump, const int* d_r_nJump,
+                          const float* d_m_val, const int* d_l_jump, int nBlock,
+                          const int64_t* d_mstart, hipStream_t s) {
+    S.col.alloc(std::max<int64_t>(S.nnz, 1) * sizeof(int));
+    S.val.alloc(std::max<int64_t>(S.nnz, 1) * sizeof(float));
+    S.rowptr.alloc((size_t)(S.l_nJump + 1) * sizeof(int64_t));
+    if (nBlock <= 1) {
+        scan_i32_to_i32_inclusive(d_r_jump, S.col.as<int>(), S.nnz, s);
+    } else {
+        DevBuf P((size_t)std::max<int64_t>(S.nnz, 1) * sizeof(int64_t));
+        scan_i32_to_i64_inclusive(d_r_jump, P.as<int64_t>(), S.nnz, s);
+        rebase_blocks<<<std::min(blocks_for(S.nnz, 256), 65535u), 256, 0, s>>>(
+            P.as<int64_t>(), S.nnz, d_mstart, nBlock, S.col.as<int>());
+        MONA_HIP(hipGetLastError());
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+    MONA_HIP(hipMemcpyAsync(S.val.p, d_m_val, S.nnz * sizeof(float), hipMemcpyDeviceToDevice, s));
+    scan_i32_to_i64_exclusive(d_r_nJump, S.rowptr.as<int64_t>(), S.l_nJump, s);
+    if (S.squeezed) {
+        S.rowidx.alloc((size_t)S.l_nJump * sizeof(int));
+        if (nBlock <= 1) {
+            scan_i32_to_i32_inclusive(d_l_jump, S.rowidx.as<int>(), S.l_nJump, s);
+        } else {
+            MONA_CHECK(false, MONA_ERR_ARG, "multi-block squeezed matrices are not supported");
+        }
+    }
+    // range checks: every decoded column must address v, every row w
+    DevBuf flag(sizeof(int) * 2);
+    MONA_HIP(hipMemsetAsync(flag.p, 0, sizeof(int) * 2, s));
+    check_range_i32<<<std::min(blocks_for(S.nnz, 256), 4096u), 256, 0, s>>>(
+        S.col.as<int>(), S.nnz, 0, S.r_size, flag.as<int>());
+    if (S.squeezed) {
+        check_range_i32<<<std::min(blocks_for(S.l_nJump, 256), 4096u), 256, 0, s>>>(
+            S.rowidx.as<int>(), S.l_nJump, 0, S.l_size, flag.as<int>() + 1);
+        check_increasing_i32<<<std::min(blocks_for(S.l_nJump, 256), 4096u), 256, 0, s>>>(
+            S.rowidx.as<int>(), S.l_nJump, flag.as<int>() + 1);
+    }
+    DevBuf mr(sizeof(int));
+    MONA_HIP(hipMemsetAsync(mr.p, 0, sizeof(int), s));
+    max_row_len<<<blocks_for(S.l_nJump, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), S.l_nJump,
+                                                           mr.as<int>());
+    MONA_HIP(hipGetLastError());
+    int h[2] = {0, 0};
+    MONA_HIP(hipMemcpyAsync(h, flag.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    MONA_HIP(hipMemcpyAsync(&S.max_row, mr.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MONA_HIP(hipStreamSynchronize(s));
+    MONA_CHECK(!h[0], MONA_ERR_ARG, "r_jump decodes to a column outside [0, r_size)");
+    MONA_CHECK(!h[1], MONA_ERR_ARG,
+               "l_jump decodes to rows outside [0, l_size) or not strictly increasing");
+}
+
+static void basic_checks(int r_size, int l_size, int l_nJump, int64_t nnz, bool squeezed,
+                         bool strict) {
+    // bmSparseMat.check rules (src/sparseMat/m/bmSparseMat.m:222-256, 368-377)
+    MONA_CHECK(r_size >= 0 && l_size >= 0 && l_nJump >= 0, MONA_ERR_ARG, "negative size");
+    if (strict) {
+        MONA_CHECK(l_size >= 2 && r_size >= 2 && l_nJump >= 2, MONA_ERR_ARG,
+                   "l_size, r_size and l_nJump must be >= 2 (check 2.5)");
+        MONA_CHECK(nnz > 0, MONA_ERR_ARG, "the matrix has no entries (check 3.1)");
+    }
+    MONA_CHECK(l_nJump <= l_size || squeezed || !strict, MONA_ERR_ARG, "l_nJump > l_size");
+    MONA_CHECK(!squeezed || l_nJump <= l_size, MONA_ERR_ARG, "l_nJump > l_size");
+    MONA_CHECK(nnz < ((int64_t)1 << 40), MONA_ERR_ARG, "nnz too large");
+}
+
+void sparse_set_host(SparseDev& S, int r_size, std::vector<int>&& r_jump, std::vector<int>&& r_nJump,
+                     std::vector<float>&& m_val, int l_size, std::vector<int>&& l_jump) {
+    int l_nJump = (int)r_nJump.size();
+    int64_t nnz = (int64_t)m_val.size();
+    bool squeezed = !l_jump.empty();
+    int64_t sum = checked_nnz(r_nJump.data(), l_nJump);
+    MONA_CHECK(sum == nnz, MONA_ERR_ARG, "sum(r_nJump) != numel(m_val) (check 3.2)");
+    MONA_CHECK((int64_t)r_jump.size() == nnz, MONA_ERR_ARG, "numel(r_jump) != numel(m_val) (check 9.2)");
+    MONA_CHECK(!squeezed || (int)l_jump.size() == l_nJump, MONA_ERR_ARG,
+               "numel(l_jump) != l_nJump (check 10.2)");
+    basic_checks(r_size, l_size, l_nJump, nnz, squeezed, true);
+    S.r_size = r_size; S.l_size = l_size; S.l_nJump = l_nJump; S.nnz = nnz;
+    S.squeezed = squeezed;
+    S.h_r_jump = std::move(r_jump);
+    S.h_r_nJump = std::move(r_nJump);
+    S.h_m_val = std::move(m_val);
+    S.h_l_jump = std::move(l_jump);
+    S.on_device = false;
+}
+
+void sparse_set_host(SparseDev& S, int r_size, const int* r_jump, const int* r_nJump,
+                     const float* m_val, int l_size, const int* l_jump, int l_nJump, int64_t nnz) {
+    MONA_CHECK(l_nJump >= 0 && nnz >= 0, MONA_ERR_ARG, "negative size");
+    MONA_CHECK((r_jump && m_val) || nnz == 0, MONA_ERR_ARG, "NULL r_jump / m_val");
+    MONA_CHECK(r_nJump || l_nJump == 0, MONA_ERR_ARG, "NULL r_nJump");
+    sparse_set_host(S, r_size, std::vector<int>(r_jump, r_jump + nnz),
+                    std::vector<int>(r_nJump, r_nJump + l_nJump), std::vector<float>(m_val, m_val + nnz),
+                    l_size, l_jump ? std::vector<int>(l_jump, l_jump + l_nJump) : std::vector<int>());
+}
+
+void sparse_upload(SparseDev& S, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(S.mu);
+    if (S.on_device) return;
+    int64_t nnz = S.nnz;
+    int l_nJump = S.l_nJump;
+    DevBuf d_rj(std::max<int64_t>(nnz, 1) * sizeof(int)), d_rn((size_t)std::max(l_nJump, 1) * sizeof(int)),
+        d_mv(std::max<int64_t>(nnz, 1) * sizeof(float)), d_lj;
+    MONA_HIP(hipMemcpyAsync(d_rj.p, S.h_r_jump.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(d_rn.p, S.h_r_nJump.data(), (size_t)l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(d_mv.p, S.h_m_val.data(), nnz * sizeof(float), hipMemcpyHostToDevice, s));
+    if (S.squeezed) {
+        d_lj.alloc((size_t)l_nJump * sizeof(int));
+        MONA_HIP(hipMemcpyAsync(d_lj.p, S.h_l_jump.data(), (size_t)l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    decode_device(S, d_rj.as<int>(), d_rn.as<int>(), d_mv.as<float>(), d_lj.as<int>(), 1, nullptr, s);
+    MONA_HIP(hipStreamSynchronize(s));
+    S.on_device = true;
+}
+
+void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
+                      const float* yscale, hipStream_t s) {
+    MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
+    if (S.squeezed || S.l_nJump != S.l_size)
+        fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * 2, s);
+    int64_t total = (int64_t)S.l_nJump * n_vec;
+    if (total == 0) return;
+    const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
+    if (yscale)
+        spmv_c2<true><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(),
+                                                             S.val.as<float>(), ri, S.l_nJump, n_vec,
+                                                             v, w, yscale);
+    else
+        spmv_c2<false><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(),
+                                                              S.val.as<float>(), ri, S.l_nJump, n_vec,
+                                                              v, w, nullptr);
+    MONA_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------------- host-pointer products
+
+namespace {
+
+enum class Layout { ColSplit, RowSplit, ColReal, RowReal, ColInterleaved };
+
+struct HostProduct {
+    int r_size, l_size, l_nJump, n_vec;
+    const int *r_jump, *r_nJump, *l_jump;
+    const float* m_val;
+    int nBlock = 1;
+    const int* l_block_start = nullptr;
+    const int* block_length = nullptr;
+    const long long* m_block_start = nullptr;
+};
+
+void host_product(const HostProduct& a, Layout lay, const float* v0, const float* v1, float* w0,
+                  float* w1) {
+    HostCallCtx& C = host_ctx();
+    std::lock_guard<std::mutex> lk(C.mu);
+    C.init();
+    hipStream_t s = C.stream;
+    MONA_CHECK(a.n_vec >= 0, MONA_ERR_ARG, "n_vec < 0");
+    MONA_CHECK(a.r_nJump != nullptr && a.r_jump != nullptr && a.m_val != nullptr, MONA_ERR_ARG,
+               "NULL matrix array");
+    int64_t nnz = checked_nnz(a.r_nJump, a.l_nJump);
+    bool squeezed = a.l_jump != nullptr;
+    basic_checks(a.r_size, a.l_size, a.l_nJump, nnz, squeezed, false);
+    MONA_CHECK(squeezed || a.l_nJump <= a.l_size, MONA_ERR_ARG, "l_nJump > l_size");
+    bool cplx = (lay == Layout::ColSplit || lay == Layout::RowSplit || lay == Layout::ColInterleaved);
+    int ncomp = cplx ? 2 : 1;
+    int64_t vlen = (int64_t)a.r_size * a.n_vec, wlen = (int64_t)a.l_size * a.n_vec;
+    if (lay == Layout::ColInterleaved) { vlen *= 2; wlen *= 2; }
+
+    // staging
+    C.slot[0].ensure(std::max<int64_t>(nnz, 1) * sizeof(int));
+    C.slot[1].ensure((size_t)std::max(a.l_nJump, 1) * sizeof(int));
+    C.slot[2].ensure(std::max<int64_t>(nnz, 1) * sizeof(float));
+    C.slot[3].ensure((size_t)std::max(a.l_nJump, 1) * sizeof(int));
+    C.slot[4].ensure(std::max<int64_t>(vlen, 1) * sizeof(float));
+    C.slot[5].ensure(std::max<int64_t>(vlen, 1) * sizeof(float));
+    C.slot[6].ensure(std::max<int64_t>(wlen, 1) * sizeof(float));
+    C.slot[7].ensure(std::max<int64_t>(wlen, 1) * sizeof(float));
+    MONA_HIP(hipMemcpyAsync(C.slot[0].p, a.r_jump, nnz * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(C.slot[1].p, a.r_nJump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(C.slot[2].p, a.m_val, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+    if (squeezed)
+        MONA_HIP(hipMemcpyAsync(C.slot[3].p, a.l_jump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(C.slot[4].p, v0, vlen * sizeof(float), hipMemcpyHostToDevice, s));
+    if (v1) MONA_HIP(hipMemcpyAsync(C.slot[5].p, v1, vlen * sizeof(float), hipMemcpyHostToDevice, s));
+
+    SparseDev S;
+    S.r_size = a.r_size; S.l_size = a.l_size; S.l_nJump = a.l_nJump; S.nnz = nnz;
+    S.squeezed = squeezed;
+    DevBuf mstart;
+    if (a.nBlock > 1) {
+        MONA_CHECK(!squeezed, MONA_ERR_ARG, "multi-block squeezed matrices are not supported");
+        MONA_CHECK(a.m_block_start && a.l_block_start && a.block_length, MONA_ERR_ARG,
+                   "NULL block arrays");
+        // block starts must tile the stored rows in order
+        int64_t msum = 0;
+        int lsum = 0;
+        for (int b = 0; b < a.nBlock; b++) {
+            MONA_CHECK(a.l_block_start[b] == lsum && a.m_block_start[b] == msum, MONA_ERR_ARG,
+                       "block partition does not match r_nJump");
+            for (int i = 0; i < a.block_length[b]; i++) {
+                MONA_CHECK(lsum < a.l_nJump, MONA_ERR_ARG, "block lengths exceed l_nJump");
+                msum += a.r_nJump[lsum++];
+            }
+        }
+        MONA_CHECK(lsum == a.l_nJump, MONA_ERR_ARG, "block lengths do not cover l_nJump");
+        mstart.alloc((size_t)a.nBlock * sizeof(int64_t));
+        std::vector<int64_t> ms(a.m_block_start, a.m_block_start + a.nBlock);
+        MONA_HIP(hipMemcpyAsync(mstart.p, ms.data(), ms.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    }
+    decode_device(S, C.slot[0].as<int>(), C.slot[1].as<int>(), C.slot[2].as<float>(),
+                  squeezed ? C.slot[3].as<int>() : nullptr, a.nBlock, mstart.as<int64_t>(), s);
+
+    float* dw0 = C.slot[6].as<float>();
+    float* dw1 = C.slot[7].as<float>();
+    fill_f32(dw0, 0.0f, wlen, s);
+    if (cplx && lay != Layout::ColInterleaved) fill_f32(dw1, 0.0f, wlen, s);
+    int nplanes = a.n_vec * ncomp;
+    int64_t total = (int64_t)a.l_nJump * nplanes;
+    const int* ri = squeezed ? S.rowidx.as<int>() : nullptr;
+    const float* dv0 = C.slot[4].as<float>();
+    const float* dv1 = C.slot[5].as<float>();
+    if (total > 0) {
+        unsigned nb = blocks_for(total, 256);
+        switch (lay) {
+            case Layout::ColSplit:
+                spmv_planes<true><<<nb, 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(), S.val.as<float>(), ri,
+                                                     a.l_nJump, nplanes, 2, dv0, dv1, a.r_size, 1, dw0, dw1,
+                                                     a.l_size, 1);
+                break;
+            case Layout::ColReal:
+                spmv_planes<true><<<nb, 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(), S.val.as<float>(), ri,
+                                                     a.l_nJump, nplanes, 1, dv0, dv0, a.r_size, 1, dw0, dw0,
+                                                     a.l_size, 1);
+                break;
+            case Layout::ColInterleaved:
+                // column k of complex values: re at 2*(k*r_size+col), im at +1
+                spmv_planes<true><<<nb, 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(), S.val.as<float>(), ri,
+                                                     a.l_nJump, nplanes, 2, dv0, dv0 + 1, 2 * (int64_t)a.r_size, 2,
+                                                     dw0, dw0 + 1, 2 * (int64_t)a.l_size, 2);
+                break;
+            case Layout::RowSplit:
+                spmv_planes<false><<<nb, 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(), S.val.as<float>(), ri,
+                                                      a.l_nJump, nplanes, 2, dv0, dv1, 1, a.n_vec, dw0, dw1, 1,
+                                                      a.n_vec);
+                break;
+            case Layout::RowReal:
+                spmv_planes<false><<<nb, 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(), S.val.as<float>(), ri,
+                                                      a.l_nJump, nplanes, 1, dv0, dv0, 1, a.n_vec, dw0, dw0, 1,
+                                                      a.n_vec);
+                break;
+        }
+        MONA_HIP(hipGetLastError());
+    }
+    MONA_HIP(hipMemcpyAsync(w0, dw0, wlen * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (cplx && lay != Layout::ColInterleaved)
+        MONA_HIP(hipMemcpyAsync(w1, dw1, wlen * sizeof(float), hipMemcpyDeviceToHost, s));
+    MONA_HIP(hipStreamSynchronize(s));
+}
+
+HostProduct mk(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size, int* l_jump,
+               int l_nJump, int n_vec) {
+    HostProduct a;
+    a.r_size = r_size; a.l_size = l_size; a.l_nJump = l_nJump; a.n_vec = n_vec;
+    a.r_jump = r_jump; a.r_nJump = r_nJump; a.l_jump = l_jump; a.m_val = m_val;
+    return a;
+}
+
+}  // namespace
+}  // namespace mona
+
+using namespace mona;
+
+// ============================================================== C ABI
+
+extern "C" {
+
+const char* mona_last_error(void) { return mona::last_error(); }
+
+int mona_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return n;
+}
+
+const char* mona_version(void) { return "libmona 0.1 (gfx950)"; }
+
+void bmSparseMat_cC_oBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                               int* l_jump, int l_nJump, float* v_real, float* v_imag, int n_vec,
+                               float* w_real, float* w_imag) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::ColSplit, v_real, v_imag, w_real, w_imag);
+    });
+    if (st) fatal("bmSparseMat_cC_oBlock_omp", st);
+}
+
+void bmSparseMat_cC_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                           int* l_jump, int l_nJump, float* v_real, float* v_imag, int n_vec,
+                           float* w_real, float* w_imag) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::ColSplit, v_real, v_imag, w_real, w_imag);
+    });
+    if (st) fatal("bmSparseMat_cC_oBlock", st);
+}
+
+void bmSparseMat_cC_oBlock_2_omp(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                                 int* l_jump, int l_nJump, float* v_real, float* v_imag, int n_vec,
+                                 float* w_real, float* w_imag) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::ColSplit, v_real, v_imag, w_real, w_imag);
+    });
+    if (st) fatal("bmSparseMat_cC_oBlock_2_omp", st);
+}
+
+void bmSparseMat_cR_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                           int* l_jump, int l_nJump, float* v_real, float* v_imag, int n_vec,
+                           float* w_real, float* w_imag) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::RowSplit, v_real, v_imag, w_real, w_imag);
+    });
+    if (st) fatal("bmSparseMat_cR_oBlock", st);
+}
+
+void bmSparseMat_rC_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                           int* l_jump, int l_nJump, float* v, int n_vec, float* w) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::ColReal, v, nullptr, w, nullptr);
+    });
+    if (st) fatal("bmSparseMat_rC_oBlock", st);
+}
+
+void bmSparseMat_rR_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                           int* l_jump, int l_nJump, float* v, int n_vec, float* w) {
+    int st = guarded([&] {
+        host_product(mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec),
+                     Layout::RowReal, v, nullptr, w, nullptr);
+    });
+    if (st) fatal("bmSparseMat_rR_oBlock", st);
+}
+
+void bmSparseMat_cR_nBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                               int* l_jump, int l_nJump, int nBlock, int* block_length,
+                               int* l_block_start, long long* m_block_start, float* v_real,
+                               float* v_imag, int n_vec, float* w_real, float* w_imag) {
+    int st = guarded([&] {
+        HostProduct a = mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec);
+        a.nBlock = nBlock; a.block_length = block_length; a.l_block_start = l_block_start;
+        a.m_block_start = m_block_start;
+        host_product(a, Layout::RowSplit, v_real, v_imag, w_real, w_imag);
+    });
+    if (st) fatal("bmSparseMat_cR_nBlock_omp", st);
+}
+
+void bmSparseMat_rR_nBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size,
+                               int* l_jump, int l_nJump, int nBlock, int* block_length,
+                               int* l_block_start, long long* m_block_start, float* v, int n_vec,
+                               float* w) {
+    int st = guarded([&] {
+        HostProduct a = mk(r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, n_vec);
+        a.nBlock = nBlock; a.block_length = block_length; a.l_block_start = l_block_start;
+        a.m_block_start = m_block_start;
+        host_product(a, Layout::RowReal, v, nullptr, w, nullptr);
+    });
+    if (st) fatal("bmSparseMat_rR_nBlock_omp", st);
+}
+
+void bmSparseMat_r_nJump2index_mex(int l_size, int* r_nJump, int* ind) {
+    int st = guarded([&] {
+        HostCallCtx& C = host_ctx();
+        std::lock_guard<std::mutex> lk(C.mu);
+        C.init();
+        hipStream_t s = C.stream;
+        MONA_CHECK(l_size >= 0, MONA_ERR_ARG, "l_size < 0");
+        int64_t nnz = checked_nnz(r_nJump, l_size);
+        if (nnz == 0) return;
+        C.slot[1].ensure((size_t)std::max(l_size, 1) * sizeof(int));
+        C.slot[8].ensure((size_t)(l_size + 1) * sizeof(int64_t));
+        C.slot[9].ensure((size_t)nnz * sizeof(int));
+        MONA_HIP(hipMemcpyAsync(C.slot[1].p, r_nJump, (size_t)l_size * sizeof(int), hipMemcpyHostToDevice, s));
+        scan_i32_to_i64_exclusive(C.slot[1].as<int>(), C.slot[8].as<int64_t>(), l_size, s);
+        expand_rows<<<blocks_for(l_size, 256), 256, 0, s>>>(C.slot[8].as<int64_t>(), l_size,
+                                                            C.slot[9].as<int>());
+        MONA_HIP(hipGetLastError());
+        MONA_HIP(hipMemcpyAsync(ind, C.slot[9].p, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    });
+    if (st) fatal("bmSparseMat_r_nJump2index_mex", st);
+}
+
+// ---------------------------------------------------------------- handles
+
+int mona_sparse_create(int r_size, const int* r_jump, const int* r_nJump, const float* m_val,
+                       int l_size, const int* l_jump, int l_nJump, long long nnz,
+                       mona_sparse* out) {
+    return guarded([&] {
+        MONA_CHECK(out, MONA_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        auto* h = new mona_sparse_s;
+        try {
+            sparse_set_host(h->S, r_size, r_jump, r_nJump, m_val, l_size, l_jump, l_nJump, nnz);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int mona_sparse_destroy(mona_sparse S) {
+    return guarded([&] { delete S; });
+}
+
+int mona_sparse_info(mona_sparse S, long long* nnz, int* l_nJump, int* r_size, int* l_size,
+                     int* squeezed) {
+    return guarded([&] {
+        MONA_CHECK(S, MONA_ERR_ARG, "NULL handle");
+        if (nnz) *nnz = S->S.nnz;
+        if (l_nJump) *l_nJump = S->S.l_nJump;
+        if (r_size) *r_size = S->S.r_size;
+        if (l_size) *l_size = S->S.l_size;
+        if (squeezed) *squeezed = S->S.squeezed ? 1 : 0;
+    });
+}
+
+int mona_sparse_export(mona_sparse S, int* r_jump, int* r_nJump, float* m_val, int* l_jump) {
+    return guarded([&] {
+        MONA_CHECK(S, MONA_ERR_ARG, "NULL handle");
+        const SparseDev& D = S->S;
+        MONA_CHECK((int64_t)D.h_r_jump.size() == D.nnz, MONA_ERR_STATE, "host copy not kept");
+        if (r_jump) std::memcpy(r_jump, D.h_r_jump.data(), D.nnz * sizeof(int));
+        if (r_nJump) std::memcpy(r_nJump, D.h_r_nJump.data(), (size_t)D.l_nJump * sizeof(int));
+        if (m_val) std::memcpy(m_val, D.h_m_val.data(), D.nnz * sizeof(float));
+        if (l_jump && D.squeezed) std::memcpy(l_jump, D.h_l_jump.data(), (size_t)D.l_nJump * sizeof(int));
+    });
+}
+
+int mona_sparse_apply(mona_sparse S, const void* v, void* w, int n_vec, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(S && v && w && n_vec > 0, MONA_ERR_ARG, "bad argument");
+        sparse_upload(S->S, stream);
+        spmv_c2_rowmajor(S->S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, nullptr,
+                         stream);
+    });
+}
+
+}  // extern "C"

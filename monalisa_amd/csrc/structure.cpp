@@ -1,0 +1,338 @@
+// Trajectory -> gridding matrices Gn, Gu, Gut in the bmSparseMat jump
+// encoding, following src/gridding123/m/bmTraj2SparseMat.m:58-297 with
+// MATLAB's float64 semantics for every index computation:
+//   t/dK_u + fix(N/2+1); delete points with t < 1 or t > N; floor(t + 0.5)
+//   (odd window) or floor(t) (even window, offsets -nWin/2-1 .. nWin/2);
+//   periodic wrap mod(n-1, N)+1; x-fastest linear cell index;
+//   weights normpdf / besseli-Kaiser of the Euclidean distance;
+//   rows normalised by diag(1./sum(A,2)) (sum in ascending column order);
+// and the conversion of bmSparseMat_matlabSparse2bmSparseMat.m:6-40 +
+// bmSparseMat.m:62-188 (ascending columns per row, row squeeze, one block,
+// running deltas with an absolute 0-based first entry).
+//
+// Setup code run once per trajectory on the host cores; the products that
+// use the matrices run on the GPU.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mona_internal.hpp"
+
+namespace mona {
+
+namespace {
+
+double bessel_i0(double x) {
+    // power series sum_k ((x/2)^k / k!)^2
+    double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+    for (int k = 1; k < 500; k++) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < sum * 1e-17) break;
+    }
+    return sum;
+}
+
+struct KernelSpec {
+    int type;  // 0 gauss, 1 kaiser
+    double p[3];
+    double I0a;
+    double weight(double d) const {
+        if (type == MONA_KERNEL_GAUSS) {
+            double s = p[0];
+            double r = d / s;
+            return std::exp(-0.5 * r * r) / (std::sqrt(2.0 * M_PI) * s);
+        }
+        double r = d / p[0];
+        double w = 1.0 - r * r;
+        if (w < 0) w = 0;
+        w = p[1] * std::sqrt(w);
+        return bessel_i0(w) / I0a;
+    }
+};
+
+double as_single(double v) { return (double)(float)v; }
+
+}  // namespace
+
+struct HostSparse {
+    int r_size = 0, l_size = 0, l_nJump = 0;
+    std::vector<int> r_jump, r_nJump, l_jump;  // l_jump empty when not squeezed
+    std::vector<float> m_val;
+};
+
+// Builds the requested matrices.  t is [nPt][imDim].
+void traj_to_sparse(int imDim, int nPt, const double* t, const double* ve, const int* N_in,
+                    const double* dK_in, int kernelType, int nWin, const double* kp, int nParam,
+                    bool want_gn, bool want_gu, HostSparse* Gn, HostSparse* Gu, HostSparse* Gut) {
+    MONA_CHECK(imDim >= 1 && imDim <= 3, MONA_ERR_ARG, "imDim must be 1, 2 or 3");
+    MONA_CHECK(nPt >= 1, MONA_ERR_ARG, "nPt must be >= 1");
+    MONA_CHECK(nWin >= 1 && nWin <= 9, MONA_ERR_ARG, "nWin must be in [1, 9]");
+    MONA_CHECK(kernelType == MONA_KERNEL_GAUSS || kernelType == MONA_KERNEL_KAISER, MONA_ERR_ARG,
+               "unknown kernel type");
+    MONA_CHECK((kernelType == MONA_KERNEL_GAUSS && nParam == 2) ||
+                   (kernelType == MONA_KERNEL_KAISER && nParam == 3),
+               MONA_ERR_ARG, "Wrong list of gridding kernel parameters.");
+    MONA_CHECK(!want_gn || ve, MONA_ERR_ARG, "Gn needs the volume elements ve");
+    KernelSpec K;
+    K.type = kernelType;
+    for (int i = 0; i < 3; i++) K.p[i] = i < nParam ? as_single(kp[i]) : 0.0;
+    K.I0a = kernelType == MONA_KERNEL_KAISER ? bessel_i0(K.p[1]) : 1.0;
+
+    double N[3] = {1, 1, 1}, dK[3] = {1, 1, 1};
+    int64_t Ni[3] = {1, 1, 1};
+    int64_t Nu = 1;
+    double Du = 1.0;
+    double shift[3] = {0, 0, 0};
+    for (int d = 0; d < imDim; d++) {
+        MONA_CHECK(N_in[d] >= 1, MONA_ERR_ARG, "N_u must be >= 1");
+        N[d] = as_single((double)N_in[d]);
+        Ni[d] = N_in[d];
+        dK[d] = as_single(dK_in[d]);
+        MONA_CHECK(dK[d] > 0, MONA_ERR_ARG, "dK_u must be > 0");
+        Nu *= Ni[d];
+        shift[d] = std::trunc(N[d] / 2 + 1);
+        Du = Du * ((1.0 / dK[d]) / N[d]);
+    }
+    MONA_CHECK(Nu < 2147483647LL, MONA_ERR_ARG, "prod(N_u) must be < intmax('int32')");
+
+    // neighbour offsets (bmTraj2SparseMat.m:134-164), x fastest
+    std::vector<int> r;
+    double fshift;
+    if (nWin % 2 == 0) {
+        for (int v = -nWin / 2 - 1; v <= nWin / 2; v++) r.push_back(v);
+        fshift = 0.0;
+    } else {
+        for (int v = -(nWin / 2); v <= nWin / 2; v++) r.push_back(v);
+        fshift = 0.5;
+    }
+    int nw = (int)r.size();
+    int nNb = nw;
+    for (int d = 1; d < imDim; d++) nNb *= nw;
+
+    // per-point entries, rows of Gu (columns sorted ascending)
+    std::vector<int64_t> cnt(nPt);
+    std::vector<int> cols((size_t)nPt * nNb);
+    std::vector<double> wts((size_t)nPt * nNb);    // w * Du, normalised later
+    std::vector<double> wraw((size_t)nPt * nNb);   // w (for Gn)
+    std::vector<unsigned char> keep(nPt);
+    std::vector<double> Dn(nPt, 0.0);
+
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < nPt; p++) {
+        double tg[3], tf[3], tr[3];
+        bool del = false;
+        for (int d = 0; d < imDim; d++) {
+            tg[d] = t[(int64_t)p * imDim + d] / dK[d] + shift[d];
+            if (tg[d] < 1 || tg[d] > N[d]) del = true;
+            tf[d] = std::floor(tg[d] + fshift);
+            tr[d] = tg[d] - tf[d];
+        }
+        if (ve) {
+            double dn = ve[p];
+            for (int d = 0; d < imDim; d++) dn = dn / dK[d];
+            Dn[p] = dn;
+        }
+        keep[p] = !del;
+        if (del) { cnt[p] = 0; continue; }
+        std::pair<int, double> e[1000];
+        for (int j = 0; j < nNb; j++) {
+            int c[3] = {r[j % nw], imDim > 1 ? r[(j / nw) % nw] : 0, imDim > 2 ? r[j / (nw * nw)] : 0};
+            double sq = 0.0;
+            int64_t lin = 0, stride = 1;
+            for (int d = 0; d < imDim; d++) {
+                double dd = tr[d] - c[d];
+                sq = sq + dd * dd;
+                double n = tf[d] + c[d];
+                double m = std::fmod(n - 1, N[d]);
+                if (m < 0) m += N[d];
+                lin += (int64_t)m * stride;
+                stride *= Ni[d];
+            }
+            double w = K.weight(std::sqrt(sq));
+            e[j] = {(int)lin, w};
+        }
+        std::sort(e, e + nNb, [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+            return a.first < b.first;
+        });
+        // merge duplicate cells (sparse() sums them)
+        int m = 0;
+        for (int j = 0; j < nNb; j++) {
+            if (m > 0 && e[m - 1].first == e[j].first) e[m - 1].second += e[j].second;
+            else e[m++] = e[j];
+        }
+        cnt[p] = m;
+        int* cp = &cols[(size_t)p * nNb];
+        double* wp = &wts[(size_t)p * nNb];
+        double* rp = &wraw[(size_t)p * nNb];
+        for (int j = 0; j < m; j++) {
+            cp[j] = e[j].first;
+            rp[j] = e[j].second;
+            wp[j] = e[j].second * Du;
+        }
+    }
+
+    if (want_gu) {
+        // Gu rows: m = (w*Du) * (1/sum_row)
+        std::vector<int64_t> rowptr(nPt + 1, 0);
+        for (int p = 0; p < nPt; p++) rowptr[p + 1] = rowptr[p] + cnt[p];
+        int64_t nnz = rowptr[nPt];
+        MONA_CHECK(nnz > 0, MONA_ERR_ARG, "every trajectory point lies outside the grid");
+        Gu->r_size = (int)Nu;
+        Gu->l_size = nPt;
+        Gu->l_nJump = nPt;
+        Gu->r_nJump.assign(nPt, 0);
+        Gu->r_jump.assign(nnz, 0);
+        Gu->m_val.assign(nnz, 0.0f);
+        Gu->l_jump.clear();
+        std::vector<int> gcol(nnz);
+        std::vector<double> gval(nnz);
+#pragma omp parallel for schedule(static)
+        for (int p = 0; p < nPt; p++) {
+            int64_t o = rowptr[p];
+            Gu->r_nJump[p] = (int)cnt[p];
+            double s = 0.0;
+            for (int j = 0; j < cnt[p]; j++) s = s + wts[(size_t)p * nNb + j];
+            double inv = 1.0 / s;
+            for (int j = 0; j < cnt[p]; j++) {
+                double v = wts[(size_t)p * nNb + j] * inv;
+                gcol[o + j] = cols[(size_t)p * nNb + j];
+                gval[o + j] = v;
+                Gu->m_val[o + j] = (float)v;
+            }
+        }
+        int prev = 0;
+        for (int64_t k = 0; k < nnz; k++) {
+            Gu->r_jump[k] = gcol[k] - prev;
+            prev = gcol[k];
+        }
+        // Gut = Gu' (rows = cells, squeezed), entries in ascending point order
+        std::vector<int64_t> ccount(Nu + 1, 0);
+        for (int64_t k = 0; k < nnz; k++) ccount[gcol[k] + 1]++;
+        for (int64_t c = 0; c < Nu; c++) ccount[c + 1] += ccount[c];
+        std::vector<int> tcol(nnz);
+        std::vector<float> tval(nnz);
+        {
+            std::vector<int64_t> cur(ccount.begin(), ccount.end() - 1);
+            for (int p = 0; p < nPt; p++)
+                for (int64_t k = rowptr[p]; k < rowptr[p + 1]; k++) {
+                    int64_t pos = cur[gcol[k]]++;
+                    tcol[pos] = p;
+                    tval[pos] = (float)gval[k];
+                }
+        }
+        Gut->r_size = nPt;
+        Gut->l_size = (int)Nu;
+        Gut->r_nJump.clear();
+        Gut->l_jump.clear();
+        int prow = 0;
+        for (int64_t c = 0; c < Nu; c++) {
+            int64_t n = ccount[c + 1] - ccount[c];
+            if (n == 0) continue;
+            Gut->r_nJump.push_back((int)n);
+            Gut->l_jump.push_back((int)(c - prow));
+            prow = (int)c;
+        }
+        Gut->l_nJump = (int)Gut->r_nJump.size();
+        Gut->r_jump.resize(nnz);
+        int pc = 0;
+        for (int64_t k = 0; k < nnz; k++) {
+            Gut->r_jump[k] = tcol[k] - pc;
+            pc = tcol[k];
+        }
+        Gut->m_val.swap(tval);
+    }
+
+    if (want_gn) {
+        // Gn = rownorm(sparse(pt, cell, w.*Dn)') : rows = cells, ascending points
+        std::vector<int64_t> rowptr(nPt + 1, 0);
+        for (int p = 0; p < nPt; p++) rowptr[p + 1] = rowptr[p] + cnt[p];
+        int64_t nnz = rowptr[nPt];
+        MONA_CHECK(nnz > 0, MONA_ERR_ARG, "every trajectory point lies outside the grid");
+        std::vector<int64_t> ccount(Nu + 1, 0);
+        for (int p = 0; p < nPt; p++)
+            for (int j = 0; j < cnt[p]; j++) ccount[cols[(size_t)p * nNb + j] + 1]++;
+        for (int64_t c = 0; c < Nu; c++) ccount[c + 1] += ccount[c];
+        std::vector<int> tcol(nnz);
+        std::vector<double> tval(nnz);
+        {
+            std::vector<int64_t> cur(ccount.begin(), ccount.end() - 1);
+            for (int p = 0; p < nPt; p++)
+                for (int j = 0; j < cnt[p]; j++) {
+                    int64_t pos = cur[cols[(size_t)p * nNb + j]]++;
+                    tcol[pos] = p;
+                    tval[pos] = wraw[(size_t)p * nNb + j] * Dn[p];
+                }
+        }
+        Gn->r_size = nPt;
+        Gn->l_size = (int)Nu;
+        Gn->r_nJump.clear();
+        Gn->l_jump.clear();
+        Gn->m_val.resize(nnz);
+        int prow = 0;
+        for (int64_t c = 0; c < Nu; c++) {
+            int64_t b = ccount[c], e = ccount[c + 1];
+            if (e == b) continue;
+            double s = 0.0;
+            for (int64_t k = b; k < e; k++) s = s + tval[k];
+            double inv = 1.0 / s;
+            for (int64_t k = b; k < e; k++) Gn->m_val[k] = (float)(tval[k] * inv);
+            Gn->r_nJump.push_back((int)(e - b));
+            Gn->l_jump.push_back((int)(c - prow));
+            prow = (int)c;
+        }
+        Gn->l_nJump = (int)Gn->r_nJump.size();
+        Gn->r_jump.resize(nnz);
+        int pc = 0;
+        for (int64_t k = 0; k < nnz; k++) {
+            Gn->r_jump[k] = tcol[k] - pc;
+            pc = tcol[k];
+        }
+    }
+}
+
+}  // namespace mona
+
+using namespace mona;
+
+extern "C" int mona_traj2sparsemat(int imDim, int nPt, const double* t, const double* ve,
+                                   const int* N_u, const double* dK_u, int kernelType, int nWin,
+                                   const double* kernelParam, int nParam, int which,
+                                   mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut) {
+    return guarded([&] {
+        bool want_gn = which & 1, want_gu = which & 2;
+        MONA_CHECK(want_gn || want_gu, MONA_ERR_ARG, "nothing requested");
+        MONA_CHECK(t && N_u && dK_u && kernelParam, MONA_ERR_ARG, "NULL input");
+        MONA_CHECK(!want_gn || Gn, MONA_ERR_ARG, "Gn output is NULL");
+        MONA_CHECK(!want_gu || (Gu && Gut), MONA_ERR_ARG, "Gu/Gut output is NULL");
+        HostSparse hn, hu, hut;
+        traj_to_sparse(imDim, nPt, t, ve, N_u, dK_u, kernelType, nWin, kernelParam, nParam, want_gn,
+                       want_gu, &hn, &hu, &hut);
+        auto make = [&](HostSparse& h) {
+            auto* s = new mona_sparse_s;
+            try {
+                sparse_set_host(s->S, h.r_size, std::move(h.r_jump), std::move(h.r_nJump),
+                                std::move(h.m_val), h.l_size, std::move(h.l_jump));
+            } catch (...) {
+                delete s;
+                throw;
+            }
+            return s;
+        };
+        if (want_gn) *Gn = make(hn);
+        if (want_gu) {
+            *Gu = make(hu);
+            try {
+                *Gut = make(hut);
+            } catch (...) {
+                delete *Gu;
+                *Gu = nullptr;
+                throw;
+            }
+        }
+    });
+}

@@ -1,0 +1,196 @@
+"""Device-resident reconstruction plans: the reference's operator and solver
+interface (bmShanna / bmNakatsha / bmSensa / bmSteva /
+bmTevaMorphosia_chain / bmNasha) on HIP.
+
+torch tensors are used only as device-memory plumbing (allocation and
+streams); every operation runs in libmona.so.  Layouts (complex64):
+
+* image ``x``:   [prod(n_u)]  x fastest (Fortran order of the n_u block);
+* k-space ``y``: [nPt, nCh];
+* coil maps ``C``: [prod(n_u), nCh].
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, check, ptr, stream_of
+from .sparse import BmSparseMat, KERNELS, kernel_defaults
+
+
+def _require_cuda(t, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class Plan:
+    """Geometry + gridding kernel + coil maps + frames (Gu, Gut pairs)."""
+
+    def __init__(self, N_u, n_u, dK_u, nCh, kernelType=None, nWin=None, kernelParam=None):
+        kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+        self.N_u = [int(v) for v in np.ravel(N_u)]
+        self.n_u = [int(v) for v in np.ravel(n_u)] if n_u is not None else list(self.N_u)
+        self.dK_u = [float(v) for v in np.ravel(dK_u)]
+        self.imDim = len(self.N_u)
+        self.nCh = int(nCh)
+        self.kernelType, self.nWin, self.kernelParam = kernelType, nWin, kernelParam
+        Na = (C.c_int * self.imDim)(*self.N_u)
+        na = (C.c_int * self.imDim)(*self.n_u)
+        da = (C.c_double * self.imDim)(*self.dK_u)
+        kp = (C.c_double * len(kernelParam))(*kernelParam)
+        h = C.c_void_p()
+        check(lib.mona_plan_create(self.imDim, Na, na, da, self.nCh, KERNELS[kernelType], nWin, kp,
+                                   len(kernelParam), C.byref(h)))
+        self._h = h
+        self._frames = {}
+        self.nu = int(np.prod(self.n_u))
+        self.Nu = int(np.prod(self.N_u))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None:
+            lib.mona_plan_destroy(h)
+            self._h = None
+
+    def set_coils(self, Cmaps, stream=None):
+        _require_cuda(Cmaps, "C")
+        check(lib.mona_plan_set_coils(self._h, ptr(Cmaps), stream_of(stream)))
+
+    def set_frame(self, f, Gu: BmSparseMat, Gut: BmSparseMat):
+        check(lib.mona_plan_set_frame(self._h, int(f), Gu.handle(), Gut.handle()))
+        self._frames[f] = (Gu, Gut)   # keep the handles alive
+
+    def nPt(self, f=0):
+        return self._frames[f][0].l_size
+
+    def K(self):
+        import torch
+        out = torch.empty(self.nu, dtype=torch.float32, device="cuda")
+        check(lib.mona_plan_get_K(self._h, ptr(out), stream_of()))
+        return out
+
+    def KFC(self):
+        import torch
+        a = torch.empty((self.nu, self.nCh), dtype=torch.complex64, device="cuda")
+        b = torch.empty_like(a)
+        check(lib.mona_plan_get_KFC(self._h, ptr(a), ptr(b), stream_of()))
+        return a, b
+
+    # -- operators ---------------------------------------------------------
+    def bmShanna(self, x, f=0, out=None, stream=None):
+        """y = Gu * cfft(zerofill(x .* KFC))  (bmShanna_MATLAB.m:6-56)."""
+        import torch
+        _require_cuda(x, "x")
+        if out is None:
+            out = torch.empty((self.nPt(f), self.nCh), dtype=torch.complex64, device=x.device)
+        check(lib.mona_shanna(self._h, int(f), ptr(x), ptr(out), stream_of(stream)))
+        return out
+
+    def bmNakatsha(self, y, f=0, C_flag=True, out=None, stream=None):
+        """x = sum_c KFC_conj .* crop(cifft(Gut * y))  (bmNakatsha_MATLAB.m:6-60)."""
+        import torch
+        _require_cuda(y, "y")
+        if out is None:
+            shape = (self.nu,) if C_flag else (self.nu, self.nCh)
+            out = torch.empty(shape, dtype=torch.complex64, device=y.device)
+        check(lib.mona_nakatsha(self._h, int(f), ptr(y), ptr(out), 1 if C_flag else 0, stream_of(stream)))
+        return out
+
+    # -- solvers -----------------------------------------------------------
+    def bmSensa(self, x, y, ve, nCGD, ve_max=None, nIter=1, f=0, stream=None):
+        """CG-SENSE in place on x (bmSensa.m:6-96); returns the per-outer-
+        iteration residuals <y - Mx, HY (y - Mx)>."""
+        _require_cuda(x, "x")
+        _require_cuda(y, "y")
+        _require_cuda(ve, "ve")
+        res = (C.c_double * max(nIter, 1))()
+        check(lib.mona_sensa(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
+                             float(ve_max) if ve_max else 0.0, int(nIter), int(nCGD), res,
+                             stream_of(stream)))
+        return list(res)[:nIter]
+
+    def bmSteva(self, x, z, u, y, ve, delta, rho, nCGD, ve_max=None, nIter=1, f=0, stream=None):
+        """Spatial-TV ADMM in place on x, z, u (bmSteva.m:6-138).  z, u are
+        [imDim, nu] complex64 (given) or None (z = grad x, u = 0)."""
+        import torch
+        _require_cuda(x, "x")
+        given = z is not None
+        if z is None:
+            z = torch.zeros((self.imDim, self.nu), dtype=torch.complex64, device=x.device)
+            u = torch.zeros_like(z)
+        d = (C.c_float * 2)(*_sched(delta))
+        r = (C.c_float * 2)(*_sched(rho))
+        hist = (C.c_double * (2 * max(nIter, 1)))()
+        check(lib.mona_steva(self._h, int(f), ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(y),
+                             C.cast(ptr(ve), C.POINTER(C.c_float)), float(ve_max) if ve_max else 0.0,
+                             d, r, int(nIter), int(nCGD), hist, stream_of(stream)))
+        h = np.array(list(hist)[:2 * nIter]).reshape(nIter, 2)
+        return z, u, h
+
+    def bmTevaMorphosia_chain(self, x, z, u, y_list, ve_list, delta, rho, nCGD, ve_max, nIter=1,
+                              stream=None):
+        """Temporal-TV ADMM over all frames (bmTevaMorphosia_chain.m:6-294),
+        Tu = [] (identity deformations), regul_mode 'normal'.  x, z, u are
+        [nFr, nu] complex64 (z, u None: z = F x, u = 0)."""
+        import torch
+        nFr = x.shape[0]
+        given = z is not None
+        if z is None:
+            z = torch.zeros_like(x)
+            u = torch.zeros_like(x)
+        ycat = torch.cat([yy.reshape(-1) for yy in y_list])
+        vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
+        yo = np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]
+        vo = np.cumsum([0] + [vv.numel() for vv in ve_list])[:-1]
+        yoff = (C.c_longlong * nFr)(*[int(v) for v in yo])
+        voff = (C.c_longlong * nFr)(*[int(v) for v in vo])
+        d = (C.c_float * 2)(*_sched(delta))
+        r = (C.c_float * 2)(*_sched(rho))
+        hist = (C.c_double * (3 * max(nIter, 1)))()
+        check(lib.mona_teva_chain(self._h, ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(ycat), yoff,
+                                  C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), d, r,
+                                  int(nIter), int(nCGD), hist, stream_of(stream)))
+        h = np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
+        return z, u, h
+
+    def bmNasha(self, y, Gn: BmSparseMat, Cmaps=None, stream=None):
+        """Gridded recon (bmNasha.m:6-91)."""
+        import torch
+        shape = (self.nu,) if Cmaps is not None else (self.nu, self.nCh)
+        out = torch.empty(shape, dtype=torch.complex64, device=y.device)
+        check(lib.mona_nasha(self._h, Gn.handle(), ptr(y), ptr(Cmaps), ptr(out), stream_of(stream)))
+        return out
+
+
+def _sched(v):
+    v = np.abs(np.ravel(np.asarray(v, dtype=np.float64)))
+    if v.size == 1:
+        return [float(v[0]), float(v[0])]
+    return [float(v[0]), float(v[1])]
+
+
+def cfft(data, N, inverse=False, stream=None):
+    """Centred n-D DFT in place on a [prod(N), nBatch] complex64 device
+    tensor (fftshift(fft(ifftshift(.))) per axis; inverse scaled 1/N)."""
+    _require_cuda(data, "data")
+    N = [int(v) for v in np.ravel(N)]
+    nb = data.numel() // int(np.prod(N))
+    Na = (C.c_int * len(N))(*N)
+    check(lib.mona_cfft(len(N), Na, nb, ptr(data), 1 if inverse else 0, stream_of(stream)))
+    return data
+
+
+def timing(enable=None, kind=None):
+    """Kernel timing: timing(True) starts, timing(kind=k) -> (launches, ms)."""
+    if enable is not None:
+        check(lib.mona_timing_enable(1 if enable else 0))
+    if kind is not None:
+        n = C.c_longlong()
+        ms = C.c_double()
+        check(lib.mona_timing_read(int(kind), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+    return None

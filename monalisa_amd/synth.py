@@ -1,0 +1,154 @@
+"""Seeded synthetic inputs for the benchmark and test workloads: radial
+trajectories, volume elements (density compensation), coil maps and
+phantoms.  Every generator takes an explicit ``numpy.random.Generator`` or
+seed so that the same call yields the same arrays on any machine.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+GOLDEN_ANGLE_2D = math.pi * (3.0 - math.sqrt(5.0))  # 111.246... degrees
+
+
+def phyllotaxis_angles(nSeg, nShot, selfNav=False):
+    """Spiral-phyllotaxis spoke angles (src/traj3/bmPhyllotaxisAngle3.m:21-61):
+    spoke (i, j) at index (i-1) + (j-1)*nSeg gets phi = mod(c*goldAngle, 2 pi),
+    theta = pi/(2 sqrt(nseg_pure)) * sqrt(c) with the counter c running over
+    i outer, j inner."""
+    gold = (1 + math.sqrt(5)) / 2
+    gold_angle = 2 * math.pi - 2 * math.pi / gold
+    ntot = nSeg * nShot
+    pure = ntot - nShot if selfNav else ntot
+    q = math.pi / (2 * math.sqrt(pure))
+    phi = np.zeros(ntot)
+    theta = np.zeros(ntot)
+    c = 1
+    for i in range(nSeg):
+        for j in range(nShot):
+            idx = i + j * nSeg
+            if selfNav and i == 0:
+                continue
+            phi[idx] = math.fmod(c * gold_angle, 2 * math.pi)
+            theta[idx] = q * math.sqrt(c)
+            c += 1
+    return theta, phi
+
+
+def koosh_ball(nSeg, nShot, nSample, k_max, dK_n=1.0):
+    """3-D full-radial phyllotaxis trajectory, [3, nSample * nSeg * nShot]
+    (src/traj3/bmTraj_fullRadial3_phyllotaxis_lineAssym2.m:9-71) with
+    nSample points per spoke at r in [-1/2, 1/2) (step 1/nSample) scaled to
+    k in [-k_max, k_max) * dK_n."""
+    theta, phi = phyllotaxis_angles(nSeg, nShot)
+    r = (np.arange(nSample) - nSample / 2) / nSample
+    R = r[:, None]
+    x = R * np.cos(phi)[None, :] * np.sin(theta)[None, :]
+    y = R * np.sin(phi)[None, :] * np.sin(theta)[None, :]
+    z = R * np.cos(theta)[None, :] * np.ones_like(R)
+    t = np.stack([x, y, z], 0) * (2 * k_max) * dK_n       # [3, nSample, nSpoke]
+    return t.reshape(3, -1, order="F")
+
+
+def radial2d_golden(nSpoke, nSample, k_max, dK_n=1.0):
+    """2-D full-radial spokes at golden-angle increments, [2, nSample*nSpoke],
+    nSample points per spoke at k in [-k_max, k_max) * dK_n."""
+    ang = np.arange(nSpoke) * GOLDEN_ANGLE_2D
+    r = (np.arange(nSample) - nSample / 2) / nSample * (2 * k_max) * dK_n
+    x = r[:, None] * np.cos(ang)[None, :]
+    y = r[:, None] * np.sin(ang)[None, :]
+    return np.stack([x, y], 0).reshape(2, -1, order="F")
+
+
+def ve_radial3(t, nSample, nSpoke):
+    """Volume elements of a 3-D full-radial trajectory
+    (src/geom3/bmVolumeElement_full_radial3.m:42-49): dr * 4 pi |k|^2 /
+    (2 nLine), the centre point of each line gets (4/3) pi dr0^3 / (2 nLine)."""
+    k = np.sqrt((t ** 2).sum(0)).reshape(nSample, nSpoke, order="F")
+    dr = np.abs(np.diff(k[:, 0]))
+    dr = float(np.median(dr)) if dr.size else 1.0
+    v = dr * 4 * math.pi * k ** 2 / (2 * nSpoke)
+    v0 = (4.0 / 3.0) * math.pi * (dr / 2) ** 3 / (2 * nSpoke)
+    v[nSample // 2, :] = v0
+    return v.reshape(-1, order="F")
+
+
+def ve_radial2(t, nSample, nSpoke):
+    """Ramp density compensation of 2-D full-radial spokes
+    (src/geom2/bmVolumeElement_voronoi_full_radial2.m:11-77 for equally
+    spread angles): dr * |k| * pi / nSpoke, centre points pi (dr/2)^2 / nSpoke."""
+    k = np.sqrt((t ** 2).sum(0)).reshape(nSample, nSpoke, order="F")
+    dr = float(np.median(np.abs(np.diff(k[:, 0]))))
+    v = dr * k * math.pi / nSpoke
+    v[nSample // 2, :] = math.pi * (dr / 2) ** 2 / nSpoke
+    return v.reshape(-1, order="F")
+
+
+def coil_maps(n_u, nCh, rng):
+    """Smooth Gaussian-magnitude blobs (centres on a ring/sphere around the
+    FoV) times a low-order polynomial phase, normalised so that
+    sum_c |C_c|^2 = 1.  Returns [prod(n_u), nCh] complex64, x fastest."""
+    n_u = [int(v) for v in n_u]
+    d = len(n_u)
+    grids = np.meshgrid(*[(np.arange(n) - n / 2) / n for n in n_u], indexing="ij")
+    C = np.zeros(tuple(n_u) + (nCh,), dtype=np.complex128)
+    for c in range(nCh):
+        direction = rng.normal(size=d)
+        direction /= np.linalg.norm(direction) + 1e-12
+        centre = 0.6 * direction
+        width = 0.35 + 0.15 * rng.random()
+        r2 = sum((g - cc) ** 2 for g, cc in zip(grids, centre))
+        mag = np.exp(-r2 / (2 * width ** 2))
+        coef = rng.normal(scale=1.5, size=(d + 1,))
+        phase = coef[0] + sum(coef[i + 1] * grids[i] for i in range(d))
+        phase = phase + 0.5 * rng.normal() * grids[0] * grids[-1]
+        C[..., c] = mag * np.exp(1j * phase)
+    ss = np.sqrt((np.abs(C) ** 2).sum(-1, keepdims=True))
+    C = C / np.maximum(ss, 1e-12)
+    return C.reshape(-1, nCh, order="F").astype(np.complex64)
+
+
+def ellipsoid_phantom(n_u, rng, n_ellipsoids=8):
+    """Piecewise-constant phantom of random ellipsoids inside the FoV,
+    [prod(n_u)] complex64 (x fastest)."""
+    n_u = [int(v) for v in n_u]
+    d = len(n_u)
+    grids = np.meshgrid(*[(np.arange(n) - n / 2) / (n / 2) for n in n_u], indexing="ij")
+    img = np.zeros(tuple(n_u), dtype=np.complex128)
+    inside = sum((g / 0.85) ** 2 for g in grids) <= 1.0
+    img[inside] = 1.0
+    for _ in range(n_ellipsoids):
+        c = rng.uniform(-0.45, 0.45, size=d)
+        ax = rng.uniform(0.08, 0.3, size=d)
+        val = rng.uniform(-0.6, 0.9) + 1j * rng.uniform(-0.2, 0.2)
+        m = sum(((g - cc) / a) ** 2 for g, cc, a in zip(grids, c, ax)) <= 1.0
+        img[m] += val
+    return img.reshape(-1, order="F").astype(np.complex64)
+
+
+def add_noise(y, snr_db, rng):
+    """Complex Gaussian noise at the given SNR (signal RMS / noise RMS)."""
+    y = np.asarray(y)
+    rms = float(np.sqrt(np.mean(np.abs(y) ** 2)))
+    sigma = rms * 10 ** (-snr_db / 20.0) / math.sqrt(2)
+    n = rng.normal(scale=sigma, size=y.shape) + 1j * rng.normal(scale=sigma, size=y.shape)
+    return (y + n).astype(np.complex64)
+
+
+def dirichlet_bins(nSpoke, nBin, alpha, rng):
+    """Uneven split of nSpoke spokes into nBin contiguous-by-index bins with
+    counts from Dirichlet(alpha) (every bin gets at least one spoke).
+    Returns a list of spoke-index arrays."""
+    p = rng.dirichlet([alpha] * nBin)
+    counts = np.maximum(1, np.floor(p * nSpoke).astype(int))
+    while counts.sum() > nSpoke:
+        counts[np.argmax(counts)] -= 1
+    while counts.sum() < nSpoke:
+        counts[np.argmin(counts)] += 1
+    perm = rng.permutation(nSpoke)
+    out, o = [], 0
+    for c in counts:
+        out.append(np.sort(perm[o:o + c]))
+        o += c
+    return out

@@ -1,0 +1,729 @@
+"""ORACLE -- test infrastructure only.
+
+Float64 NumPy restatement of the reference's reconstruction hot path.  Only
+tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py
+may import this module, and only as the checker (or, for the baseline, as the
+timed CPU stand-in); the product library never calls it.
+
+Every function cites the reference file:line it follows.  What pins each one
+(tests/test_oracle.py):
+
+* ``traj2sparsemat`` / ``to_bmsparse`` / ``cpp_prepare``: brute-force dense
+  construction that loops over grid cells instead of neighbour offsets
+  (``dense_gridding_matrix_bruteforce``), row sums == 1, Gut == Gu^T exactly,
+  ascending columns, jump decode, every ``check()`` rule, and the
+  bmSparseMat_r_nJump2index closed form ``np.repeat``.
+* ``spmv``: dense float64 product of the decoded matrix; the float32
+  sequential form is the C port (port.c), pinned bit-exact against the
+  reference's own compiled kernel.
+* ``bmK``: the closed form Sum_x w(x) cos(2 pi k.x / N) evaluated directly
+  (``bmK_direct``), which must equal the reference's oversampled-grid DFT.
+* ``cfft``/``cifft``: numpy.fft with explicit shifts vs the (-1)^n modulation
+  identity and vs a dense DFT matrix on tiny sizes.
+* ``shanna``/``nakatsha``: adjointness <A x, y>_HY = <x, A^H y>_HX.
+* ``sensa``: on a tiny problem, CG converges to the dense least-squares
+  solution of the normal equations.
+* ``back_gradient``/``back_gradient_T``: the transpose identity on inputs whose
+  first slice is zero; ``prox_one_norm``: closed-form soft threshold.
+* ``steva``/``teva_chain``: parity unpinned beyond the above building blocks
+  and monotone objective checks (no reference run exists to compare with).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy import special
+
+# ---------------------------------------------------------------------------
+# kernel defaults  (src/varargin/bmVarargin_kernelType_nWin_kernelParam.m:10-28)
+# ---------------------------------------------------------------------------
+
+
+def kernel_defaults(kernelType=None, nWin=None, kernelParam=None):
+    if not kernelType:
+        kernelType = 'gauss'
+    if nWin is None:
+        nWin = 3
+    if kernelParam is None:
+        kernelParam = [0.61, 10.0] if kernelType == 'gauss' else [1.95, 10.0, 10.0]
+    # double(single(.)) round trip of bmVarargin_kernelType_nWin_kernelParam.m:30-31
+    nWin = float(np.float32(nWin))
+    kernelParam = [float(np.float32(v)) for v in kernelParam]
+    if kernelType == 'gauss' and len(kernelParam) == 3:
+        raise ValueError('Wrong list of gridding kernel parameters.')
+    if kernelType == 'kaiser' and len(kernelParam) == 2:
+        raise ValueError('Wrong list of gridding kernel parameters.')
+    return kernelType, nWin, kernelParam
+
+
+def kernel_weight(d, kernelType, kernelParam):
+    """Radial gridding kernel (bmTraj2SparseMat.m:191-204, bmK.m:70-83)."""
+    d = np.asarray(d, dtype=np.float64)
+    if kernelType == 'gauss':
+        sigma = kernelParam[0]
+        # MATLAB normpdf(x,0,sigma) = exp(-0.5*(x/sigma)^2) / (sqrt(2*pi)*sigma)
+        return np.exp(-0.5 * (d / sigma) ** 2) / (math.sqrt(2.0 * math.pi) * sigma)
+    tau, alpha = kernelParam[0], kernelParam[1]
+    w = np.maximum(1.0 - (d / tau) ** 2, 0.0)
+    w = alpha * np.sqrt(w)
+    return special.i0(w) / special.i0(alpha)
+
+
+def neighbour_offsets(imDim, nWin):
+    """Neighbour offsets, x fastest (bmTraj2SparseMat.m:134-164)."""
+    nWin = int(nWin)
+    if nWin % 2 == 0:
+        r = np.arange(-nWin // 2 - 1, nWin // 2 + 1)
+        shift = 0.0
+    else:
+        h = nWin // 2
+        r = np.arange(-h, h + 1)
+        shift = 0.5
+    grids = np.meshgrid(*([r] * imDim), indexing='ij')
+    c = np.stack([g.reshape(-1, order='F') for g in grids], axis=0).astype(np.float64)
+    return c, shift
+
+
+# ---------------------------------------------------------------------------
+# structure construction (src/gridding123/m/bmTraj2SparseMat.m:58-297)
+# ---------------------------------------------------------------------------
+
+
+def traj_entries(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """Per-point neighbour cells and weights, before sparse assembly.
+
+    t: [imDim, nPt] trajectory (physical k units), ve: [nPt] volume elements.
+    Returns dict with ind_1 (1-based linear cell, [nNb, nKeep]),
+    pt (0-based point index of each kept point), weight [nNb, nKeep],
+    Dn [nKeep], Du, keep mask.
+    """
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    t = np.array(t, dtype=np.float64, copy=True)
+    if t.ndim == 1:
+        t = t[None, :]
+    imDim, nPt = t.shape
+    N_u = [float(np.float32(v)) for v in np.ravel(N_u)]
+    dK_u = [float(np.float32(v)) for v in np.ravel(dK_u)]
+    Dn = np.array(ve, dtype=np.float64).reshape(-1).copy() if ve is not None else None
+    Du = 1.0
+    shift = []
+    for d in range(imDim):
+        t[d] = t[d] / dK_u[d]
+        if Dn is not None:
+            Dn = Dn / dK_u[d]
+        shift.append(math.trunc(N_u[d] / 2 + 1))
+        FoV = 1.0 / dK_u[d]
+        Du = Du * (FoV / N_u[d])
+    t = t + np.array(shift, dtype=np.float64)[:, None]
+    delete = np.zeros(nPt, dtype=bool)
+    for d in range(imDim):
+        delete |= (t[d] < 1) | (t[d] > N_u[d])
+    c, fshift = neighbour_offsets(imDim, nWin)
+    nNb = c.shape[1]
+    t_floor = np.floor(t + fshift)
+    t_rest = t - t_floor
+    dd = t_rest[:, None, :] - c[:, :, None]          # [imDim, nNb, nPt]
+    sq = np.zeros(dd.shape[1:])
+    for d in range(imDim):
+        sq = sq + dd[d] ** 2
+    dist = np.sqrt(sq)
+    w = kernel_weight(dist, kernelType, kernelParam)  # [nNb, nPt]
+    n = t_floor[:, None, :] + c[:, :, None]
+    lin = np.ones((nNb, nPt))
+    stride = 1.0
+    for d in range(imDim):
+        nd = np.mod(n[d] - 1, N_u[d]) + 1
+        lin = lin + (nd - 1) * stride
+        stride *= N_u[d]
+    keep = ~delete
+    return dict(ind_1=lin[:, keep], weight=w[:, keep], pt=np.nonzero(keep)[0],
+                Dn=None if Dn is None else Dn[keep], Du=Du, keep=keep, nNb=nNb,
+                nPt=nPt, Nu_tot=int(np.prod(N_u)), N_u=N_u, dK_u=dK_u,
+                kernelType=kernelType, nWin=nWin, kernelParam=kernelParam,
+                t_grid=t)
+
+
+def _assemble(rows, cols, vals, shape):
+    """MATLAB sparse(rows, cols, vals): duplicates summed, zeros dropped.
+    Returns row-major COO sorted by (row, col) with float64 values."""
+    key = rows.astype(np.int64) * shape[1] + cols.astype(np.int64)
+    order = np.argsort(key, kind='stable')
+    key = key[order]
+    vals = vals[order]
+    uniq, start = np.unique(key, return_index=True)
+    summed = np.add.reduceat(vals, start) if len(vals) else vals
+    nz = summed != 0
+    uniq, summed = uniq[nz], summed[nz]
+    return (uniq // shape[1]).astype(np.int64), (uniq % shape[1]).astype(np.int64), summed
+
+
+def _row_normalise(r, c, v, nrow):
+    """diag(1./sum(A,2)) * A with the row sum taken in ascending column order
+    (bmTraj2SparseMat.m:249-254, 275-280)."""
+    s = np.zeros(nrow)
+    np.add.at(s, r, v)   # rows visited in ascending column order (sorted COO)
+    inv = np.zeros(nrow)
+    nzr = s != 0
+    inv[nzr] = 1.0 / s[nzr]
+    return v * inv[r]
+
+
+def traj2sparsemat(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=None,
+                   which=('Gn', 'Gu', 'Gut')):
+    """bmTraj2SparseMat (src/gridding123/m/bmTraj2SparseMat.m:12-314).
+    Returns dict of bmSparseMat field dicts (cpp_prepared, one_block)."""
+    e = traj_entries(t, ve, N_u, dK_u, kernelType, nWin, kernelParam)
+    nNb, nPt, Nu = e['nNb'], e['nPt'], e['Nu_tot']
+    rows = np.repeat(e['pt'], nNb)                       # ind_2 - 1
+    cols = (e['ind_1'].T.reshape(-1) - 1).astype(np.int64)  # ind_1 - 1
+    w = e['weight'].T.reshape(-1)
+    prov = dict(N_u=e['N_u'], d_u=e['dK_u'], kernel_type=e['kernelType'],
+                nWin=e['nWin'], kernelParam=e['kernelParam'])
+    out = {}
+    if 'Gn' in which:
+        Dn = np.repeat(e['Dn'], nNb)
+        # sparse(ind_2, ind_1, w.*Dn)' : [Nu, nPt]
+        r, c, v = _assemble(cols, rows, w * Dn, (Nu, nPt))
+        v = _row_normalise(r, c, v, Nu)
+        G = to_bmsparse(r, c, v, (Nu, nPt), prov)
+        l_squeeze(G)
+        cpp_prepare(G)
+        out['Gn'] = G
+    if 'Gu' in which or 'Gut' in which:
+        r, c, v = _assemble(rows, cols, w * e['Du'], (nPt, Nu))
+        v = _row_normalise(r, c, v, nPt)
+        Gu = to_bmsparse(r, c, v, (nPt, Nu), prov)
+        Gut = to_bmsparse(c, r, v, (Nu, nPt), prov)
+        cpp_prepare(Gu)
+        l_squeeze(Gut)
+        cpp_prepare(Gut)
+        out['Gu'] = Gu
+        out['Gut'] = Gut
+    return out
+
+
+def to_bmsparse(r, c, v, shape, prov=None):
+    """bmSparseMat_matlabSparse2bmSparseMat
+    (src/sparseMat/m/bmSparseMat_matlabSparse2bmSparseMat.m:6-40):
+    entries row by row, ascending column index, m_val rounded to single."""
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    r_nJump = np.bincount(r, minlength=shape[0]).astype(np.int32)
+    G = dict(r_size=int(shape[1]), l_size=int(shape[0]), l_nJump=int(shape[0]),
+             r_nJump=r_nJump, r_ind=(c + 1).astype(np.int64),
+             m_val=v.astype(np.float32), m_val64=v, l_ind=None, l_jump=None,
+             r_jump=None, type='matlab_ind', l_squeeze_flag=False,
+             block_type='void')
+    if prov:
+        G.update(prov)
+    return G
+
+
+def l_squeeze(G):
+    """bmSparseMat.l_squeeze (src/sparseMat/m/bmSparseMat.m:62-94)."""
+    assert G['type'] == 'matlab_ind'
+    mask = G['r_nJump'] != 0
+    G['r_nJump'] = G['r_nJump'][mask].astype(np.int32)
+    G['l_ind'] = (np.arange(1, G['l_size'] + 1)[mask]).astype(np.int64)
+    G['l_nJump'] = int(mask.sum())
+    G['type'] = 'l_squeezed_matlab_ind'
+    G['l_squeeze_flag'] = True
+
+
+def cpp_prepare(G):
+    """bmSparseMat.cpp_prepare('one_block') (src/sparseMat/m/bmSparseMat.m:100-188)
+    with bmSparseMat_blockPartition(...,'one') (bmSparseMat_blockPartition.m:26-33)."""
+    r_ind = G['r_ind']
+    if r_ind.size == 0:
+        raise ValueError('The sparse matrix got a zero block. Preparation aborted.')
+    rj = np.empty(r_ind.shape, dtype=np.int64)
+    rj[0] = r_ind[0] - 1
+    rj[1:] = np.diff(r_ind)
+    G['r_jump'] = rj.astype(np.int32)
+    if G['type'] == 'l_squeezed_matlab_ind':
+        li = G['l_ind']
+        lj = np.empty(li.shape, dtype=np.int64)
+        lj[0] = li[0] - 1
+        lj[1:] = np.diff(li)
+        G['l_jump'] = lj.astype(np.int32)
+        G['type'] = 'l_squeezed_cpp_prepared'
+    else:
+        G['l_jump'] = None
+        G['type'] = 'cpp_prepared'
+    G['nBlock'] = 1
+    G['l_block_start'] = 0
+    G['block_length'] = int(len(G['r_nJump']))
+    G['m_block_start'] = 0
+    G['block_type'] = 'one_block'
+    G['r_ind'] = None
+    G['l_ind'] = None
+
+
+def decode(G):
+    """Absolute column per entry, row pointer and output row per stored row
+    of a cpp_prepared one-block bmSparseMat."""
+    col = np.cumsum(G['r_jump'].astype(np.int64))
+    rowptr = np.zeros(len(G['r_nJump']) + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(G['r_nJump'].astype(np.int64))
+    if G.get('l_jump') is not None:
+        rows = np.cumsum(G['l_jump'].astype(np.int64))
+    else:
+        rows = np.arange(len(G['r_nJump']), dtype=np.int64)
+    return col, rowptr, rows
+
+
+def to_dense(G, dtype=np.float64, exact=False):
+    col, rowptr, rows = decode(G)
+    A = np.zeros((G['l_size'], G['r_size']), dtype=dtype)
+    vals = G['m_val'].astype(np.float64)
+    rid = np.repeat(rows, np.diff(rowptr))
+    np.add.at(A, (rid, col), vals)
+    return A
+
+
+def to_scipy(G):
+    from scipy import sparse
+    col, rowptr, rows = decode(G)
+    rid = np.repeat(rows, np.diff(rowptr))
+    return sparse.csr_matrix((G['m_val'].astype(np.float64), (rid, col)),
+                             shape=(G['l_size'], G['r_size']))
+
+
+def spmv(G, V):
+    """W = S V for V [r_size, nVec] (float64)."""
+    return to_scipy(G) @ V
+
+
+def r_nJump2index(r_nJump):
+    """bmSparseMat_r_nJump2index_mex.cpp:70-89 (0-based)."""
+    r = np.asarray(r_nJump, dtype=np.int64)
+    return np.repeat(np.arange(len(r), dtype=np.int32), r)
+
+
+def check(G):
+    """The rules of bmSparseMat.check (src/sparseMat/m/bmSparseMat.m:192-393)
+    for a cpp_prepared matrix; returns list of failed rule names."""
+    fails = []
+    rn = G['r_nJump']
+    if G['l_size'] < 2 or G['r_size'] < 2 or G['l_nJump'] < 2:
+        fails.append('2.5')
+    if np.any(rn < 0):
+        fails.append('2.6')
+    s = int(rn.astype(np.int64).sum())
+    if s == 0:
+        fails.append('3.1')
+    elif s != G['m_val'].size:
+        fails.append('3.2')
+    if rn.size != G['l_nJump']:
+        fails.append('3.3')
+    if G['r_jump'] is None:
+        fails.append('8.2')
+    if G['type'] == 'cpp_prepared':
+        if G.get('l_jump') is not None:
+            fails.append('9.1')
+        if s != G['r_jump'].size:
+            fails.append('9.2')
+    if G['type'] == 'l_squeezed_cpp_prepared':
+        if G.get('l_jump') is None:
+            fails.append('10.1')
+        elif G['l_jump'].size != G['l_nJump']:
+            fails.append('10.2')
+    if not (G['r_size'] < 2**31 - 1 and G['l_size'] < 2**31 - 1):
+        fails.append('11.1')
+    return fails
+
+
+def dense_gridding_matrix_bruteforce(t, N_u, dK_u, kernelType=None, nWin=None,
+                                     kernelParam=None):
+    """Independent restatement used to pin traj2sparsemat: loops over every
+    grid cell of the (periodic) grid and keeps the cells whose per-axis
+    wrapped offset from the point's rounded position is within the window.
+    Returns the un-normalised dense [nPt, Nu] weight matrix (w * Du)."""
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    t = np.atleast_2d(np.asarray(t, dtype=np.float64))
+    imDim, nPt = t.shape
+    N = [int(v) for v in np.ravel(N_u)]
+    dK = [float(np.float32(v)) for v in np.ravel(dK_u)]
+    Nu = int(np.prod(N))
+    A = np.zeros((nPt, Nu))
+    Du = 1.0
+    for d in range(imDim):
+        Du *= (1.0 / dK[d]) / N[d]
+    h = int(nWin) // 2
+    assert int(nWin) % 2 == 1
+    grids = np.meshgrid(*[np.arange(n) for n in N], indexing='ij')
+    cell_lin = np.zeros(grids[0].shape, dtype=np.int64)
+    stride = 1
+    for d in range(imDim):
+        cell_lin += grids[d] * stride
+        stride *= N[d]
+    for p in range(nPt):
+        tg = [t[d, p] / dK[d] + math.trunc(N[d] / 2 + 1) for d in range(imDim)]
+        if any(tg[d] < 1 or tg[d] > N[d] for d in range(imDim)):
+            continue
+        base = [math.floor(tg[d] + 0.5) for d in range(imDim)]   # 1-based
+        ok = np.ones(grids[0].shape, dtype=bool)
+        offs = []
+        for d in range(imDim):
+            # signed wrapped offset of cell (1-based grids[d]+1) from base
+            o = np.mod(grids[d] + 1 - base[d] + N[d] // 2, N[d]) - N[d] // 2
+            ok &= np.abs(o) <= h
+            offs.append(o)
+        sq = np.zeros(grids[0].shape)
+        for d in range(imDim):
+            sq = sq + ((tg[d] - base[d]) - offs[d]) ** 2
+        w = kernel_weight(np.sqrt(sq), kernelType, kernelParam)
+        A[p, cell_lin[ok]] += w[ok] * Du
+    return A
+
+
+# ---------------------------------------------------------------------------
+# centred DFTs, zero-fill, crop
+# ---------------------------------------------------------------------------
+
+
+def cfft(x, axes):
+    """fftshift(fft(ifftshift(x,n),[],n),n) per axis (bmShanna_MATLAB.m:46-50)."""
+    for a in axes:
+        x = np.fft.fftshift(np.fft.fft(np.fft.ifftshift(x, axes=a), axis=a), axes=a)
+    return x
+
+
+def cifft(x, axes):
+    """fftshift(ifft(ifftshift(x,n),[],n),n) per axis (bmNakatsha_MATLAB.m:36-40)."""
+    for a in axes:
+        x = np.fft.fftshift(np.fft.ifft(np.fft.ifftshift(x, axes=a), axis=a), axes=a)
+    return x
+
+
+def center_slices(N, n):
+    """Centre window N/2+1-n/2 : N/2+n/2 (1-based) of bmImZeroFill.m:41-51 /
+    bmImCrope.m, as 0-based slices."""
+    return tuple(slice(N[d] // 2 - n[d] // 2, N[d] // 2 - n[d] // 2 + n[d])
+                 for d in range(len(N)))
+
+
+def zero_fill(x, N, n):
+    out = np.zeros(tuple(N) + x.shape[len(N):], dtype=np.complex128)
+    out[center_slices(N, n)] = x
+    return out
+
+
+def crop(x, N, n):
+    return x[center_slices(N, n)]
+
+
+# ---------------------------------------------------------------------------
+# deapodisation (src/fourier123/prep_function/bmK.m:6-156, bmKF.m, bmKF_conj.m)
+# ---------------------------------------------------------------------------
+
+
+def bmK(N_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """K on the N_u grid (x fastest index = axis 0), float64, before the
+    single() cast.  Follows bmK.m:29-151 literally: kernel on the 2x finer
+    grid, per-axis box mask, centred DFT, centre crop, 1/|Re| normalised,
+    clamp K_max."""
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    N = [int(v) for v in np.ravel(N_u)]
+    imDim = len(N)
+    osf = 2
+    Nos = [round(n * osf) for n in N]
+    ax = [np.arange(-n * osf // 2, n * osf // 2) / osf for n in N]
+    g = np.meshgrid(*ax, indexing='ij')
+    d = np.sqrt(sum(gi ** 2 for gi in g))
+    w = kernel_weight(d, kernelType, kernelParam)
+    K_max = kernelParam[-1]
+    h = int(nWin) // 2
+    if imDim == 1:
+        # bmK.m:88 masks x < nWin_half (reference quirk reproduced)
+        w[(g[0] < h) | (g[0] > h)] = 0
+    else:
+        for gi in g:
+            w[(gi < -h) | (gi > h)] = 0
+    F = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(w)))
+    F = F / (np.prod(Nos) * np.prod([1.0 / n for n in N]))
+    sl = tuple(slice(Nos[i] // 2 - N[i] // 2, Nos[i] // 2 + N[i] // 2) for i in range(imDim))
+    K = np.abs(np.real(F[sl]))
+    K = K / np.max(np.abs(K))
+    K = 1.0 / K
+    return np.minimum(K, K_max)
+
+
+def bmK_direct(N_u, kernelType=None, nWin=None, kernelParam=None):
+    """Closed form of the same K: Sum over the window points x (half-integer
+    spacing) of w(|x|) cos(2 pi sum_d x_d k_d / N_d), k in [-N/2, N/2)."""
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    N = [int(v) for v in np.ravel(N_u)]
+    h = int(nWin) // 2
+    xs = np.arange(-2 * h, 2 * h + 1) / 2.0
+    pts = np.meshgrid(*([xs] * len(N)), indexing='ij')
+    pts = np.stack([p.ravel() for p in pts], axis=1)
+    wts = kernel_weight(np.sqrt((pts ** 2).sum(1)), kernelType, kernelParam)
+    ks = np.meshgrid(*[np.arange(-n // 2, n // 2) for n in N], indexing='ij')
+    phase = sum(pts[:, d][:, None] * ks[d].ravel()[None, :] / N[d] for d in range(len(N)))
+    K = np.abs((wts[:, None] * np.cos(2 * np.pi * phase)).sum(0)).reshape([n for n in N])
+    K = K / wts.sum()
+    return np.minimum(1.0 / K, kernelParam[-1])
+
+
+def kf(C, N_u, n_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """KFC = K.*F.*C, F = 1/prod(N_u)/prod(dK_u)  (bmKF.m:6-35).
+    C: [n_u..., nCh] complex.  Returns [n_u..., nCh] complex128."""
+    K = crop(bmK(N_u, dK_u, kernelType, nWin, kernelParam), N_u, n_u)
+    F = 1.0 / np.prod(N_u) / np.prod(dK_u)
+    return K[..., None] * F * C
+
+
+def kf_conj(C_conj, N_u, n_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """KFC_conj = K.*F_conj.*conj(C), F_conj = 1/prod(dK_u) (bmKF_conj.m:6-38)."""
+    K = crop(bmK(N_u, dK_u, kernelType, nWin, kernelParam), N_u, n_u)
+    return K[..., None] * (1.0 / np.prod(dK_u)) * C_conj
+
+
+# ---------------------------------------------------------------------------
+# operators: images are [n_u..., ] arrays indexed (x, y, z); grids flatten
+# x fastest (Fortran order) to match the reference's column layout.
+# ---------------------------------------------------------------------------
+
+
+def _flatF(a, ndim):
+    return a.reshape((-1,) + a.shape[ndim:], order='F')
+
+
+def _unflatF(a, shape):
+    return a.reshape(tuple(shape) + a.shape[1:], order='F')
+
+
+def shanna(x, Gu, KFC, N_u, n_u):
+    """Forward model y = Gu * cfft(zero_fill(x .* KFC))
+    (src/fourier123/map_function/nonCartesian/bmShanna_MATLAB.m:6-56).
+    x: [n_u...] image, KFC [n_u..., nCh].  Returns y [nPt, nCh]."""
+    d = len(N_u)
+    xc = x[..., None] * KFC
+    g = cfft(zero_fill(xc, N_u, n_u), axes=range(d))
+    return spmv(Gu, _flatF(g, d))
+
+
+def nakatsha(y, Gut, KFC_conj, C_flag, N_u, n_u):
+    """Adjoint x = sum_c KFC_conj .* crop(cifft(Gut * y))
+    (src/fourier123/map_function/nonCartesian/bmNakatsha_MATLAB.m:6-60)."""
+    d = len(N_u)
+    g = _unflatF(spmv(Gut, y), N_u)
+    g = crop(cifft(g, axes=range(d)), N_u, n_u) * KFC_conj
+    return g.sum(-1) if C_flag else g
+
+
+def hx_of(N_u, dK_u):
+    """HX = prod(dX_u), dX_u = (1/dK_u)/N_u  (bmSensa.m:26-27)."""
+    return float(np.prod([(1.0 / dk) / n for dk, n in zip(dK_u, N_u)]))
+
+
+def sensa(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, nCGD, ve_max=None, nIter=1,
+          kernel=(None, None, None)):
+    """CG-SENSE, float64 scalars and vectors
+    (src/fourier123/solver_function/imDim/nonCartesian/bmSensa.m:6-96).
+    Returns (x, residuals per outer iteration)."""
+    eps = 10 * np.finfo(np.float32).eps
+    KFC = kf(C, N_u, n_u, dK_u, *kernel)
+    KFCc = kf_conj(np.conj(C), N_u, n_u, dK_u, *kernel)
+    HX = hx_of(N_u, dK_u)
+    ve = np.broadcast_to(np.asarray(ve, dtype=np.float64).reshape(-1, 1), y.shape)
+    if ve_max is None:
+        ve_max = ve.max()
+    HY = np.minimum(ve, ve_max)
+    x = np.array(x, dtype=np.complex128)
+    M = lambda v: shanna(v, Gu, KFC, N_u, n_u)
+    Mh = lambda r: (1.0 / HX) * nakatsha(HY * r, Gut, KFCc, True, N_u, n_u)
+    Rs = []
+    for _ in range(nIter):
+        res = y - M(x)
+        dag = Mh(res)
+        sqn = np.real(np.vdot(dag, HX * dag))
+        p = dag
+        for i in range(1, nCGD + 1):
+            if sqn < eps:
+                break
+            Mp = M(p)
+            den = np.real(np.vdot(Mp, HY * Mp))
+            a = sqn / den
+            x = x + a * p
+            if i == nCGD:
+                break
+            res = res - a * Mp
+            dag = Mh(res)
+            sqn_new = np.real(np.vdot(dag, HX * dag))
+            b = sqn_new / sqn
+            sqn = sqn_new
+            p = dag + b * p
+        r = y - M(x)
+        Rs.append(float(np.real(np.vdot(r, HY * r))))
+    return x, Rs
+
+
+def back_gradient(x, dX, axis):
+    """bmBackGradient_n (src/optim/bmBackGradient_n.m:36-44): backward
+    difference with circular shift, first slice set to 0."""
+    g = (x - np.roll(x, 1, axis=axis)) / dX[axis]
+    idx = [slice(None)] * x.ndim
+    idx[axis] = 0
+    g[tuple(idx)] = 0
+    return g
+
+
+def back_gradient_T(x, dX, axis):
+    """bmBackGradient_nT (src/optim/bmBackGradient_nT.m:34-44)."""
+    g = x.copy()
+    idx = [slice(None)] * x.ndim
+    idx[axis] = 0
+    g[tuple(idx)] = 0
+    return (g - np.roll(x, -1, axis=axis)) / dX[axis]
+
+
+def prox_one_norm(w, s):
+    """Soft threshold applied to real and imaginary parts separately
+    (src/linSpace/bmProx_oneNorm.m:30-41)."""
+    def st(v):
+        return (v > s) * (v - s) + (v < -s) * (v + s)
+    return st(np.real(w)) + 1j * st(np.imag(w))
+
+
+def steva(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
+          ve_max=None, nIter=1, kernel=(None, None, None)):
+    """Spatial-TV ADMM (src/fourier123/solver_function/imDim/nonCartesian/
+    bmSteva.m:6-138), float64.  z, u: lists of per-axis images or None."""
+    eps = 10 * np.finfo(np.float32).eps
+    d = len(N_u)
+    KFC = kf(C, N_u, n_u, dK_u, *kernel)
+    KFCc = kf_conj(np.conj(C), N_u, n_u, dK_u, *kernel)
+    dX = [(1.0 / dk) / n for dk, n in zip(dK_u, N_u)]
+    HX = hx_of(N_u, dK_u)
+    HZ = HX
+    ve = np.broadcast_to(np.asarray(ve, dtype=np.float64).reshape(-1, 1), y.shape)
+    if ve_max is None:
+        ve_max = ve.max()
+    HY = np.minimum(ve, ve_max)
+    delta = np.broadcast_to(np.atleast_1d(np.abs(delta)), (nIter,)) if np.size(delta) == 1 \
+        else np.linspace(abs(delta[0]), abs(delta[1]), nIter)
+    rho = np.broadcast_to(np.atleast_1d(np.abs(rho)), (nIter,)) if np.size(rho) == 1 \
+        else np.linspace(abs(rho[0]), abs(rho[1]), nIter)
+    x = np.array(x, dtype=np.complex128)
+    M = lambda v: shanna(v, Gu, KFC, N_u, n_u)
+    Mh = lambda r: (1.0 / HX) * nakatsha(HY * r, Gut, KFCc, True, N_u, n_u)
+    Fop = lambda v: np.stack([back_gradient(v, dX, a) for a in range(d)])
+    FT = lambda w: sum(back_gradient_T(w[a], dX, a) for a in range(d))
+    z = Fop(x) if z is None else np.array(z, dtype=np.complex128)
+    u = np.zeros_like(z) if u is None else np.array(u, dtype=np.complex128)
+    hist = []
+    for c in range(nIter):
+        rc = rho[c]
+        ry = y - M(x)
+        rz = (z - u) - Fop(x)
+        dag = Mh(ry) + rc * (1.0 / HX) * FT(HZ * rz)
+        p = dag
+        sqn = np.real(np.vdot(dag, HX * dag))
+        for i in range(1, nCGD + 1):
+            if sqn < eps:
+                break
+            Mp = M(p)
+            Fp = Fop(p)
+            den = np.real(np.vdot(Mp, HY * Mp)) + np.real(np.vdot(Fp, rc * HZ * Fp))
+            a = sqn / den
+            x = x + a * p
+            if i == nCGD:
+                break
+            ry = ry - a * Mp
+            rz = rz - a * Fp
+            dag = Mh(ry) + rc * (1.0 / HX) * FT(HZ * rz)
+            sqn_new = np.real(np.vdot(dag, HX * dag))
+            b = sqn_new / sqn
+            sqn = sqn_new
+            p = dag + b * p
+        Fx_u = Fop(x) + u
+        z = prox_one_norm(Fx_u, delta[c] / rc)
+        u = Fx_u - z
+        r = y - M(x)
+        R = float(np.real(np.vdot(r, HY * r)))
+        g = Fop(x)
+        TV = HZ * (np.abs(g.real).sum() + np.abs(g.imag).sum())
+        hist.append((R, TV))
+    return x, z, u, hist
+
+
+def teva_chain(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
+               ve_max, nIter=1, kernel=(None, None, None)):
+    """Temporal-TV ADMM over nFr cyclic frames with Tu = [] (identity
+    deformation), regul_mode 'normal'
+    (src/fourier123/solver_function/imDim_morphosia/nonCartesian/
+    bmTevaMorphosia_chain.m:6-294), float64.
+    x, y, ve, Gu, Gut, z, u: lists over frames."""
+    eps = 10 * np.finfo(np.float32).eps
+    nFr = len(y)
+    KFC = kf(C, N_u, n_u, dK_u, *kernel)
+    KFCc = kf_conj(np.conj(C), N_u, n_u, dK_u, *kernel)
+    HX = hx_of(N_u, dK_u)
+    HZ = HX
+    HY = [np.minimum(np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1, 1), yy.shape),
+                     ve_max) for v, yy in zip(ve, y)]
+    delta = np.broadcast_to(np.atleast_1d(np.abs(delta)), (nIter,)) if np.size(delta) == 1 \
+        else np.linspace(abs(delta[0]), abs(delta[1]), nIter)
+    rho = np.broadcast_to(np.atleast_1d(np.abs(rho)), (nIter,)) if np.size(rho) == 1 \
+        else np.linspace(abs(rho[0]), abs(rho[1]), nIter)
+    x = [np.array(v, dtype=np.complex128) for v in x]
+    M = lambda v: [shanna(v[i], Gu[i], KFC, N_u, n_u) for i in range(nFr)]
+    Mh = lambda r: [(1.0 / HX) * nakatsha(HY[i] * r[i], Gut[i], KFCc, True, N_u, n_u)
+                    for i in range(nFr)]
+    Fop = lambda v: [v[i] - v[(i - 1) % nFr] for i in range(nFr)]
+    FT = lambda w: [(1.0 / HX) * (HZ * w[i] - HZ * w[(i + 1) % nFr]) for i in range(nFr)]
+    sq = lambda a, H: sum(float(np.real(np.vdot(ai, (H[i] if isinstance(H, list) else H) * ai)))
+                          for i, ai in enumerate(a))
+    z = Fop(x) if z is None else [np.array(v, dtype=np.complex128) for v in z]
+    u = [np.zeros_like(v) for v in x] if u is None else [np.array(v, dtype=np.complex128) for v in u]
+    hist = []
+    for c in range(nIter):
+        dl, rc = delta[c], rho[c]
+        ry = [yy - m for yy, m in zip(y, M(x))]
+        Fx = Fop(x)
+        rz = [(z[i] - u[i]) - Fx[i] for i in range(nFr)]
+        dag = [a + rc * b for a, b in zip(Mh(ry), FT(rz))]
+        p = dag
+        sqn = sq(dag, HX)
+        for j in range(1, nCGD + 1):
+            if sqn < eps:
+                break
+            Mp = M(p)
+            Fp = Fop(p)
+            den = sq(Mp, HY) + sq(Fp, rc * HZ)
+            a = sqn / den
+            x = [xi + a * pi for xi, pi in zip(x, p)]
+            if j == nCGD:
+                break
+            ry = [r - a * m for r, m in zip(ry, Mp)]
+            rz = [r - a * f for r, f in zip(rz, Fp)]
+            dag = [a_ + rc * b_ for a_, b_ in zip(Mh(ry), FT(rz))]
+            sqn_new = sq(dag, HX)
+            b = sqn_new / sqn
+            sqn = sqn_new
+            p = [dg + b * pi for dg, pi in zip(dag, p)]
+        Fx_u = [ui + fi for ui, fi in zip(u, Fop(x))]
+        z = [prox_one_norm(v, dl / rc) for v in Fx_u]
+        u = [a_ - b_ for a_, b_ in zip(Fx_u, z)]
+        dafi = sq([yy - m for yy, m in zip(y, M(x))], HY)
+        regul = sum(HZ * (np.abs(f.real).sum() + np.abs(f.imag).sum()) for f in Fop(x))
+        hist.append((0.5 * dafi + 0.5 * dl * regul, dafi, regul))
+    return x, z, u, hist
+
+
+def nasha(y, Gn, N_u, n_u, dK_u, C=None, kernel=(None, None, None)):
+    """Gridded reconstruction (src/fourier123/map_function/nonCartesian/
+    bmNasha.m:6-91): n2u with Gn, bmIDF (centred ifft times prod(N_u dK_u)),
+    times K, crop, optional coil combine (bmCoilSense_pinv.m:6-12)."""
+    d = len(N_u)
+    K = bmK(N_u, dK_u, *kernel)
+    g = _unflatF(spmv(Gn, y), N_u)
+    g = cifft(g, axes=range(d)) * (np.prod(N_u) * np.prod(dK_u))
+    g = g * K[..., None]
+    g = crop(g, N_u, n_u)
+    if C is not None:
+        g = (np.conj(C) * g).sum(-1) / (np.abs(C) ** 2).sum(-1)
+    return g

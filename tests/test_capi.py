@@ -1,0 +1,38 @@
+"""The C-ABI library loads and exports every entry point include/mona.h
+declares (no device calls)."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "mona.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z_0-9\s\*]*?\b([A-Za-z_][A-Za-z_0-9]*)\s*\(", src, re.M)
+    return sorted(set(n for n in names if n not in ("defined",)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "bmSparseMat_cC_oBlock_omp" in names and "mona_sensa" in names
+    assert len(names) > 30
+
+
+@pytest.mark.parametrize("name", declared_functions())
+def test_symbol_exported(name):
+    from monalisa_amd import lib
+    assert hasattr(lib, name), name
+
+
+def test_binding_table_covers_header():
+    from monalisa_amd._lib import SIGNATURES
+    assert set(declared_functions()) == set(SIGNATURES)
+
+
+def test_no_device_status():
+    from monalisa_amd import lib
+    assert lib.mona_device_count() >= 0
+    assert lib.mona_version().startswith(b"libmona")

@@ -1,0 +1,128 @@
+"""GPU drop-in products vs the reference kernel's golden vectors and the
+oracle's C restatement: bit-exact."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_libs as OL
+from test_oracle_port import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def lib():
+    from monalisa_amd import lib as L
+    return L
+
+
+def gpu_cC(fn, M, vr, vi):
+    n_vec = vr.shape[0]
+    wr = np.full((n_vec, M["l_size"]), np.nan, np.float32)
+    wi = np.full((n_vec, M["l_size"]), np.nan, np.float32)
+    fn(M["r_size"], OL.ip(M["r_jump"]), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]), M["l_size"],
+       OL.ip(M.get("l_jump")), M["l_nJump"], OL.fp(np.ascontiguousarray(vr)), OL.fp(np.ascontiguousarray(vi)),
+       n_vec, OL.fp(wr), OL.fp(wi))
+    return wr, wi
+
+
+@pytest.mark.parametrize("case", load_golden(), ids=lambda c: c[0])
+def test_cC_oBlock_omp_matches_reference_golden(case):
+    name, M, vr, vi, wr, wi = case
+    print(f"[gpu] cC_oBlock_omp golden case {name}", flush=True)
+    L = lib()
+    for fn in (L.bmSparseMat_cC_oBlock_omp, L.bmSparseMat_cC_oBlock, L.bmSparseMat_cC_oBlock_2_omp):
+        gr, gi = gpu_cC(fn, M, vr, vi)
+        assert np.array_equal(gr.view(np.int32), wr.view(np.int32))
+        assert np.array_equal(gi.view(np.int32), wi.view(np.int32))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_all_layouts_bit_exact_vs_port(seed):
+    print(f"[gpu] layouts seed {seed}", flush=True)
+    rng = np.random.default_rng(100 + seed)
+    M = OL.random_matrix(rng, int(rng.integers(2, 500)), int(rng.integers(2, 700)),
+                         int(rng.integers(1, 60)), squeeze=bool(seed % 2))
+    n = int(rng.integers(1, 7))
+    vr = rng.normal(size=(n, M["r_size"])).astype(np.float32)
+    vi = rng.normal(size=(n, M["r_size"])).astype(np.float32)
+    P = OL.port()
+    L = lib()
+    er, ei = OL.call_cC(P.port_cC_oBlock_omp, M, vr, vi)
+    gr, gi = gpu_cC(L.bmSparseMat_cC_oBlock_omp, M, vr, vi)
+    assert np.array_equal(gr, er) and np.array_equal(gi, ei)
+    # row layout
+    vrr, vir = np.ascontiguousarray(vr.T), np.ascontiguousarray(vi.T)
+    rr = np.full((M["l_size"], n), np.nan, np.float32)
+    ri = np.full((M["l_size"], n), np.nan, np.float32)
+    L.bmSparseMat_cR_oBlock(M["r_size"], OL.ip(M["r_jump"]), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]),
+                            M["l_size"], OL.ip(M["l_jump"]), M["l_nJump"], OL.fp(vrr), OL.fp(vir), n,
+                            OL.fp(rr), OL.fp(ri))
+    assert np.array_equal(rr.T, er) and np.array_equal(ri.T, ei)
+    # real column / row layouts
+    rc = np.full((n, M["l_size"]), np.nan, np.float32)
+    L.bmSparseMat_rC_oBlock(M["r_size"], OL.ip(M["r_jump"]), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]),
+                            M["l_size"], OL.ip(M["l_jump"]), M["l_nJump"], OL.fp(vr), n, OL.fp(rc))
+    assert np.array_equal(rc, er)
+    rrow = np.full((M["l_size"], n), np.nan, np.float32)
+    L.bmSparseMat_rR_oBlock(M["r_size"], OL.ip(M["r_jump"]), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]),
+                            M["l_size"], OL.ip(M["l_jump"]), M["l_nJump"], OL.fp(vrr), n, OL.fp(rrow))
+    assert np.array_equal(rrow.T, er)
+
+
+def test_multiblock_row_layout():
+    """cR/rR_nBlock_omp: same sums with the column cursor restarting at each
+    block (first entry of each block absolute), as cpp_prepare('multi_block')
+    writes it (bmSparseMat.m:156-170)."""
+    rng = np.random.default_rng(5)
+    M = OL.random_matrix(rng, 120, 90, 10)
+    n = 2
+    bl = [37, 50, 33]
+    lbs = np.array([0, 37, 87], np.int32)
+    rowptr = np.concatenate([[0], np.cumsum(M["r_nJump"])])
+    mbs = rowptr[lbs].astype(np.int64)
+    col = np.cumsum(M["r_jump"].astype(np.int64))
+    rj = M["r_jump"].copy()
+    for m in mbs:
+        if m < col.size:
+            rj[m] = col[m]
+    vr = rng.normal(size=(M["r_size"], n)).astype(np.float32)
+    vi = rng.normal(size=(M["r_size"], n)).astype(np.float32)
+    er, ei = OL.call_cC(OL.port().port_cC_oBlock_omp, M, np.ascontiguousarray(vr.T), np.ascontiguousarray(vi.T))
+    L = lib()
+    wr = np.full((M["l_size"], n), np.nan, np.float32)
+    wi = np.full((M["l_size"], n), np.nan, np.float32)
+    L.bmSparseMat_cR_nBlock_omp(M["r_size"], OL.ip(rj), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]), M["l_size"],
+                                None, M["l_nJump"], 3, OL.ip(np.array(bl, np.int32)), OL.ip(lbs),
+                                mbs.ctypes.data_as(C.POINTER(C.c_longlong)), OL.fp(vr), OL.fp(vi), n,
+                                OL.fp(wr), OL.fp(wi))
+    assert np.array_equal(wr.T, er) and np.array_equal(wi.T, ei)
+
+
+def test_r_nJump2index_gpu():
+    from monalisa_amd.sparse import bmSparseMat_r_nJump2index
+    r = np.random.default_rng(2).integers(0, 30, size=10000).astype(np.int32)
+    assert np.array_equal(bmSparseMat_r_nJump2index(r), np.repeat(np.arange(10000), r))
+
+
+def test_sparse_apply_device_and_determinism():
+    import torch
+    from monalisa_amd.sparse import BmSparseMat
+    from monalisa_amd._lib import check, ptr, stream_of
+    rng = np.random.default_rng(9)
+    M = OL.random_matrix(rng, 3000, 4000, 30, squeeze=True)
+    S = BmSparseMat(M["r_size"], M["l_size"], M["r_jump"], M["r_nJump"], M["m_val"], M["l_jump"])
+    n = 4
+    v = (rng.normal(size=(M["r_size"], n)) + 1j * rng.normal(size=(M["r_size"], n))).astype(np.complex64)
+    vd = torch.from_numpy(v).cuda()
+    w1 = torch.empty((M["l_size"], n), dtype=torch.complex64, device="cuda")
+    w2 = torch.empty_like(w1)
+    L = lib()
+    check(L.mona_sparse_apply(S.handle(), ptr(vd), ptr(w1), n, stream_of()))
+    check(L.mona_sparse_apply(S.handle(), ptr(vd), ptr(w2), n, stream_of()))
+    torch.cuda.synchronize()
+    a = w1.cpu().numpy()
+    assert np.array_equal(a.view(np.int32), w2.cpu().numpy().view(np.int32))
+    er, ei = OL.call_cC(OL.port().port_cC_oBlock_omp, M, np.ascontiguousarray(v.real.T),
+                        np.ascontiguousarray(v.imag.T))
+    assert np.array_equal(a.real.T, er) and np.array_equal(a.imag.T, ei)
