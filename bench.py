@@ -1,0 +1,259 @@
+"""Benchmark: 3-D golden-angle koosh-ball CG-SENSE, N=128 (256^3 grid),
+10,000 spokes x 256 samples (2.56M points), 20 coils, KB width 3, 20 CG
+iterations (bmSensa with nIter=1, nCGD=20), FP32 vectors with FP64 CG
+scalars, on one MI355X.
+
+Prints ONE JSON line (rank 0).  value = CG-SENSE reconstructions per second
+with all inputs resident in HBM.  Extra fields: NUFFT pairs/s, the roofline
+of the dominant gridding kernel (HIP events around every launch in the timed
+region), and the CPU baseline (the reference's own sparse kernel built from
+its sources, plus scipy FFT standing in for MATLAB's fft) timed on this
+host on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # the workload the metric is quoted on
+    "c1": dict(n=128, nCh=20, nSeg=25, nShot=400, nSample=256, nCGD=20),
+    # small shape for quick checks
+    "small": dict(n=64, nCh=8, nSeg=10, nShot=200, nSample=128, nCGD=10),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(cfg, seed=1234):
+    import torch
+    from monalisa_amd import synth
+    from monalisa_amd.nufft import Plan
+    from monalisa_amd.sparse import bmTraj2SparseMat
+    rng = np.random.default_rng(seed)
+    n, nCh = cfg["n"], cfg["nCh"]
+    n_u, N_u, dK_u = [n] * 3, [2 * n] * 3, [0.5] * 3
+    nSpoke = cfg["nSeg"] * cfg["nShot"]
+    t0 = time.time()
+    t = synth.koosh_ball(cfg["nSeg"], cfg["nShot"], cfg["nSample"], k_max=n / 2)
+    ve = synth.ve_radial3(t, cfg["nSample"], nSpoke)
+    G = bmTraj2SparseMat(t, ve, N_u, dK_u, "kaiser")
+    log(f"[bench] trajectory + Gu/Gut ({t.shape[1]} points, nnz {G['Gu'].nnz}) in {time.time() - t0:.1f}s")
+    C = synth.coil_maps(n_u, nCh, rng)
+    phantom = synth.ellipsoid_phantom(n_u, rng)
+    P = Plan(N_u, n_u, dK_u, nCh, "kaiser")
+    Cd = torch.from_numpy(C).cuda()
+    P.set_coils(Cd)
+    P.set_frame(0, G["Gu"], G["Gut"])
+    xd = torch.from_numpy(phantom).cuda()
+    y = P.bmShanna(xd)
+    # complex Gaussian noise at SNR 30 dB, seeded on the device
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    rms = torch.sqrt(torch.mean(torch.abs(y) ** 2)).item()
+    sigma = rms * 10 ** (-30 / 20) / math.sqrt(2)
+    noise = torch.complex(torch.randn(y.shape, generator=gen, device="cuda"),
+                          torch.randn(y.shape, generator=gen, device="cuda")) * sigma
+    y = (y + noise).contiguous()
+    ve_d = torch.from_numpy(ve.astype(np.float32)).cuda()
+    torch.cuda.synchronize()
+    return dict(P=P, G=G, y=y, ve=ve_d, ve_np=ve, C=C, n_u=n_u, N_u=N_u, dK_u=dK_u, nCh=nCh,
+                phantom=phantom, xd=xd, P_pts=t.shape[1])
+
+
+def gridding_bytes(prob):
+    """Algorithmic HBM bytes per launch of the two gridding products
+    (jump-CSR form; complex64 = 8 B, int32 col + float val = 8 B per entry,
+    int64 row pointer 8 B per stored row, int32 row index 4 B)."""
+    Gu, Gut = prob["G"]["Gu"], prob["G"]["Gut"]
+    nCh = prob["nCh"]
+    P = Gu.l_size
+    U = Gut.l_nJump
+    G = int(np.prod(prob["N_u"]))
+    nnz = Gu.nnz
+    u2n = 8 * nCh * (U + P) + 8 * nnz + 8 * (P + 1)
+    n2u = 8 * nCh * (P + G) + 4 * P + 8 * nnz + 8 * (U + 1) + 4 * U
+    return u2n, n2u, U
+
+
+def cpu_baseline(prob, threads):
+    """One forward and one adjoint at the full workload on the host cores:
+    sparse products by the reference's own bmSparseMat_cC_oBlock_omp (built
+    from its sources into oracle/_ref; the oracle's C restatement when that
+    build is absent), centred FFTs by scipy.fft (workers = threads),
+    elementwise by numpy.  A CG-SENSE recon costs 22 forwards + 20 adjoints."""
+    import ctypes as C
+    import scipy.fft as sfft
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_libs as OL
+    from oracle import mona_oracle as O
+    ref = OL.ref()
+    kind = "reference" if ref is not None else "port"
+    fn = ref.ref_cC_oBlock_omp if ref is not None else OL.port().port_cC_oBlock_omp
+    P = prob["P"]
+    Gu, Gut = prob["G"]["Gu"], prob["G"]["Gut"]
+    n_u, N_u, nCh = prob["n_u"], prob["N_u"], prob["nCh"]
+    KFC, KFCc = [a.cpu().numpy() for a in P.KFC()]
+    x = prob["phantom"]
+    Mu = dict(r_size=Gu.r_size, l_size=Gu.l_size, r_jump=Gu.r_jump, r_nJump=Gu.r_nJump, m_val=Gu.m_val,
+              l_nJump=Gu.l_nJump, l_jump=None)
+    Mt = dict(r_size=Gut.r_size, l_size=Gut.l_size, r_jump=Gut.r_jump, r_nJump=Gut.r_nJump,
+              m_val=Gut.m_val, l_nJump=Gut.l_nJump, l_jump=Gut.l_jump)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+
+    def cfft_cpu(g, inverse):
+        axes = (1, 2, 3)
+        g = sfft.ifftshift(g, axes=axes)
+        g = (sfft.ifftn if inverse else sfft.fftn)(g, axes=axes, workers=threads, overwrite_x=True)
+        return sfft.fftshift(g, axes=axes)
+
+    # forward: x.*KFC -> zero-fill -> cfft -> Gu
+    t0 = time.perf_counter()
+    xc = (x[:, None] * KFC).T.reshape((nCh,) + tuple(n_u[::-1]))   # [c, z, y, x]
+    g = np.zeros((nCh,) + tuple(N_u[::-1]), np.complex64)
+    sl = tuple(slice(N // 2 - n // 2, N // 2 - n // 2 + n) for N, n in zip(N_u[::-1], n_u[::-1]))
+    g[(slice(None),) + sl] = xc
+    g = cfft_cpu(g, False).astype(np.complex64)
+    gr = np.ascontiguousarray(g.real.reshape(nCh, -1))
+    gi = np.ascontiguousarray(g.imag.reshape(nCh, -1))
+    del g
+    yr, yi = OL.call_cC(fn, Mu, gr, gi)
+    t_fwd = time.perf_counter() - t0
+    del gr, gi
+    # adjoint: Gut -> cifft -> crop -> .*KFC_conj -> coil sum
+    t0 = time.perf_counter()
+    wr, wi = OL.call_cC(fn, Mt, yr, yi)
+    g = (wr + 1j * wi).astype(np.complex64).reshape((nCh,) + tuple(N_u[::-1]))
+    del wr, wi
+    g = cfft_cpu(g, True)
+    g = g[(slice(None),) + sl].reshape(nCh, -1).T
+    xa = (g * KFCc).sum(1)
+    t_adj = time.perf_counter() - t0
+    recon_s = 22 * t_fwd + 20 * t_adj
+    return dict(value=1.0 / recon_s, unit="recons/s", cores=threads, kind=kind,
+                sample=(f"one forward ({t_fwd:.2f} s) + one adjoint ({t_adj:.2f} s) at the full workload, "
+                        f"scaled to a 20-iteration CG-SENSE (22 forward + 20 adjoint, vector updates "
+                        f"not counted); sparse products: {'the reference kernel built from its sources' if kind == 'reference' else 'the oracle C restatement'} "
+                        f"(OpenMP, min(2*nCh, threads) threads), FFT: scipy.fft with {threads} workers"),
+                nufft_pairs_per_s=1.0 / (t_fwd + t_adj))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--pairs", type=int, default=10, help="NUFFT pairs timed after the recons")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from monalisa_amd import nufft
+    cfg = CONFIGS[args.config]
+    prob = build_problem(cfg)
+    P, y, ve = prob["P"], prob["y"], prob["ve"]
+    nu = int(np.prod(prob["n_u"]))
+    stream = torch.cuda.current_stream()
+
+    def recon():
+        x = torch.zeros(nu, dtype=torch.complex64, device="cuda")
+        P.bmSensa(x, y, ve, cfg["nCGD"], nIter=1, stream=stream)
+        return x
+
+    for _ in range(args.warmup):
+        recon()
+    torch.cuda.synchronize()
+    nufft.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x = recon()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_u2n, ms_u2n = nufft.timing(kind=0)
+    n_n2u, ms_n2u = nufft.timing(kind=1)
+    n_fft, ms_fft = nufft.timing(kind=2)
+    n_ew, ms_ew = nufft.timing(kind=3)
+    nufft.timing(False)
+
+    # NUFFT pairs (forward + adjoint) after the timed recons
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.pairs):
+        yy = P.bmShanna(x)
+        P.bmNakatsha(yy)
+    torch.cuda.synchronize()
+    pair_s = (time.perf_counter() - t1) / max(args.pairs, 1)
+
+    # image quality vs the phantom (sanity, not a parity claim)
+    xr = x.cpu().numpy()
+    ph = prob["phantom"]
+    a = np.vdot(xr, ph) / np.vdot(xr, xr)
+    img_err = float(np.linalg.norm(a * xr - ph) / np.linalg.norm(ph))
+
+    u2n_b, n2u_b, U = gridding_bytes(prob)
+    avg_u2n = ms_u2n / max(n_u2n, 1)
+    avg_n2u = ms_n2u / max(n_n2u, 1)
+    if ms_n2u >= ms_u2n:
+        dom, bytes_, avg = "n2u (Gut product, jump-CSR gather)", n2u_b, avg_n2u
+    else:
+        dom, bytes_, avg = "u2n (Gu product, jump-CSR gather)", u2n_b, avg_u2n
+    achieved = bytes_ / (avg * 1e-3) / 1e9
+    ms_per = elapsed / args.steps * 1e3
+    out = {
+        "metric": "NUFFT pairs/s + CG-SENSE recons/s at 3D N=128 20ch; p50/p99 served latency",
+        "value": args.steps / elapsed,
+        "unit": "recons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 vectors, fp64 CG scalars)",
+        "data": "synthetic (seeded koosh-ball trajectory, Gaussian-blob coil maps, ellipsoid phantom, 30 dB noise)",
+        "config": {"workload": f"3D koosh-ball CG-SENSE n_u={cfg['n']}^3 N_u={2 * cfg['n']}^3 "
+                               f"{cfg['nSeg'] * cfg['nShot']} spokes x {cfg['nSample']} samples "
+                               f"({prob['P_pts']} points), {cfg['nCh']} coils, KB nWin=3, "
+                               f"{cfg['nCGD']} CG iterations",
+                   "parallelism": "single GPU"},
+        "nufft_pairs_per_s": 1.0 / pair_s,
+        "kernel_ms_per_recon": {"u2n": ms_u2n / args.steps, "n2u": ms_n2u / args.steps,
+                                "fft": ms_fft / args.steps, "elementwise": ms_ew / args.steps},
+        "launches": {"u2n": n_u2n, "n2u": n_n2u},
+        "image_rel_err_vs_phantom": img_err,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "bytes_per_launch": bytes_, "avg_launch_ms": avg,
+                     "u2n_GBps": u2n_b / (avg_u2n * 1e-3) / 1e9 if n_u2n else None,
+                     "n2u_GBps": n2u_b / (avg_n2u * 1e-3) / 1e9 if n_n2u else None,
+                     "touched_cells_U": U},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        try:
+            cb = cpu_baseline(prob, threads)
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu"] = out["value"] / cb["value"]
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

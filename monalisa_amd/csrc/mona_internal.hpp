@@ -126,6 +126,14 @@ struct SparseDev {
     DevBuf val;               // float[nnz]
     DevBuf rowptr;            // int64[l_nJump+1]
     DevBuf rowidx;            // int32[l_nJump] output row (only if squeezed)
+    // rows longer than kLongRow entries, split into chunks for the operator
+    // products (fixed-order partial sums; see spmv_c2_rowmajor)
+    int n_long = 0;
+    int64_t n_chunk = 0;
+    DevBuf long_row;          // int32[n_long]   stored-row index
+    DevBuf long_first;        // int32[n_long+1] first chunk of each long row
+    DevBuf chunk_begin;       // int64[2*n_chunk] [begin, end) entry offsets of each chunk
+    mutable DevBuf partial;   // float2[n_chunk * n_vec] scratch
     // host copy of the reference encoding (kept for export; decoded onto the
     // device on first use)
     std::vector<int> h_r_jump, h_r_nJump, h_l_jump;
@@ -143,10 +151,16 @@ void sparse_set_host(SparseDev& S, int r_size, std::vector<int>&& r_jump, std::v
 // Uploads and decodes on the current device (idempotent, thread-safe).
 void sparse_upload(SparseDev& S, hipStream_t s);
 
+// Rows with more entries than this are summed in chunks by the operator path.
+constexpr int kLongRow = 64;
+
 // w = S v for complex vectors stored interleaved (float2) with `n_vec`
 // vectors, entry (row, k) at base[row * n_vec + k] (vector index fastest).
-// Rows of w not stored in S are set to 0.  Sequential row sums in stored
-// order, no FMA: bit-identical to the reference CPU kernel.
+// Rows of w not stored in S are set to 0.  Rows of up to kLongRow entries
+// are summed sequentially in stored order without FMA (bit-identical to the
+// reference CPU kernel); longer rows are summed as consecutive chunks of
+// kLongRow entries whose partial sums are then added in chunk order
+// (deterministic; differs from the sequential sum only by rounding).
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
                       const float* yscale, hipStream_t s);
 

@@ -127,8 +127,10 @@ spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
     int64_t total = (int64_t)nrows * n_vec;
     if (t >= total) return;
     int i = (int)(t / n_vec), k = (int)(t - (int64_t)i * n_vec);
+    int64_t jb = rowptr[i], je = rowptr[i + 1];
+    if (je - jb > kLongRow) return;  // summed by spmv_c2_chunks + spmv_c2_combine
     float ar = 0.0f, ai = 0.0f;
-    for (int64_t j = rowptr[i], e = rowptr[i + 1]; j < e; j++) {
+    for (int64_t j = jb; j < je; j++) {
         int cj = col[j];
         float m = val[j];
         float2 x = v[(int64_t)cj * n_vec + k];
@@ -137,6 +139,49 @@ spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
         ar = ar + pr;
         ai = ai + pi;
     }
+    int row = rowidx ? rowidx[i] : i;
+    w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
+}
+
+// Partial sums of the chunks of long rows: chunk c covers entries
+// [chunk_be[2c], chunk_be[2c+1]).
+template <bool SCALE>
+__global__ void __launch_bounds__(256)
+spmv_c2_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const int* __restrict__ col,
+               const float* __restrict__ val, int n_vec, const float2* __restrict__ v,
+               const float* __restrict__ scale, float2* __restrict__ partial) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_chunk * n_vec) return;
+    int64_t c = t / n_vec;
+    int k = (int)(t - c * n_vec);
+    float ar = 0.0f, ai = 0.0f;
+    for (int64_t j = chunk_be[2 * c], e = chunk_be[2 * c + 1]; j < e; j++) {
+        int cj = col[j];
+        float m = val[j];
+        float2 x = v[(int64_t)cj * n_vec + k];
+        if (SCALE) { float s = scale[cj]; x.x = s * x.x; x.y = s * x.y; }
+        float pr = m * x.x, pi = m * x.y;
+        ar = ar + pr;
+        ai = ai + pi;
+    }
+    partial[t] = make_float2(ar, ai);
+}
+
+// Adds the chunk partials of each long row in chunk order.
+__global__ void __launch_bounds__(256)
+spmv_c2_combine(const int* __restrict__ long_row, const int* __restrict__ long_first, int n_long,
+                const int* __restrict__ rowidx, int n_vec, const float2* __restrict__ partial,
+                float2* __restrict__ w) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_long * n_vec) return;
+    int r = (int)(t / n_vec), k = (int)(t - (int64_t)r * n_vec);
+    float ar = 0.0f, ai = 0.0f;
+    for (int c = long_first[r], e = long_first[r + 1]; c < e; c++) {
+        float2 p = partial[(int64_t)c * n_vec + k];
+        ar = ar + p.x;
+        ai = ai + p.y;
+    }
+    int i = long_row[r];
     int row = rowidx ? rowidx[i] : i;
     w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
 }
@@ -275,6 +320,37 @@ void sparse_upload(SparseDev& S, hipStream_t s) {
         MONA_HIP(hipMemcpyAsync(d_lj.p, S.h_l_jump.data(), (size_t)l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
     }
     decode_device(S, d_rj.as<int>(), d_rn.as<int>(), d_mv.as<float>(), d_lj.as<int>(), 1, nullptr, s);
+    // chunk lists of the long rows: chunk c covers entries
+    // [chunk_be[2c], chunk_be[2c+1]); long row r owns chunks
+    // long_first[r] .. long_first[r+1]-1
+    {
+        std::vector<int> lrow, first;
+        std::vector<int64_t> be;
+        int64_t off = 0;
+        for (int i = 0; i < l_nJump; i++) {
+            int n = S.h_r_nJump[i];
+            if (n > kLongRow) {
+                lrow.push_back(i);
+                first.push_back((int)(be.size() / 2));
+                for (int j = 0; j < n; j += kLongRow) {
+                    be.push_back(off + j);
+                    be.push_back(off + std::min(n, j + kLongRow));
+                }
+            }
+            off += n;
+        }
+        first.push_back((int)(be.size() / 2));
+        S.n_long = (int)lrow.size();
+        S.n_chunk = (int64_t)(be.size() / 2);
+        if (S.n_long > 0) {
+            S.long_row.alloc(lrow.size() * sizeof(int));
+            S.long_first.alloc(first.size() * sizeof(int));
+            S.chunk_begin.alloc(be.size() * sizeof(int64_t));
+            MONA_HIP(hipMemcpyAsync(S.long_row.p, lrow.data(), lrow.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            MONA_HIP(hipMemcpyAsync(S.long_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            MONA_HIP(hipMemcpyAsync(S.chunk_begin.p, be.data(), be.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        }
+    }
     MONA_HIP(hipStreamSynchronize(s));
     S.on_device = true;
 }
@@ -296,6 +372,23 @@ void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
                                                               S.val.as<float>(), ri, S.l_nJump, n_vec,
                                                               v, w, nullptr);
     MONA_HIP(hipGetLastError());
+    if (S.n_long > 0) {
+        S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+        int64_t tc = S.n_chunk * n_vec;
+        const int64_t* cb = S.chunk_begin.as<int64_t>();
+        float2* part = S.partial.as<float2>();
+        if (yscale)
+            spmv_c2_chunks<true><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, S.col.as<int>(),
+                                                                     S.val.as<float>(), n_vec, v, yscale, part);
+        else
+            spmv_c2_chunks<false><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, S.col.as<int>(),
+                                                                      S.val.as<float>(), n_vec, v, nullptr, part);
+        MONA_HIP(hipGetLastError());
+        int64_t tl = (int64_t)S.n_long * n_vec;
+        spmv_c2_combine<<<blocks_for(tl, 256), 256, 0, s>>>(S.long_row.as<int>(), S.long_first.as<int>(),
+                                                            S.n_long, ri, n_vec, part, w);
+        MONA_HIP(hipGetLastError());
+    }
 }
 
 // -------------------------------------------------------------- host-pointer products

@@ -1,0 +1,105 @@
+"""GPU FFT, deapodisation and operators against the float64 oracle."""
+import numpy as np
+import pytest
+
+from problems import make_problem, oracle_mats, nd, flat, rel
+from oracle import mona_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,nb", [((8,), 3), ((16, 12), 2), ((8, 6, 10), 2), ((64, 64), 1),
+                                  ((32, 32, 32), 2), ((192,), 5), ((24, 40), 20), ((30, 20), 1)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_cfft_vs_numpy(N, nb, inverse):
+    import torch
+    from monalisa_amd.nufft import cfft
+    print(f"[gpu] cfft N={N} nb={nb} inv={inverse}", flush=True)
+    rng = np.random.default_rng(sum(N) + nb)
+    a = (rng.normal(size=(int(np.prod(N)), nb)) + 1j * rng.normal(size=(int(np.prod(N)), nb))).astype(np.complex64)
+    d = torch.from_numpy(a).cuda()
+    cfft(d, N, inverse)
+    got = d.cpu().numpy()
+    ref = nd(a.astype(np.complex128), N, nb)
+    ref = (O.cifft if inverse else O.cfft)(ref, axes=range(len(N)))
+    assert rel(got, flat(ref, N)) < 2e-6
+
+
+def _plan(p):
+    import torch
+    from monalisa_amd.nufft import Plan
+    from monalisa_amd.sparse import bmTraj2SparseMat
+    G = bmTraj2SparseMat(p["t"], p["ve"], p["N_u"], p["dK_u"], p["kernel"])
+    P = Plan(p["N_u"], p["n_u"], p["dK_u"], p["nCh"], p["kernel"])
+    P.set_coils(torch.from_numpy(p["C"]).cuda())
+    P.set_frame(0, G["Gu"], G["Gut"])
+    return P
+
+
+@pytest.mark.parametrize("kind", ["2d", "3d", "2d_rect"])
+@pytest.mark.parametrize("kernel", ["kaiser", "gauss"])
+def test_deapodisation_vs_oracle(kind, kernel):
+    p = make_problem(kind, kernel=kernel)
+    print(f"[gpu] deapod {kind} {kernel}", flush=True)
+    P = _plan(p)
+    K = P.K().cpu().numpy()
+    Kref = O.crop(O.bmK(p["N_u"], p["dK_u"], kernel), p["N_u"], p["n_u"])
+    assert rel(K, flat(Kref, p["n_u"])) < 1e-6
+    KFC, KFCc = [a.cpu().numpy() for a in P.KFC()]
+    Cn = nd(p["C"], p["n_u"], p["nCh"])
+    assert rel(KFC, flat(O.kf(Cn, p["N_u"], p["n_u"], p["dK_u"], kernel), p["n_u"])) < 1e-6
+    assert rel(KFCc, flat(O.kf_conj(np.conj(Cn), p["N_u"], p["n_u"], p["dK_u"], kernel), p["n_u"])) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["2d", "3d", "2d_rect"])
+def test_forward_adjoint_vs_oracle(kind):
+    import torch
+    p = make_problem(kind)
+    print(f"[gpu] forward/adjoint {kind}", flush=True)
+    P = _plan(p)
+    R = oracle_mats(p)
+    Cn = nd(p["C"], p["n_u"], p["nCh"])
+    KFC = O.kf(Cn, p["N_u"], p["n_u"], p["dK_u"], p["kernel"])
+    KFCc = O.kf_conj(np.conj(Cn), p["N_u"], p["n_u"], p["dK_u"], p["kernel"])
+    x = torch.from_numpy(p["x"]).cuda()
+    y = P.bmShanna(x).cpu().numpy()
+    yref = O.shanna(nd(p["x"].astype(np.complex128), p["n_u"]), R["Gu"], KFC, p["N_u"], p["n_u"])
+    assert rel(y, yref) < 1e-5
+    rng = np.random.default_rng(1)
+    yy = (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape)).astype(np.complex64)
+    xa = P.bmNakatsha(torch.from_numpy(yy).cuda()).cpu().numpy()
+    xref = O.nakatsha(yy.astype(np.complex128), R["Gut"], KFCc, True, p["N_u"], p["n_u"])
+    assert rel(xa, flat(xref, p["n_u"])) < 1e-5
+    xc = P.bmNakatsha(torch.from_numpy(yy).cuda(), C_flag=False).cpu().numpy()
+    xcref = O.nakatsha(yy.astype(np.complex128), R["Gut"], KFCc, False, p["N_u"], p["n_u"])
+    assert rel(xc, flat(xcref, p["n_u"])) < 1e-5
+
+
+def test_adjointness_gpu():
+    """<A x, y> = <x, A^H y> * (prod dK_u) relation of the operator pair:
+    KFC_conj = conj(KFC) * prod(N_u) and the inverse FFT carries 1/prod(N_u),
+    so <M x, y> = <x, M^H y> exactly (bmKF.m:27 vs bmKF_conj.m:30)."""
+    import torch
+    p = make_problem("3d", nCh=2)
+    P = _plan(p)
+    rng = np.random.default_rng(4)
+    x = (rng.normal(size=p["x"].shape) + 1j * rng.normal(size=p["x"].shape)).astype(np.complex64)
+    y = P.bmShanna(torch.from_numpy(x).cuda()).cpu().numpy()
+    yy = (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape)).astype(np.complex64)
+    xa = P.bmNakatsha(torch.from_numpy(yy).cuda()).cpu().numpy()
+    lhs = np.vdot(yy.astype(np.complex128), y.astype(np.complex128))
+    rhs = np.vdot(xa.astype(np.complex128), x.astype(np.complex128))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_operators_deterministic():
+    import torch
+    p = make_problem("2d")
+    P = _plan(p)
+    x = torch.from_numpy(p["x"]).cuda()
+    a = P.bmShanna(x).cpu().numpy()
+    b = P.bmShanna(x).cpu().numpy()
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    ya = P.bmNakatsha(torch.from_numpy(a).cuda()).cpu().numpy()
+    yb = P.bmNakatsha(torch.from_numpy(a).cuda()).cpu().numpy()
+    assert np.array_equal(ya.view(np.int32), yb.view(np.int32))
