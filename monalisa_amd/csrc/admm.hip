@@ -1,23 +1,477 @@
-// ADMM total-variation solvers and the gridded reconstruction (filled in
-// below the operator layer).
+// ADMM total-variation solvers and the gridded reconstruction.
+//
+//   bmSteva               spatial TV, one frame   (bmSteva.m:6-138)
+//   bmTevaMorphosia_chain temporal TV over frames (bmTevaMorphosia_chain.m:6-294, Tu = [])
+//   bmNasha               n2u with Gn + centred ifft + deapodisation (bmNasha.m:6-91)
+//
+// The finite differences, their transposes and the soft-threshold prox are
+// fused elementwise kernels; inner CG scalars are FP64 on the device.
+#include <algorithm>
+#include <cmath>
+
+#include "fft.hpp"
 #include "recon.hpp"
+
+namespace mona {
+
+namespace {
+
+inline unsigned nb(int64_t n) { return blocks_for(n, 256); }
+
+struct Dims {
+    int n[3];
+    float inv_dx[3];
+    int imDim;
+    int64_t nu;
+};
+
+__device__ __forceinline__ void coords(int64_t i, const Dims& d, int c[3]) {
+    c[0] = (int)(i % d.n[0]);
+    c[1] = (int)((i / d.n[0]) % d.n[1]);
+    c[2] = (int)(i / ((int64_t)d.n[0] * d.n[1]));
+}
+
+__device__ __forceinline__ int64_t stride_of(const Dims& d, int a) {
+    return a == 0 ? 1 : (a == 1 ? (int64_t)d.n[0] : (int64_t)d.n[0] * d.n[1]);
+}
+
+// g[a][i] = (x[i] - x[i - e_a (cyclic)]) / dX_a, 0 on the first slice of
+// axis a (bmBackGradient_n.m:36-44).  With zu != NULL writes
+// out = (z - u) - g instead (the ADMM residual res_z).
+__global__ void k_grad(const float2* __restrict__ x, float2* __restrict__ out, const float2* __restrict__ z,
+                       const float2* __restrict__ u, Dims d) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nu) return;
+    int c[3];
+    coords(i, d, c);
+    float2 xi = x[i];
+    for (int a = 0; a < d.imDim; a++) {
+        float2 g = make_float2(0.f, 0.f);
+        if (c[a] != 0) {
+            float2 xm = x[i - stride_of(d, a)];
+            g = make_float2((xi.x - xm.x) * d.inv_dx[a], (xi.y - xm.y) * d.inv_dx[a]);
+        }
+        int64_t o = (int64_t)a * d.nu + i;
+        if (z) {
+            float2 zz = z[o], uu = u[o];
+            out[o] = make_float2((zz.x - uu.x) - g.x, (zz.y - uu.y) - g.y);
+        } else {
+            out[o] = g;
+        }
+    }
+}
+
+// out[i] (+)= coef * sum_a (w_a[i] * [c_a != 0] - w_a[i + e_a (cyclic)]) / dX_a
+// (bmBackGradient_nT.m:34-44 summed over axes by bmBackGradientT.m).
+__global__ void k_gradT(const float2* __restrict__ w, float2* __restrict__ out, Dims d, float coef, int accumulate) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nu) return;
+    int c[3];
+    coords(i, d, c);
+    float2 s = make_float2(0.f, 0.f);
+    for (int a = 0; a < d.imDim; a++) {
+        const float2* wa = w + (int64_t)a * d.nu;
+        float2 gi = c[a] != 0 ? wa[i] : make_float2(0.f, 0.f);
+        int64_t j = (c[a] == d.n[a] - 1) ? i - (int64_t)(d.n[a] - 1) * stride_of(d, a) : i + stride_of(d, a);
+        float2 wn = wa[j];
+        s.x += (gi.x - wn.x) * d.inv_dx[a];
+        s.y += (gi.y - wn.y) * d.inv_dx[a];
+    }
+    s.x *= coef;
+    s.y *= coef;
+    if (accumulate) {
+        float2 o = out[i];
+        s.x += o.x;
+        s.y += o.y;
+    }
+    out[i] = s;
+}
+
+__device__ __forceinline__ float soft(float v, float s) {
+    return (v > s) ? (v - s) : ((v < -s) ? (v + s) : 0.0f);
+}
+
+// w = Fx + u ; z = prox_{s}(w) (re and im separately, bmProx_oneNorm.m:30-41) ; u = w - z
+__global__ void k_prox_update(const float2* __restrict__ Fx, float2* __restrict__ z, float2* __restrict__ u,
+                              int64_t n, float s) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 f = Fx[i], uu = u[i];
+    float2 w = make_float2(uu.x + f.x, uu.y + f.y);
+    float2 zz = make_float2(soft(w.x, s), soft(w.y, s));
+    z[i] = zz;
+    u[i] = make_float2(w.x - zz.x, w.y - zz.y);
+}
+
+// Temporal difference over cyclic frames: out_f = x_f - x_{f-1}
+// (bmTevaMorphosia_chain.m:266-272 with identity deformation).  With z, u:
+// out = (z - u) - (x_f - x_{f-1}).
+__global__ void k_tdiff(const float2* __restrict__ x, float2* __restrict__ out, const float2* __restrict__ z,
+                        const float2* __restrict__ u, int nFr, int64_t nu) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nFr * nu) return;
+    int f = (int)(t / nu);
+    int64_t i = t - (int64_t)f * nu;
+    int fm = (f + nFr - 1) % nFr;
+    float2 a = x[t], b = x[(int64_t)fm * nu + i];
+    float2 g = make_float2(a.x - b.x, a.y - b.y);
+    if (z) {
+        float2 zz = z[t], uu = u[t];
+        out[t] = make_float2((zz.x - uu.x) - g.x, (zz.y - uu.y) - g.y);
+    } else {
+        out[t] = g;
+    }
+}
+
+// out_f += coef * (z_f - z_{f+1})   (private_dagF, bmTevaMorphosia_chain.m:285-291)
+__global__ void k_tdiffT_acc(const float2* __restrict__ z, float2* __restrict__ out, int nFr, int64_t nu,
+                             float coef) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nFr * nu) return;
+    int f = (int)(t / nu);
+    int64_t i = t - (int64_t)f * nu;
+    int fp = (f + 1) % nFr;
+    float2 a = z[t], b = z[(int64_t)fp * nu + i];
+    float2 o = out[t];
+    out[t] = make_float2(o.x + coef * (a.x - b.x), o.y + coef * (a.y - b.y));
+}
+
+constexpr int kRT = 256;
+constexpr int kRB = 1024;
+
+// FP64 sum of |Re| + |Im|
+__global__ void __launch_bounds__(kRT) k_abs1_partial(const float2* a, int64_t n, double* partial) {
+    __shared__ double sh[kRT];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+        float2 v = a[i];
+        s += fabs((double)v.x) + fabs((double)v.y);
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kRT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kRT) k_sum_partial(const double* partial, int np, double* out, double scale) {
+    __shared__ double sh[kRT];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += kRT) s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kRT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0] * scale;
+}
+
+void abs1(const float2* a, int64_t n, double scale, double* out, double* partial, hipStream_t s) {
+    int nbk = (int)std::min<int64_t>(kRB, std::max<int64_t>(1, (n + kRT - 1) / kRT));
+    k_abs1_partial<<<nbk, kRT, 0, s>>>(a, n, partial);
+    MONA_HIP(hipGetLastError());
+    k_sum_partial<<<1, kRT, 0, s>>>(partial, nbk, out, scale);
+    MONA_HIP(hipGetLastError());
+}
+
+// objective = 0.5 * dafi + 0.5 * delta * regul
+__global__ void k_objective(double* h, double delta) { h[0] = 0.5 * h[1] + 0.5 * delta * h[2]; }
+
+// Gridded-recon epilogue (bmNasha.m:66-87): per-coil image
+// v_c = grid[crop(i)][c] * F * K[i]; combined sum(conj(C) v)/sum|C|^2.
+__global__ void k_nasha_epilogue(const float2* __restrict__ grid, const float* __restrict__ K,
+                                 const float2* __restrict__ C, float2* __restrict__ x, Geom g, float F) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nu) return;
+    int ix = (int)(i % g.n[0]);
+    int iy = (int)((i / g.n[0]) % g.n[1]);
+    int iz = (int)(i / ((int64_t)g.n[0] * g.n[1]));
+    int64_t cell = (ix + g.off[0]) + (int64_t)g.N[0] * ((iy + g.off[1]) + (int64_t)g.N[1] * (iz + g.off[2]));
+    const float2* gp = grid + cell * g.nCh;
+    float k = K[i];
+    if (C) {
+        float2 num = make_float2(0.f, 0.f);
+        float den = 0.f;
+        for (int c = 0; c < g.nCh; c++) {
+            float2 v = gp[c];
+            v = make_float2(v.x * F * k, v.y * F * k);
+            float2 cc = C[i * g.nCh + c];
+            num.x += cc.x * v.x + cc.y * v.y;     // conj(C) * v
+            num.y += cc.x * v.y - cc.y * v.x;
+            den += cc.x * cc.x + cc.y * cc.y;
+        }
+        x[i] = make_float2(num.x / den, num.y / den);
+    } else {
+        for (int c = 0; c < g.nCh; c++) {
+            float2 v = gp[c];
+            x[i * g.nCh + c] = make_float2(v.x * F * k, v.y * F * k);
+        }
+    }
+}
+
+std::vector<float> schedule(const float* v, int nIter) {
+    // linspace(v0, v1, nIter) in single (private_init_delta_rho, bmSteva.m:140-157)
+    std::vector<float> out(std::max(nIter, 1));
+    double a = std::fabs((double)v[0]), b = std::fabs((double)v[1]);
+    for (int i = 0; i < nIter; i++)
+        out[i] = (float)(nIter == 1 ? b : a + (b - a) * (double)i / (double)(nIter - 1));
+    return out;
+}
+
+}  // namespace
+
+// ================================================================ bmSteva
+void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const float2* y, const float* ve,
+           float ve_max, const float* delta_in, const float* rho_in, int nIter, int nCGD, double* hist_out,
+           hipStream_t s) {
+    MONA_CHECK(nIter >= 0 && nCGD >= 0, MONA_ERR_ARG, "nIter and nCGD must be >= 0");
+    const Frame& fr = P.frame(f);
+    int64_t Pn = fr.Gu->l_size, ny = Pn * P.nCh, nu = P.nu;
+    int D = P.imDim;
+    if (!(ve_max > 0)) ve_max = device_max(ve, Pn, s);
+    std::vector<float> delta = schedule(delta_in, nIter), rho = schedule(rho_in, nIter);
+    Dims d{};
+    for (int a = 0; a < 3; a++) {
+        d.n[a] = P.n[a];
+        d.inv_dx[a] = a < D ? 1.0f / ((1.0f / (float)P.dK[a]) / (float)P.N[a]) : 1.0f;
+    }
+    d.imDim = D;
+    d.nu = nu;
+    Work& w = P.work;
+    w.hy.ensure((size_t)Pn * sizeof(float));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.i1.ensure((size_t)nu * sizeof(float2));
+    w.i2.ensure((size_t)nu * sizeof(float2));
+    w.z1.ensure((size_t)D * nu * sizeof(float2));
+    w.z2.ensure((size_t)D * nu * sizeof(float2));
+    w.sc.ensure(sizeof(CgScalars));
+    w.partial.ensure(kRB * sizeof(double));
+    w.hist.ensure((size_t)std::max(nIter, 1) * 2 * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2 *res_y = w.k1.as<float2>(), *Mp = w.k2.as<float2>();
+    float2 *dag = w.i1.as<float2>(), *p = w.i2.as<float2>();
+    float2 *res_z = w.z1.as<float2>(), *Fp = w.z2.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    double* hist = w.hist.as<double>();
+    compute_hy(ve, ve_max, hy, Pn, s);
+    const float invHX = (float)(1.0 / P.HX);
+    const float HZ = (float)P.HX;
+    if (!zu_given) {
+        k_grad<<<nb(nu), 256, 0, s>>>(x, z, nullptr, nullptr, d);
+        MONA_HIP(hipGetLastError());
+        fill_f32(reinterpret_cast<float*>(u), 0.0f, (int64_t)D * nu * 2, s);
+    }
+    for (int c = 0; c < nIter; c++) {
+        float rc = rho[c];
+        cg_reset(sc, s);
+        P.forward(f, x, Mp, s);
+        vsub(y, Mp, res_y, ny, s);
+        k_grad<<<nb(nu), 256, 0, s>>>(x, res_z, z, u, d);                  // res_z = (z-u) - Fx
+        MONA_HIP(hipGetLastError());
+        P.adjoint(f, res_y, dag, true, hy, invHX, s);                      // dagM
+        k_gradT<<<nb(nu), 256, 0, s>>>(res_z, dag, d, rc * invHX * HZ, 1);  // + rho/HX F^T(HZ res_z)
+        MONA_HIP(hipGetLastError());
+        vcopy(dag, p, nu, s);
+        sqnorm(dag, nu, nullptr, 1, P.HX, &sc->sqn, false, part, s);
+        for (int i = 1; i <= nCGD; i++) {
+            P.forward(f, p, Mp, s);
+            k_grad<<<nb(nu), 256, 0, s>>>(p, Fp, nullptr, nullptr, d);
+            MONA_HIP(hipGetLastError());
+            sqnorm(Mp, ny, hy, P.nCh, 1.0, &sc->den, false, part, s);
+            sqnorm(Fp, (int64_t)D * nu, nullptr, 1, (double)rc * HZ, &sc->den, true, part, s);
+            cg_alpha(sc, s);
+            vaxpy(x, p, &sc->a, 1.0f, nu, s);
+            if (i == nCGD) break;
+            vaxpy(res_y, Mp, &sc->a, -1.0f, ny, s);
+            vaxpy(res_z, Fp, &sc->a, -1.0f, (int64_t)D * nu, s);
+            P.adjoint(f, res_y, dag, true, hy, invHX, s);
+            k_gradT<<<nb(nu), 256, 0, s>>>(res_z, dag, d, rc * invHX * HZ, 1);
+            MONA_HIP(hipGetLastError());
+            sqnorm(dag, nu, nullptr, 1, P.HX, &sc->sqn_new, false, part, s);
+            cg_beta(sc, s);
+            vxpby(p, dag, &sc->b, nu, s);
+        }
+        // z, u update
+        k_grad<<<nb(nu), 256, 0, s>>>(x, Fp, nullptr, nullptr, d);
+        MONA_HIP(hipGetLastError());
+        k_prox_update<<<nb((int64_t)D * nu), 256, 0, s>>>(Fp, z, u, (int64_t)D * nu, delta[c] / rc);
+        MONA_HIP(hipGetLastError());
+        // monitoring: R = <r, HY r>, TV = HZ * sum |Re| + |Im| of grad x
+        P.forward(f, x, Mp, s);
+        vsub(y, Mp, res_y, ny, s);
+        sqnorm(res_y, ny, hy, P.nCh, 1.0, hist + 2 * c, false, part, s);
+        abs1(Fp, (int64_t)D * nu, (double)HZ, hist + 2 * c + 1, part, s);
+    }
+    if (hist_out && nIter > 0) {
+        MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+}
+
+// ================================================================ bmTevaMorphosia_chain
+void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const float2* y,
+                const long long* y_off, const float* ve, const long long* ve_off, float ve_max,
+                const float* delta_in, const float* rho_in, int nIter, int nCGD, double* hist_out, hipStream_t s) {
+    int nFr = (int)P.frames.size();
+    MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
+    MONA_CHECK(ve_max > 0, MONA_ERR_ARG, "ve_max must be > 0");
+    int64_t nu = P.nu, nx = (int64_t)nFr * nu;
+    std::vector<int64_t> Pf(nFr), koff(nFr + 1, 0);
+    for (int f = 0; f < nFr; f++) {
+        Pf[f] = P.frame(f).Gu->l_size;
+        koff[f + 1] = koff[f] + Pf[f] * P.nCh;
+    }
+    int64_t ny = koff[nFr];
+    std::vector<float> delta = schedule(delta_in, nIter), rho = schedule(rho_in, nIter);
+    Work& w = P.work;
+    w.hy.ensure((size_t)(ny / P.nCh) * sizeof(float));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.z1.ensure((size_t)nx * sizeof(float2));
+    w.z2.ensure((size_t)nx * sizeof(float2));
+    w.z3.ensure((size_t)nx * sizeof(float2));
+    w.z4.ensure((size_t)nx * sizeof(float2));
+    w.sc.ensure(sizeof(CgScalars));
+    w.partial.ensure(kRB * sizeof(double));
+    w.hist.ensure((size_t)std::max(nIter, 1) * 3 * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2 *res_y = w.k1.as<float2>(), *Mp = w.k2.as<float2>();
+    float2 *res_z = w.z1.as<float2>(), *Fp = w.z2.as<float2>(), *dag = w.z3.as<float2>(), *p = w.z4.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    double* hist = w.hist.as<double>();
+    const float invHX = (float)(1.0 / P.HX);
+    const float HZ = (float)P.HX;
+    // HY per frame, packed like the k-space vectors (one weight per point)
+    for (int f = 0; f < nFr; f++) compute_hy(ve + ve_off[f], ve_max, hy + koff[f] / P.nCh, Pf[f], s);
+    auto yF = [&](int f) { return y + y_off[f]; };
+    auto M = [&](const float2* xx, float2* out) {
+        for (int f = 0; f < nFr; f++) P.forward(f, xx + (int64_t)f * nu, out + koff[f], s);
+    };
+    auto Mh = [&](const float2* r, float2* out) {
+        for (int f = 0; f < nFr; f++)
+            P.adjoint(f, r + koff[f], out + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, s);
+    };
+    if (!zu_given) {
+        k_tdiff<<<nb(nx), 256, 0, s>>>(x, z, nullptr, nullptr, nFr, nu);
+        MONA_HIP(hipGetLastError());
+        fill_f32(reinterpret_cast<float*>(u), 0.0f, nx * 2, s);
+    }
+    for (int c = 0; c < nIter; c++) {
+        float dl = delta[c], rc = rho[c];
+        cg_reset(sc, s);
+        M(x, Mp);
+        for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
+        k_tdiff<<<nb(nx), 256, 0, s>>>(x, res_z, z, u, nFr, nu);
+        MONA_HIP(hipGetLastError());
+        Mh(res_y, dag);
+        k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(res_z, dag, nFr, nu, rc * invHX * HZ);
+        MONA_HIP(hipGetLastError());
+        vcopy(dag, p, nx, s);
+        sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn, false, part, s);
+        for (int j = 1; j <= nCGD; j++) {
+            M(p, Mp);
+            k_tdiff<<<nb(nx), 256, 0, s>>>(p, Fp, nullptr, nullptr, nFr, nu);
+            MONA_HIP(hipGetLastError());
+            sqnorm(Mp, ny, hy, P.nCh, 1.0, &sc->den, false, part, s);
+            sqnorm(Fp, nx, nullptr, 1, (double)rc * HZ, &sc->den, true, part, s);
+            cg_alpha(sc, s);
+            vaxpy(x, p, &sc->a, 1.0f, nx, s);
+            if (j == nCGD) break;
+            vaxpy(res_y, Mp, &sc->a, -1.0f, ny, s);
+            vaxpy(res_z, Fp, &sc->a, -1.0f, nx, s);
+            Mh(res_y, dag);
+            k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(res_z, dag, nFr, nu, rc * invHX * HZ);
+            MONA_HIP(hipGetLastError());
+            sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn_new, false, part, s);
+            cg_beta(sc, s);
+            vxpby(p, dag, &sc->b, nx, s);
+        }
+        k_tdiff<<<nb(nx), 256, 0, s>>>(x, Fp, nullptr, nullptr, nFr, nu);
+        MONA_HIP(hipGetLastError());
+        k_prox_update<<<nb(nx), 256, 0, s>>>(Fp, z, u, nx, dl / rc);
+        MONA_HIP(hipGetLastError());
+        // monitoring: dafi, regul (private_dafi_regul), objective
+        M(x, Mp);
+        for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
+        sqnorm(res_y, ny, hy, P.nCh, 1.0, hist + 3 * c + 1, false, part, s);
+        k_tdiff<<<nb(nx), 256, 0, s>>>(x, Fp, nullptr, nullptr, nFr, nu);
+        MONA_HIP(hipGetLastError());
+        abs1(Fp, nx, (double)HZ, hist + 3 * c + 2, part, s);
+        k_objective<<<1, 1, 0, s>>>(hist + 3 * c, (double)dl);
+        MONA_HIP(hipGetLastError());
+    }
+    if (hist_out && nIter > 0) {
+        MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+}
+
+// ================================================================ bmNasha
+void nasha(Plan& P, const SparseDev& Gn, const float2* y, const float2* C, float2* x, hipStream_t s) {
+    MONA_CHECK(P.coils_set, MONA_ERR_STATE, "coil maps not set (the deapodisation is computed there)");
+    MONA_CHECK(Gn.l_size == P.Nu, MONA_ERR_ARG, "Gn has l_size != prod(N_u)");
+    P.ensure_grid();
+    float2* gr = P.grid.as<float2>();
+    {
+        TimedRegion t(1, s);
+        spmv_c2_rowmajor(Gn, y, gr, P.nCh, nullptr, s);
+    }
+    {
+        TimedRegion t(2, s);
+        cfft_nd(gr, P.imDim, P.N, P.nCh, true, s);
+    }
+    float pN = 1.0f, pdK = 1.0f;
+    for (int a = 0; a < P.imDim; a++) { pN *= (float)P.N[a]; pdK *= (float)P.dK[a]; }
+    float F = pN * pdK;  // bmIDF3.m:18 F = single(prod(N_u)*prod(dK_u))
+    Geom g = P.geom();
+    k_nasha_epilogue<<<nb(P.nu), 256, 0, s>>>(gr, P.K.as<float>(), C, x, g, F);
+    MONA_HIP(hipGetLastError());
+}
+
+}  // namespace mona
 
 using namespace mona;
 
 extern "C" {
 
-int mona_steva(mona_plan, int, void*, void*, void*, int, const void*, const float*, float, const float*,
-               const float*, int, int, double*, hipStream_t) {
-    return guarded([&] { MONA_CHECK(false, MONA_ERR_STATE, "mona_steva: not built yet"); });
+int mona_steva(mona_plan P, int frame, void* x, void* z, void* u, int zu_given, const void* y, const float* ve,
+               float ve_max, const float* delta, const float* rho, int nIter, int nCGD, double* hist,
+               hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && z && u && y && ve && delta && rho, MONA_ERR_ARG, "NULL argument");
+        steva(P->P, frame, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u),
+              zu_given != 0, static_cast<const float2*>(y), ve, ve_max, delta, rho, nIter, nCGD, hist, stream);
+    });
 }
 
-int mona_teva_chain(mona_plan, void*, void*, void*, int, const void*, const long long*, const float*,
-                    const long long*, float, const float*, const float*, int, int, double*, hipStream_t) {
-    return guarded([&] { MONA_CHECK(false, MONA_ERR_STATE, "mona_teva_chain: not built yet"); });
+int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given, const void* y, const long long* y_off,
+                    const float* ve, const long long* ve_off, float ve_max, const float* delta, const float* rho,
+                    int nIter, int nCGD, double* hist, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && z && u && y && y_off && ve && ve_off && delta && rho, MONA_ERR_ARG, "NULL argument");
+        teva_chain(P->P, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u), zu_given != 0,
+                   static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD, hist, stream);
+    });
 }
 
-int mona_nasha(mona_plan, mona_sparse, const void*, const void*, void*, hipStream_t) {
-    return guarded([&] { MONA_CHECK(false, MONA_ERR_STATE, "mona_nasha: not built yet"); });
+int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && Gn && y && x, MONA_ERR_ARG, "NULL argument");
+        HostCallCtx& hc = host_ctx();
+        {
+            std::lock_guard<std::mutex> lk(hc.mu);
+            hc.init();
+        }
+        sparse_upload(Gn->S, hc.stream);
+        nasha(P->P, Gn->S, static_cast<const float2*>(y), static_cast<const float2*>(C), static_cast<float2*>(x),
+              stream);
+    });
 }
 
 }  // extern "C"

@@ -169,7 +169,7 @@ __device__ void stage(float2* lds, int L, int N, int LD, int Ns, const float2* t
 }
 
 __global__ void __launch_bounds__(kFftThreads)
-fft_axis_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftArgs a,
+fft_axis_kernel(const float2* in, float2* out, FftArgs a,  // in == out allowed (no restrict)
                 const float2* __restrict__ twg) {
     extern __shared__ float2 smem[];
     const int N = a.N;

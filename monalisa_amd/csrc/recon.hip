@@ -533,10 +533,6 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
 
 using namespace mona;
 
-struct mona_plan_s {
-    Plan P;
-};
-
 extern "C" {
 
 int mona_plan_create(int imDim, const int* N_u, const int* n_u, const double* dK_u, int nCh,

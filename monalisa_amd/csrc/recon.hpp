@@ -87,3 +87,7 @@ void dstore(double* dst, const double* src, hipStream_t s);
 float device_max(const float* v, int64_t n, hipStream_t s);
 
 }  // namespace mona
+
+struct mona_plan_s {
+    mona::Plan P;
+};
