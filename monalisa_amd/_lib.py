@@ -78,6 +78,14 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C monalisa_amd/csrc)")
+    # torch bundles its own HIP runtime (soname libamdhip64.so.7).  It must be
+    # in the process before libmona so that libmona's libamdhip64.so.7
+    # dependency resolves to that same runtime: two HIP runtimes in one
+    # process do not share streams or device pointers.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     h = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(h, name)
