@@ -106,7 +106,7 @@ def coil_maps(n_u, nCh, rng):
         C[..., c] = mag * np.exp(1j * phase)
     ss = np.sqrt((np.abs(C) ** 2).sum(-1, keepdims=True))
     C = C / np.maximum(ss, 1e-12)
-    return C.reshape(-1, nCh, order="F").astype(np.complex64)
+    return np.ascontiguousarray(C.reshape(-1, nCh, order="F"), dtype=np.complex64)
 
 
 def ellipsoid_phantom(n_u, rng, n_ellipsoids=8):
