@@ -4,42 +4,55 @@
 // The centring of bmShanna_MATLAB.m:46-50 / bmNakatsha_MATLAB.m:36-40,
 // fftshift(fft(ifftshift(x))) along an axis of even length N, equals
 //   Y[k] = (-1)^(k + N/2) * sum_n (-1)^n x[n] e^(-+2 pi i n k / N)
-// so each pass multiplies by (-1)^n when it loads a line and by
-// (-1)^(k + N/2) (times 1/N for the inverse) when it stores it; no shift
+// so a pass multiplies by (-1)^n when it loads a line and by
+// (-1)^(k + N/2) (and 1/N for the inverse) when it stores it; no shift
 // copies are made.
 //
-// A workgroup owns a tile of L lines ((L+1)*N <= 4096 complex values, 32 KiB
-// of LDS with the twiddle table): W consecutive positions of the contiguous inner dimension times Bo
-// outer indices, so every global load/store row is a run of W float2.  The
-// Stockham stages run in LDS with each work-item holding its butterflies'
-// inputs in registers across one barrier (in-place autosort).
+// Pruning and fusion.  A pass reads only the window of rows that can be
+// non-zero (the zero-filled image occupies the centre n of N) and writes
+// only the window of rows that is kept (the adjoint crops to the centre), so
+// the forward operator's three passes move 1/8 -> 1/4 -> 1/2 -> 1 of the grid
+// instead of 1 -> 1 three times.  The forward pass along x takes its input
+// straight from x .* KFC (bmShanna_MATLAB.m:34-40) and the adjoint pass
+// along x ends with .* KFC_conj and the coil sum (bmNakatsha_MATLAB.m:46-55).
+//
+// Tile.  A workgroup owns L lines = Bo outer indices x W consecutive inner
+// positions, all N rows, stored transposed in LDS as [n][line]: the global
+// rows are runs of W float2, the LDS writes of a wave are contiguous, and
+// the butterfly reads/writes of a wave touch consecutive lines (no bank
+// conflicts).  Each work-item keeps its butterflies' inputs in registers
+// across one barrier per stage (in-place Stockham autosort).
+#include <cstdlib>
 #include <map>
 #include <memory>
 
-#include "mona_internal.hpp"
 #include "fft.hpp"
+#include "mona_internal.hpp"
 
 namespace mona {
 
 namespace {
 
-constexpr int kFftThreads = 256;
-constexpr int kFftMaxElems = 4096;  // twiddles + tile, per workgroup (32 KiB)
+constexpr int kThreads = 256;
 constexpr int kMaxStages = 16;
 
-struct FftArgs {
-    int N;
-    int nstage;
+struct PassArgs {
+    int N, nstage;
     int radix[kMaxStages];
-    int64_t S;        // inner stride (float2 elements)
-    int64_t outer;    // number of outer indices
-    int W;            // tile width along the inner dimension
-    int Bo;           // outer indices per tile
-    int64_t ntile_w;  // tiles along the inner dimension
-    int inverse;
-    int presign;
-    int postsign;
-    float scale;
+    int64_t S, outer;
+    int W, Bo;
+    int64_t ntile_w;
+    const float2* in;
+    int64_t in_os;
+    int in_off, in_cnt;
+    float2* out;
+    int64_t out_os;
+    int out_off, out_cnt;
+    int inverse, presign, postsign;
+    float scale, epi_scale;
+    const float2* img;   // prologue: image, [outer][in_cnt]
+    const float2* kfc;   // prologue: KFC [outer*in_cnt][S]; epilogue: KFC_conj [outer*out_cnt][S]
+    float2* img_out;     // epilogue (combine): [outer][out_cnt]
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -124,80 +137,88 @@ template <> struct Dft<8> {
     }
 };
 
-// One Stockham stage of radix R over L lines of length N held in LDS with
-// leading dimension LD.  j in [0, N/R): inputs at j + r*N/R, twiddle
-// w_(Ns R)^(j%Ns * r), outputs at (j/Ns)*Ns*R + j%Ns + r*Ns.
-template <int R>
-__device__ void stage(float2* lds, int L, int N, int LD, int Ns, const float2* tw, float sg) {
-    constexpr int MAXT = (kFftMaxElems / kFftThreads + R - 1) / R;
+// One Stockham stage of radix R over L lines of length N held transposed
+// in LDS ([n][line]).  j in [0, N/R): inputs at rows j + r*N/R, twiddle
+// w_(Ns R)^(j%Ns * r), outputs at rows (j/Ns)*Ns*R + j%Ns + r*Ns.
+template <int TILE, int R>
+__device__ void stage(float2* lds, int L, int N, int Ns, const float2* __restrict__ tw, float sg) {
+    constexpr int MAXT = (TILE / kThreads + R - 1) / R;
     float2 v[MAXT][R];
     const int nb = N / R;
     const int ntask = L * nb;
     const int tstride = N / (Ns * R);
 #pragma unroll
     for (int i = 0; i < MAXT; i++) {
-        int t = threadIdx.x + i * kFftThreads;
+        int t = threadIdx.x + i * kThreads;
         if (t < ntask) {
-            int line = t / nb, j = t - line * nb;
-            const float2* src = lds + line * LD + j;
+            int j = t / L, line = t - j * L;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[i][r] = src[r * nb];
+            for (int r = 0; r < R; r++) v[i][r] = lds[(j + r * nb) * L + line];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < MAXT; i++) {
-        int t = threadIdx.x + i * kFftThreads;
+        int t = threadIdx.x + i * kThreads;
         if (t < ntask) {
-            int line = t / nb, j = t - line * nb;
+            int j = t / L, line = t - j * L;
             int k = j % Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
                     float2 w = tw[k * r * tstride];
-                    w.y *= -sg;  // table holds exp(-2 pi i t / N); conj for inverse
+                    w.y *= -sg;  // table holds exp(-2 pi i t / N); conj for the inverse
                     v[i][r] = cmul(v[i][r], w);
                 }
             }
             Dft<R>::run(v[i], sg);
-            float2* dst = lds + line * LD + (j / Ns) * Ns * R + k;
+            int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[r * Ns] = v[i][r];
+            for (int r = 0; r < R; r++) lds[(base + r * Ns) * L + line] = v[i][r];
         }
     }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kFftThreads)
-fft_axis_kernel(const float2* in, float2* out, FftArgs a,  // in == out allowed (no restrict)
-                const float2* __restrict__ twg) {
-    extern __shared__ float2 smem[];
+// MODE 0: plain pass; 1: forward prologue (input = img .* KFC);
+// 2: adjoint epilogue with coil sum; 3: adjoint epilogue per coil.
+template <int TILE, int MODE>
+__global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const float2* __restrict__ tw) {
+    extern __shared__ float2 lds[];
     const int N = a.N;
-    float2* tw = smem;            // N twiddles
-    float2* lds = smem + N;       // tile
-    for (int i = threadIdx.x; i < N; i += kFftThreads) tw[i] = twg[i];
-
-    int64_t tw_idx = blockIdx.x % a.ntile_w;
-    int64_t to = blockIdx.x / a.ntile_w;
-    int64_t w0 = tw_idx * a.W;
-    int Wt = (int)((a.S - w0) < a.W ? (a.S - w0) : a.W);
-    int64_t o0 = to * a.Bo;
-    int Bt = (int)((a.outer - o0) < a.Bo ? (a.outer - o0) : a.Bo);
+    const int64_t tile_w = blockIdx.x % a.ntile_w;
+    const int64_t to = blockIdx.x / a.ntile_w;
+    const int64_t w0 = tile_w * a.W;
+    const int Wt = (int)((a.S - w0) < a.W ? (a.S - w0) : a.W);
+    const int64_t o0 = to * a.Bo;
+    const int Bt = (int)((a.outer - o0) < a.Bo ? (a.outer - o0) : a.Bo);
     const int L = Bt * Wt;
-    const int LD = N + 1;  // odd pitch: the line-strided load/store phases avoid bank conflicts
-    const int64_t lineN = (int64_t)N * a.S;
 
-    // load (coalesced along w), pre-sign (-1)^n
-    const int tot = L * N;
-    for (int e = threadIdx.x; e < tot; e += kFftThreads) {
+    // rows outside the input window are zero
+    const int nzero = (N - a.in_cnt) * L;
+    for (int e = threadIdx.x; e < nzero; e += kThreads) {
+        int row = e / L, line = e - row * L;
+        int n = row < a.in_off ? row : row + a.in_cnt;
+        lds[n * L + line] = make_float2(0.f, 0.f);
+    }
+    // load the input window (coalesced along w), pre-sign (-1)^n
+    const int nin = a.in_cnt * L;
+    for (int e = threadIdx.x; e < nin; e += kThreads) {
         int w = e % Wt;
         int rest = e / Wt;
-        int n = rest % N;
-        int ob = rest / N;
-        int64_t g = (o0 + ob) * lineN + (int64_t)n * a.S + w0 + w;
-        float2 v = in[g];
+        int r = rest % a.in_cnt;
+        int ob = rest / a.in_cnt;
+        int64_t o = o0 + ob;
+        float2 v;
+        if (MODE == 1) {
+            int64_t i = o * a.in_cnt + r;
+            v = cmul(a.img[i], a.kfc[i * a.S + w0 + w]);
+        } else {
+            v = a.in[o * a.in_os + (int64_t)r * a.S + w0 + w];
+        }
+        int n = a.in_off + r;
         if (a.presign && (n & 1)) { v.x = -v.x; v.y = -v.y; }
-        lds[(ob * Wt + w) * LD + n] = v;
+        lds[n * L + ob * Wt + w] = v;
     }
     __syncthreads();
 
@@ -206,27 +227,57 @@ fft_axis_kernel(const float2* in, float2* out, FftArgs a,  // in == out allowed 
     for (int s = 0; s < a.nstage; s++) {
         int R = a.radix[s];
         switch (R) {
-            case 8: stage<8>(lds, L, N, LD, Ns, tw, sg); break;
-            case 5: stage<5>(lds, L, N, LD, Ns, tw, sg); break;
-            case 4: stage<4>(lds, L, N, LD, Ns, tw, sg); break;
-            case 3: stage<3>(lds, L, N, LD, Ns, tw, sg); break;
-            default: stage<2>(lds, L, N, LD, Ns, tw, sg); break;
+            case 8: stage<TILE, 8>(lds, L, N, Ns, tw, sg); break;
+            case 5: stage<TILE, 5>(lds, L, N, Ns, tw, sg); break;
+            case 4: stage<TILE, 4>(lds, L, N, Ns, tw, sg); break;
+            case 3: stage<TILE, 3>(lds, L, N, Ns, tw, sg); break;
+            default: stage<TILE, 2>(lds, L, N, Ns, tw, sg); break;
         }
         Ns *= R;
     }
 
-    // store, post-sign (-1)^(k + N/2), scale
+    // store the output window, post-sign (-1)^(k + N/2), scale
     const int half = N / 2;
-    for (int e = threadIdx.x; e < tot; e += kFftThreads) {
+    if (MODE == 2) {
+        // one work-item per (outer, kept row): sum over the coils of the line
+        const int nout = a.out_cnt * Bt;
+        for (int e = threadIdx.x; e < nout; e += kThreads) {
+            int kl = e % a.out_cnt, ob = e / a.out_cnt;
+            int k = a.out_off + kl;
+            float sc = a.scale;
+            if (a.postsign && ((k + half) & 1)) sc = -sc;
+            int64_t i = (o0 + ob) * a.out_cnt + kl;
+            const float2* kc = a.kfc + i * a.S;
+            float2 acc = make_float2(0.f, 0.f);
+            for (int c = 0; c < Wt; c++) {
+                float2 v = lds[k * L + ob * Wt + c];
+                v = make_float2(v.x * sc, v.y * sc);
+                float2 p = cmul(v, kc[c]);
+                acc.x += p.x;
+                acc.y += p.y;
+            }
+            a.img_out[i] = make_float2(acc.x * a.epi_scale, acc.y * a.epi_scale);
+        }
+        return;
+    }
+    const int nout = a.out_cnt * L;
+    for (int e = threadIdx.x; e < nout; e += kThreads) {
         int w = e % Wt;
         int rest = e / Wt;
-        int k = rest % N;
-        int ob = rest / N;
-        int64_t g = (o0 + ob) * lineN + (int64_t)k * a.S + w0 + w;
-        float2 v = lds[(ob * Wt + w) * LD + k];
+        int kl = rest % a.out_cnt;
+        int ob = rest / a.out_cnt;
+        int k = a.out_off + kl;
         float sc = a.scale;
         if (a.postsign && ((k + half) & 1)) sc = -sc;
-        out[g] = make_float2(v.x * sc, v.y * sc);
+        float2 v = lds[k * L + ob * Wt + w];
+        v = make_float2(v.x * sc, v.y * sc);
+        if (MODE == 3) {
+            int64_t i = (o0 + ob) * a.out_cnt + kl;
+            float2 p = cmul(v, a.kfc[i * a.S + w0 + w]);
+            a.out[i * a.S + w0 + w] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
+        } else {
+            a.out[(o0 + ob) * a.out_os + (int64_t)kl * a.S + w0 + w] = v;
+        }
     }
 }
 
@@ -255,23 +306,49 @@ TwiddleCache& twiddles() {
     return c;
 }
 
+int tile_elems() {
+    static int t = [] {
+        const char* e = std::getenv("MONA_FFT_TILE");
+        int v = e ? std::atoi(e) : 8192;
+        return (v == 4096) ? 4096 : 8192;
+    }();
+    return t;
+}
+
+template <int TILE>
+void launch(const PassArgs& a, int mode, unsigned nblocks, size_t lds, const float2* tw, hipStream_t s) {
+    switch (mode) {
+        case 1: fft_pass_kernel<TILE, 1><<<nblocks, kThreads, lds, s>>>(a, tw); break;
+        case 2: fft_pass_kernel<TILE, 2><<<nblocks, kThreads, lds, s>>>(a, tw); break;
+        case 3: fft_pass_kernel<TILE, 3><<<nblocks, kThreads, lds, s>>>(a, tw); break;
+        default: fft_pass_kernel<TILE, 0><<<nblocks, kThreads, lds, s>>>(a, tw); break;
+    }
+}
+
 }  // namespace
 
 bool fft_size_supported(int N) {
-    if (N < 2 || 2 * N + 1 > kFftMaxElems) return false;
+    if (N < 2 || N > 2048) return false;
     int n = N;
     for (int f : {2, 3, 5})
         while (n % f == 0) n /= f;
     return n == 1;
 }
 
-void fft_axis(const float2* in, float2* out, int64_t outer, int N, int64_t S, bool inverse,
-              bool presign, bool postsign, float scale, hipStream_t s) {
+bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= tile_elems(); }
+
+void fft_pass(const AxisPass& p, hipStream_t s) {
+    const int N = p.N;
     MONA_CHECK(fft_size_supported(N), MONA_ERR_ARG,
                "FFT length " + std::to_string(N) + " is not 2^a 3^b 5^c <= 2048");
-    MONA_CHECK(!(presign || postsign) || N % 2 == 0, MONA_ERR_ARG,
+    MONA_CHECK(!(p.presign || p.postsign) || N % 2 == 0, MONA_ERR_ARG,
                "N_u must have all components even for the Fourier transform.");
-    FftArgs a{};
+    MONA_CHECK(p.in_off >= 0 && p.in_cnt >= 1 && p.in_off + p.in_cnt <= N, MONA_ERR_INTERNAL, "bad input window");
+    MONA_CHECK(p.out_off >= 0 && p.out_cnt >= 1 && p.out_off + p.out_cnt <= N, MONA_ERR_INTERNAL,
+               "bad output window");
+    MONA_CHECK(p.S >= 1 && p.outer >= 1, MONA_ERR_INTERNAL, "empty FFT pass");
+    const int TILE = tile_elems();
+    PassArgs a{};
     a.N = N;
     int n = N, ns = 0;
     while (n % 8 == 0 && n > 4) { a.radix[ns++] = 8; n /= 8; }
@@ -280,31 +357,68 @@ void fft_axis(const float2* in, float2* out, int64_t outer, int N, int64_t S, bo
     while (n % 3 == 0) { a.radix[ns++] = 3; n /= 3; }
     while (n % 5 == 0) { a.radix[ns++] = 5; n /= 5; }
     a.nstage = ns;
-    a.S = S;
-    a.outer = outer;
-    int Lmax = (kFftMaxElems - N) / (N + 1);
-    if (Lmax < 1) Lmax = 1;
-    if (S >= 16 || S >= Lmax) {
-        a.W = (int)(S < Lmax ? S : Lmax);
-        a.Bo = 1;
+    a.S = p.S;
+    a.outer = p.outer;
+    const int Lmax = TILE / N;
+    if (p.S <= Lmax) {
+        a.W = (int)p.S;
+        a.Bo = Lmax / (int)p.S;
+        if (p.mode == 2) a.Bo = 1;
     } else {
-        a.W = (int)S;
-        a.Bo = Lmax / (int)S;
-        if (a.Bo < 1) a.Bo = 1;
+        MONA_CHECK(p.mode == 0, MONA_ERR_INTERNAL, "fused FFT pass needs the whole inner dimension in one tile");
+        int w = 1;
+        while (w * 2 <= Lmax && w * 2 <= 64) w *= 2;
+        a.W = w;
+        a.Bo = 1;
     }
-    a.ntile_w = (S + a.W - 1) / a.W;
-    a.inverse = inverse;
-    a.presign = presign;
-    a.postsign = postsign;
-    a.scale = scale;
-    int64_t ntile_o = (outer + a.Bo - 1) / a.Bo;
+    if (a.Bo > a.outer) a.Bo = (int)a.outer;
+    a.ntile_w = (p.S + a.W - 1) / a.W;
+    a.in = p.in;
+    a.in_os = p.in_os;
+    a.in_off = p.in_off;
+    a.in_cnt = p.in_cnt;
+    a.out = p.out;
+    a.out_os = p.out_os;
+    a.out_off = p.out_off;
+    a.out_cnt = p.out_cnt;
+    a.inverse = p.inverse;
+    a.presign = p.presign;
+    a.postsign = p.postsign;
+    a.scale = p.scale;
+    a.epi_scale = p.epi_scale;
+    a.img = p.img;
+    a.kfc = p.kfc;
+    a.img_out = p.img_out;
+    int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
-    size_t lds = (size_t)(N + (size_t)a.W * a.Bo * (N + 1)) * sizeof(float2);
-    MONA_CHECK(lds <= (size_t)kFftMaxElems * sizeof(float2), MONA_ERR_INTERNAL, "FFT tile too large");
+    size_t lds = (size_t)a.W * a.Bo * N * sizeof(float2);
+    MONA_CHECK(lds <= (size_t)TILE * sizeof(float2), MONA_ERR_INTERNAL, "FFT tile too large");
     const float2* tw = twiddles().get(N);
-    fft_axis_kernel<<<(unsigned)nblocks, kFftThreads, lds, s>>>(in, out, a, tw);
+    if (TILE == 4096) launch<4096>(a, p.mode, (unsigned)nblocks, lds, tw, s);
+    else launch<8192>(a, p.mode, (unsigned)nblocks, lds, tw, s);
     MONA_HIP(hipGetLastError());
+}
+
+void fft_axis(const float2* in, float2* out, int64_t outer, int N, int64_t S, bool inverse,
+              bool presign, bool postsign, float scale, hipStream_t s) {
+    AxisPass p;
+    p.N = N;
+    p.S = S;
+    p.outer = outer;
+    p.in = in;
+    p.in_os = (int64_t)N * S;
+    p.in_off = 0;
+    p.in_cnt = N;
+    p.out = out;
+    p.out_os = (int64_t)N * S;
+    p.out_off = 0;
+    p.out_cnt = N;
+    p.inverse = inverse;
+    p.presign = presign;
+    p.postsign = postsign;
+    p.scale = scale;
+    fft_pass(p, s);
 }
 
 void cfft_nd(float2* data, int imDim, const int* N, int64_t nBatch, bool inverse, hipStream_t s) {
@@ -313,16 +427,14 @@ void cfft_nd(float2* data, int imDim, const int* N, int64_t nBatch, bool inverse
     for (int d = 0; d < imDim; d++) total *= N[d];
     for (int d = 0; d < imDim; d++) {
         int64_t outer = total / (inner * N[d]);
-        fft_axis(data, data, outer, N[d], inner, inverse, true, true,
-                 inverse ? 1.0f / (float)N[d] : 1.0f, s);
+        fft_axis(data, data, outer, N[d], inner, inverse, true, true, inverse ? 1.0f / (float)N[d] : 1.0f, s);
         inner *= N[d];
     }
 }
 
 }  // namespace mona
 
-extern "C" int mona_cfft(int imDim, const int* N, int nBatch, void* data, int inverse,
-                         hipStream_t stream) {
+extern "C" int mona_cfft(int imDim, const int* N, int nBatch, void* data, int inverse, hipStream_t stream) {
     return mona::guarded([&] {
         MONA_CHECK(imDim >= 1 && imDim <= 3 && N && data && nBatch >= 1, MONA_ERR_ARG, "bad argument");
         mona::cfft_nd(static_cast<float2*>(data), imDim, N, nBatch, inverse != 0, stream);

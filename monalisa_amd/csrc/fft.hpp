@@ -4,13 +4,43 @@
 
 namespace mona {
 
-// True when N = 2^a 3^b 5^c fits one LDS tile.
+// True when N = 2^a 3^b 5^c <= 2048.
 bool fft_size_supported(int N);
 
-// Transforms every line of length N of a [outer][N][S] float2 array (S
-// contiguous) along the middle axis; in == out is allowed.  presign
-// multiplies the input by (-1)^n, postsign the output by (-1)^(k+N/2),
-// scale multiplies the output.
+// One pass along an axis of length N over an [outer][rows][S] array (S
+// contiguous).  Input rows in_off .. in_off+in_cnt-1 of the length-N line
+// are read from in + o*in_os + r*S (r = row - in_off), the others are zero;
+// output rows out_off .. out_off+out_cnt-1 are written to
+// out + o*out_os + (row - out_off)*S.  presign multiplies input row n by
+// (-1)^n, postsign output row k by (-1)^(k+N/2), scale multiplies the
+// output.  mode 1: the input is img[o*in_cnt + r] * kfc[(o*in_cnt + r)*S + w];
+// mode 2: out is replaced by img_out[o*out_cnt + k] = epi_scale *
+// sum_w kfc[(o*out_cnt + k)*S + w] * y(k, w); mode 3: out[(o*out_cnt +
+// k)*S + w] = epi_scale * kfc[..] * y(k, w).
+struct AxisPass {
+    int N = 0;
+    int64_t S = 1, outer = 1;
+    const float2* in = nullptr;
+    int64_t in_os = 0;
+    int in_off = 0, in_cnt = 0;
+    float2* out = nullptr;
+    int64_t out_os = 0;
+    int out_off = 0, out_cnt = 0;
+    bool inverse = false, presign = true, postsign = true;
+    float scale = 1.0f, epi_scale = 1.0f;
+    int mode = 0;
+    const float2* img = nullptr;
+    const float2* kfc = nullptr;
+    float2* img_out = nullptr;
+};
+
+void fft_pass(const AxisPass& p, hipStream_t s);
+// True when a line of N rows times S inner values fits one tile (required
+// for the fused prologue / epilogue modes).
+bool fft_pass_fusable(int N, int64_t S);
+
+// Transforms every line of length N of a [outer][N][S] float2 array along
+// the middle axis; in == out is allowed.
 void fft_axis(const float2* in, float2* out, int64_t outer, int N, int64_t S, bool inverse,
               bool presign, bool postsign, float scale, hipStream_t s);
 

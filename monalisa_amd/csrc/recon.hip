@@ -376,20 +376,107 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
     MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
     const Frame& fr = frame(f);
     ensure_grid();
-    Geom g = geom();
     float2* gr = grid.as<float2>();
     {
-        TimedRegion t(3, s);
-        prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
-        launch_check();
-    }
-    {
         TimedRegion t(2, s);
-        cfft_nd(gr, imDim, N, nCh, false, s);
+        if (fft_pass_fusable(N[0], nCh)) {
+            forward_fft_pruned(x, gr, s);
+        } else {
+            Geom g = geom();
+            prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
+            launch_check();
+            cfft_nd(gr, imDim, N, nCh, false, s);
+        }
     }
     {
         TimedRegion t(0, s);
         spmv_c2_rowmajor(*fr.Gu, gr, y, nCh, nullptr, s);
+    }
+}
+
+// Scratch for the pruned passes: the largest intermediate is
+// prod(N_d, d <= a) * prod(n_d, d > a) * nCh over the axes a < imDim - 1.
+void Plan::ensure_fft_scratch() {
+    int64_t mx = 1;
+    for (int a = 0; a + 1 < imDim; a++) {
+        int64_t v = nCh;
+        for (int d = 0; d < imDim; d++) v *= (d <= a) ? N[d] : n[d];
+        mx = std::max(mx, v);
+    }
+    fft_a.ensure((size_t)mx * sizeof(float2));
+    fft_b.ensure((size_t)mx * sizeof(float2));
+}
+
+// Forward: x .* KFC zero-filled to N_u and transformed, one pass per axis;
+// pass a reads the compact [prod(n_d, d>a)][n_a][S_a] array, S_a =
+// nCh*prod(N_d, d<a), and writes [prod(n_d, d>a)][N_a][S_a].
+void Plan::forward_fft_pruned(const float2* x, float2* gr, hipStream_t s) {
+    ensure_fft_scratch();
+    const float2* src = nullptr;
+    float2* bufs[2] = {fft_a.as<float2>(), fft_b.as<float2>()};
+    int64_t S = nCh;
+    for (int a = 0; a < imDim; a++) {
+        int64_t outer = 1;
+        for (int d = a + 1; d < imDim; d++) outer *= n[d];
+        AxisPass p;
+        p.N = N[a];
+        p.S = S;
+        p.outer = outer;
+        p.in_off = N[a] / 2 - n[a] / 2;
+        p.in_cnt = n[a];
+        p.in = src;
+        p.in_os = (int64_t)n[a] * S;
+        p.out = (a == imDim - 1) ? gr : bufs[a & 1];
+        p.out_os = (int64_t)N[a] * S;
+        p.out_off = 0;
+        p.out_cnt = N[a];
+        p.inverse = false;
+        if (a == 0) {
+            p.mode = 1;
+            p.img = x;
+            p.kfc = KFC.as<float2>();
+        }
+        fft_pass(p, s);
+        src = p.out;
+        S *= N[a];
+    }
+}
+
+// Adjoint: the grid transformed back one axis at a time from the last,
+// each pass keeping only the central n_a rows; the pass along x ends with
+// .* KFC_conj (and the coil sum when combine).
+void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, hipStream_t s) {
+    ensure_fft_scratch();
+    const float2* src = gr;
+    float2* bufs[2] = {fft_a.as<float2>(), fft_b.as<float2>()};
+    for (int a = imDim - 1; a >= 0; a--) {
+        int64_t S = nCh, outer = 1;
+        for (int d = 0; d < a; d++) S *= N[d];
+        for (int d = a + 1; d < imDim; d++) outer *= n[d];
+        AxisPass p;
+        p.N = N[a];
+        p.S = S;
+        p.outer = outer;
+        p.in = src;
+        p.in_os = (int64_t)N[a] * S;
+        p.in_off = 0;
+        p.in_cnt = N[a];
+        p.out_off = N[a] / 2 - n[a] / 2;
+        p.out_cnt = n[a];
+        p.out_os = (int64_t)n[a] * S;
+        p.inverse = true;
+        p.scale = 1.0f / (float)N[a];
+        if (a == 0) {
+            p.mode = combine ? 2 : 3;
+            p.kfc = KFCc.as<float2>();
+            p.epi_scale = scale;
+            p.img_out = x;
+            p.out = x;
+        } else {
+            p.out = bufs[a & 1];
+        }
+        fft_pass(p, s);
+        src = p.out;
     }
 }
 
@@ -406,12 +493,13 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     }
     {
         TimedRegion t(2, s);
-        cfft_nd(gr, imDim, N, nCh, true, s);
-    }
-    {
-        TimedRegion t(3, s);
-        epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
-        launch_check();
+        if (fft_pass_fusable(N[0], nCh)) {
+            adjoint_fft_pruned(gr, x, combine, scale, s);
+        } else {
+            cfft_nd(gr, imDim, N, nCh, true, s);
+            epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
+            launch_check();
+        }
     }
 }
 

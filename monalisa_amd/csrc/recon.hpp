@@ -51,6 +51,7 @@ struct Plan {
     double HX = 1.0;
     bool coils_set = false;
     DevBuf K, KFC, KFCc, grid;
+    DevBuf fft_a, fft_b;  // pruned-pass intermediates
     std::vector<Frame> frames;
     Work work;
 
@@ -62,6 +63,10 @@ struct Plan {
     void ensure_grid();
     void launch_check();
     float kmax() const { return (float)kparam[ktype_ == MONA_KERNEL_GAUSS ? 1 : 2]; }
+
+    void ensure_fft_scratch();
+    void forward_fft_pruned(const float2* x, float2* grid, hipStream_t s);
+    void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, hipStream_t s);
 
     // y = M x  (x [nu], y [P][nCh])
     void forward(int f, const float2* x, float2* y, hipStream_t s);
