@@ -33,7 +33,7 @@ namespace mona {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kMaxStages = 16;
 
 struct PassArgs {
@@ -201,24 +201,66 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
         int n = row < a.in_off ? row : row + a.in_cnt;
         lds[n * L + line] = make_float2(0.f, 0.f);
     }
-    // load the input window (coalesced along w), pre-sign (-1)^n
+    // load the input window (coalesced along w), pre-sign (-1)^n.  All of a
+    // work-item's loads are issued before any LDS write so that they are in
+    // flight together.
     const int nin = a.in_cnt * L;
-    for (int e = threadIdx.x; e < nin; e += kThreads) {
-        int w = e % Wt;
-        int rest = e / Wt;
-        int r = rest % a.in_cnt;
-        int ob = rest / a.in_cnt;
-        int64_t o = o0 + ob;
-        float2 v;
-        if (MODE == 1) {
-            int64_t i = o * a.in_cnt + r;
-            v = cmul(a.img[i], a.kfc[i * a.S + w0 + w]);
-        } else {
-            v = a.in[o * a.in_os + (int64_t)r * a.S + w0 + w];
+    constexpr int EPT = TILE / kThreads;
+    if (MODE == 0 && Bt == 1 && (kThreads % Wt) == 0) {
+        // one outer index, Wt divides the workgroup: w is fixed per work-item
+        const int w = threadIdx.x % Wt;
+        const int rstep = kThreads / Wt;
+        const int r0 = threadIdx.x / Wt;
+        const float2* src = a.in + o0 * a.in_os + w0 + w;
+        float2 v[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            int r = r0 + i * rstep;
+            if (r < a.in_cnt) v[i] = src[(int64_t)r * a.S];
         }
-        int n = a.in_off + r;
-        if (a.presign && (n & 1)) { v.x = -v.x; v.y = -v.y; }
-        lds[n * L + ob * Wt + w] = v;
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            int r = r0 + i * rstep;
+            if (r < a.in_cnt) {
+                int n = a.in_off + r;
+                float2 x = v[i];
+                if (a.presign && (n & 1)) { x.x = -x.x; x.y = -x.y; }
+                lds[n * L + w] = x;
+            }
+        }
+    } else {
+        float2 v[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            int e = threadIdx.x + i * kThreads;
+            if (e < nin) {
+                int w = e % Wt;
+                int rest = e / Wt;
+                int r = rest % a.in_cnt;
+                int ob = rest / a.in_cnt;
+                int64_t o = o0 + ob;
+                if (MODE == 1) {
+                    int64_t ii = o * a.in_cnt + r;
+                    v[i] = cmul(a.img[ii], a.kfc[ii * a.S + w0 + w]);
+                } else {
+                    v[i] = a.in[o * a.in_os + (int64_t)r * a.S + w0 + w];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            int e = threadIdx.x + i * kThreads;
+            if (e < nin) {
+                int w = e % Wt;
+                int rest = e / Wt;
+                int r = rest % a.in_cnt;
+                int ob = rest / a.in_cnt;
+                int n = a.in_off + r;
+                float2 x = v[i];
+                if (a.presign && (n & 1)) { x.x = -x.x; x.y = -x.y; }
+                lds[n * L + ob * Wt + w] = x;
+            }
+        }
     }
     __syncthreads();
 

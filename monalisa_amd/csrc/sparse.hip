@@ -115,6 +115,40 @@ spmv_planes(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
     (c ? w1 : w0)[(int64_t)k * w_vs + (int64_t)row * w_rs] = acc;
 }
 
+// Sequential sum over entries [jb, je) of acc + m * v[col][k] (re and im),
+// in stored order.  Entries are taken eight at a time: their column indices
+// and values, then the eight gathers, are issued before the adds so that
+// the loads are in flight together; the additions stay in entry order.
+template <bool SCALE>
+__device__ __forceinline__ void row_sum(const int* __restrict__ col, const float* __restrict__ val,
+                                        int64_t jb, int64_t je, int n_vec, int k,
+                                        const float2* __restrict__ v, const float* __restrict__ scale,
+                                        float& ar, float& ai) {
+    constexpr int U = 8;
+    for (int64_t j = jb; j < je; j += U) {
+        int cnt = (int)((je - j) < U ? (je - j) : U);
+        int cj[U];
+        float m[U];
+        float2 x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) { cj[u] = col[j + u]; m[u] = val[j + u]; }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) {
+                x[u] = v[(int64_t)cj[u] * n_vec + k];
+                if (SCALE) { float s = scale[cj[u]]; x[u].x = s * x[u].x; x[u].y = s * x[u].y; }
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) {
+                float pr = m[u] * x[u].x, pi = m[u] * x[u].y;
+                ar = ar + pr;
+                ai = ai + pi;
+            }
+    }
+}
+
 // Complex interleaved rows, vector index fastest: v[col][n_vec] float2.
 // Optional per-column scale s[col] (a k-space weight HY applied to v before
 // the product, rounded exactly as a separate elementwise product would be).
@@ -130,15 +164,7 @@ spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
     int64_t jb = rowptr[i], je = rowptr[i + 1];
     if (je - jb > kLongRow) return;  // summed by spmv_c2_chunks + spmv_c2_combine
     float ar = 0.0f, ai = 0.0f;
-    for (int64_t j = jb; j < je; j++) {
-        int cj = col[j];
-        float m = val[j];
-        float2 x = v[(int64_t)cj * n_vec + k];
-        if (SCALE) { float s = scale[cj]; x.x = s * x.x; x.y = s * x.y; }
-        float pr = m * x.x, pi = m * x.y;
-        ar = ar + pr;
-        ai = ai + pi;
-    }
+    row_sum<SCALE>(col, val, jb, je, n_vec, k, v, scale, ar, ai);
     int row = rowidx ? rowidx[i] : i;
     w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
 }
@@ -155,15 +181,7 @@ spmv_c2_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const int*
     int64_t c = t / n_vec;
     int k = (int)(t - c * n_vec);
     float ar = 0.0f, ai = 0.0f;
-    for (int64_t j = chunk_be[2 * c], e = chunk_be[2 * c + 1]; j < e; j++) {
-        int cj = col[j];
-        float m = val[j];
-        float2 x = v[(int64_t)cj * n_vec + k];
-        if (SCALE) { float s = scale[cj]; x.x = s * x.x; x.y = s * x.y; }
-        float pr = m * x.x, pi = m * x.y;
-        ar = ar + pr;
-        ai = ai + pi;
-    }
+    row_sum<SCALE>(col, val, chunk_be[2 * c], chunk_be[2 * c + 1], n_vec, k, v, scale, ar, ai);
     partial[t] = make_float2(ar, ai);
 }
 
