@@ -258,7 +258,13 @@ void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const
     CgScalars* sc = w.sc.as<CgScalars>();
     double* part = w.partial.as<double>();
     double* hist = w.hist.as<double>();
-    compute_hy(ve, ve_max, hy, Pn, s);
+    // data and weights in the frame's tile order for the whole solve
+    w.hy0.ensure((size_t)Pn * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    compute_hy(ve, ve_max, w.hy0.as<float>(), Pn, s);
+    P.weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
+    P.to_tile_order(f, y, w.ys.as<float2>(), s);
+    y = w.ys.as<float2>();
     const float invHX = (float)(1.0 / P.HX);
     const float HZ = (float)P.HX;
     if (!zu_given) {
@@ -347,9 +353,17 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
     double* hist = w.hist.as<double>();
     const float invHX = (float)(1.0 / P.HX);
     const float HZ = (float)P.HX;
-    // HY per frame, packed like the k-space vectors (one weight per point)
-    for (int f = 0; f < nFr; f++) compute_hy(ve + ve_off[f], ve_max, hy + koff[f] / P.nCh, Pf[f], s);
-    auto yF = [&](int f) { return y + y_off[f]; };
+    // HY and the data per frame, packed like the k-space vectors and in each
+    // frame's tile order
+    w.hy0.ensure((size_t)(ny / P.nCh) * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    for (int f = 0; f < nFr; f++) {
+        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], s);
+        P.weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / P.nCh, hy + koff[f] / P.nCh, s);
+        P.to_tile_order(f, y + y_off[f], w.ys.as<float2>() + koff[f], s);
+    }
+    const float2* ysorted = w.ys.as<float2>();
+    auto yF = [&](int f) { return ysorted + koff[f]; };
     auto M = [&](const float2* xx, float2* out) {
         for (int f = 0; f < nFr; f++) P.forward(f, xx + (int64_t)f * nu, out + koff[f], s);
     };

@@ -33,7 +33,6 @@ namespace mona {
 
 namespace {
 
-constexpr int kThreads = 512;
 constexpr int kMaxStages = 16;
 
 struct PassArgs {
@@ -140,16 +139,16 @@ template <> struct Dft<8> {
 // One Stockham stage of radix R over L lines of length N held transposed
 // in LDS ([n][line]).  j in [0, N/R): inputs at rows j + r*N/R, twiddle
 // w_(Ns R)^(j%Ns * r), outputs at rows (j/Ns)*Ns*R + j%Ns + r*Ns.
-template <int TILE, int R>
+template <int TILE, int NT, int R>
 __device__ void stage(float2* lds, int L, int N, int Ns, const float2* __restrict__ tw, float sg) {
-    constexpr int MAXT = (TILE / kThreads + R - 1) / R;
+    constexpr int MAXT = (TILE / NT + R - 1) / R;
     float2 v[MAXT][R];
     const int nb = N / R;
     const int ntask = L * nb;
     const int tstride = N / (Ns * R);
 #pragma unroll
     for (int i = 0; i < MAXT; i++) {
-        int t = threadIdx.x + i * kThreads;
+        int t = threadIdx.x + i * NT;
         if (t < ntask) {
             int j = t / L, line = t - j * L;
 #pragma unroll
@@ -159,7 +158,7 @@ __device__ void stage(float2* lds, int L, int N, int Ns, const float2* __restric
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < MAXT; i++) {
-        int t = threadIdx.x + i * kThreads;
+        int t = threadIdx.x + i * NT;
         if (t < ntask) {
             int j = t / L, line = t - j * L;
             int k = j % Ns;
@@ -182,8 +181,8 @@ __device__ void stage(float2* lds, int L, int N, int Ns, const float2* __restric
 
 // MODE 0: plain pass; 1: forward prologue (input = img .* KFC);
 // 2: adjoint epilogue with coil sum; 3: adjoint epilogue per coil.
-template <int TILE, int MODE>
-__global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const float2* __restrict__ tw) {
+template <int TILE, int NT, int MODE>
+__global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
     const int N = a.N;
     const int64_t tile_w = blockIdx.x % a.ntile_w;
@@ -196,7 +195,7 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
 
     // rows outside the input window are zero
     const int nzero = (N - a.in_cnt) * L;
-    for (int e = threadIdx.x; e < nzero; e += kThreads) {
+    for (int e = threadIdx.x; e < nzero; e += NT) {
         int row = e / L, line = e - row * L;
         int n = row < a.in_off ? row : row + a.in_cnt;
         lds[n * L + line] = make_float2(0.f, 0.f);
@@ -205,11 +204,11 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
     // work-item's loads are issued before any LDS write so that they are in
     // flight together.
     const int nin = a.in_cnt * L;
-    constexpr int EPT = TILE / kThreads;
-    if (MODE == 0 && Bt == 1 && (kThreads % Wt) == 0) {
+    constexpr int EPT = TILE / NT;
+    if (MODE == 0 && Bt == 1 && (NT % Wt) == 0) {
         // one outer index, Wt divides the workgroup: w is fixed per work-item
         const int w = threadIdx.x % Wt;
-        const int rstep = kThreads / Wt;
+        const int rstep = NT / Wt;
         const int r0 = threadIdx.x / Wt;
         const float2* src = a.in + o0 * a.in_os + w0 + w;
         float2 v[EPT];
@@ -232,7 +231,7 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
         float2 v[EPT];
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
-            int e = threadIdx.x + i * kThreads;
+            int e = threadIdx.x + i * NT;
             if (e < nin) {
                 int w = e % Wt;
                 int rest = e / Wt;
@@ -249,7 +248,7 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
         }
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
-            int e = threadIdx.x + i * kThreads;
+            int e = threadIdx.x + i * NT;
             if (e < nin) {
                 int w = e % Wt;
                 int rest = e / Wt;
@@ -269,11 +268,11 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
     for (int s = 0; s < a.nstage; s++) {
         int R = a.radix[s];
         switch (R) {
-            case 8: stage<TILE, 8>(lds, L, N, Ns, tw, sg); break;
-            case 5: stage<TILE, 5>(lds, L, N, Ns, tw, sg); break;
-            case 4: stage<TILE, 4>(lds, L, N, Ns, tw, sg); break;
-            case 3: stage<TILE, 3>(lds, L, N, Ns, tw, sg); break;
-            default: stage<TILE, 2>(lds, L, N, Ns, tw, sg); break;
+            case 8: stage<TILE, NT, 8>(lds, L, N, Ns, tw, sg); break;
+            case 5: stage<TILE, NT, 5>(lds, L, N, Ns, tw, sg); break;
+            case 4: stage<TILE, NT, 4>(lds, L, N, Ns, tw, sg); break;
+            case 3: stage<TILE, NT, 3>(lds, L, N, Ns, tw, sg); break;
+            default: stage<TILE, NT, 2>(lds, L, N, Ns, tw, sg); break;
         }
         Ns *= R;
     }
@@ -283,7 +282,7 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
     if (MODE == 2) {
         // one work-item per (outer, kept row): sum over the coils of the line
         const int nout = a.out_cnt * Bt;
-        for (int e = threadIdx.x; e < nout; e += kThreads) {
+        for (int e = threadIdx.x; e < nout; e += NT) {
             int kl = e % a.out_cnt, ob = e / a.out_cnt;
             int k = a.out_off + kl;
             float sc = a.scale;
@@ -303,7 +302,7 @@ __global__ void __launch_bounds__(kThreads) fft_pass_kernel(PassArgs a, const fl
         return;
     }
     const int nout = a.out_cnt * L;
-    for (int e = threadIdx.x; e < nout; e += kThreads) {
+    for (int e = threadIdx.x; e < nout; e += NT) {
         int w = e % Wt;
         int rest = e / Wt;
         int kl = rest % a.out_cnt;
@@ -348,22 +347,28 @@ TwiddleCache& twiddles() {
     return c;
 }
 
-int tile_elems() {
+// Tile of the fused (prologue / epilogue) passes: a whole [N][nCh] line
+// group must fit, so it is the largest one (64 KiB, 512 work-items).
+constexpr int kFusedTile = 8192;
+// Tile of the plain passes: 2048, 4096 or 8192 complex values
+// (MONA_FFT_PLAIN_TILE; default 4096 = 32 KiB, 256 work-items, several
+// workgroups per CU).
+int plain_tile() {
     static int t = [] {
-        const char* e = std::getenv("MONA_FFT_TILE");
-        int v = e ? std::atoi(e) : 8192;
-        return (v == 4096) ? 4096 : 8192;
+        const char* e = std::getenv("MONA_FFT_PLAIN_TILE");
+        int v = e ? std::atoi(e) : 4096;
+        return (v == 2048 || v == 8192) ? v : 4096;
     }();
     return t;
 }
 
-template <int TILE>
+template <int TILE, int NT>
 void launch(const PassArgs& a, int mode, unsigned nblocks, size_t lds, const float2* tw, hipStream_t s) {
     switch (mode) {
-        case 1: fft_pass_kernel<TILE, 1><<<nblocks, kThreads, lds, s>>>(a, tw); break;
-        case 2: fft_pass_kernel<TILE, 2><<<nblocks, kThreads, lds, s>>>(a, tw); break;
-        case 3: fft_pass_kernel<TILE, 3><<<nblocks, kThreads, lds, s>>>(a, tw); break;
-        default: fft_pass_kernel<TILE, 0><<<nblocks, kThreads, lds, s>>>(a, tw); break;
+        case 1: fft_pass_kernel<TILE, NT, 1><<<nblocks, NT, lds, s>>>(a, tw); break;
+        case 2: fft_pass_kernel<TILE, NT, 2><<<nblocks, NT, lds, s>>>(a, tw); break;
+        case 3: fft_pass_kernel<TILE, NT, 3><<<nblocks, NT, lds, s>>>(a, tw); break;
+        default: fft_pass_kernel<TILE, NT, 0><<<nblocks, NT, lds, s>>>(a, tw); break;
     }
 }
 
@@ -377,7 +382,7 @@ bool fft_size_supported(int N) {
     return n == 1;
 }
 
-bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= tile_elems(); }
+bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= kFusedTile; }
 
 void fft_pass(const AxisPass& p, hipStream_t s) {
     const int N = p.N;
@@ -389,7 +394,7 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     MONA_CHECK(p.out_off >= 0 && p.out_cnt >= 1 && p.out_off + p.out_cnt <= N, MONA_ERR_INTERNAL,
                "bad output window");
     MONA_CHECK(p.S >= 1 && p.outer >= 1, MONA_ERR_INTERNAL, "empty FFT pass");
-    const int TILE = tile_elems();
+    const int TILE = p.mode == 0 ? plain_tile() : kFusedTile;
     PassArgs a{};
     a.N = N;
     int n = N, ns = 0;
@@ -437,8 +442,9 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     size_t lds = (size_t)a.W * a.Bo * N * sizeof(float2);
     MONA_CHECK(lds <= (size_t)TILE * sizeof(float2), MONA_ERR_INTERNAL, "FFT tile too large");
     const float2* tw = twiddles().get(N);
-    if (TILE == 4096) launch<4096>(a, p.mode, (unsigned)nblocks, lds, tw, s);
-    else launch<8192>(a, p.mode, (unsigned)nblocks, lds, tw, s);
+    if (TILE == 2048) launch<2048, 256>(a, p.mode, (unsigned)nblocks, lds, tw, s);
+    else if (TILE == 4096) launch<4096, 256>(a, p.mode, (unsigned)nblocks, lds, tw, s);
+    else launch<8192, 512>(a, p.mode, (unsigned)nblocks, lds, tw, s);
     MONA_HIP(hipGetLastError());
 }
 

@@ -150,6 +150,12 @@ void sparse_set_host(SparseDev& S, int r_size, std::vector<int>&& r_jump, std::v
                      std::vector<float>&& m_val, int l_size, std::vector<int>&& l_jump);
 // Uploads and decodes on the current device (idempotent, thread-safe).
 void sparse_upload(SparseDev& S, hipStream_t s);
+// Device matrix from decoded host arrays (absolute columns, per-row counts,
+// optional output rows); no host copy of the jump encoding is kept.
+void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector<int>& col,
+                         const std::vector<float>& val, const std::vector<int>& r_nJump,
+                         const std::vector<int>* rowidx, hipStream_t s);
+void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s);
 
 // Rows with more entries than this are summed in chunks by the operator path.
 constexpr int kLongRow = 64;

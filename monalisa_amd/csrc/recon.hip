@@ -183,6 +183,30 @@ __global__ void k_xpby_dev(float2* p, const float2* d, const double* scal, int64
         p[i] = make_float2(dv.x + b * pv.x, dv.y + b * pv.y);
     }
 }
+// dst[s][c] = src[perm[s]][c]  (caller order -> tile order)
+__global__ void k_gather_pts(const float2* __restrict__ src, const int* __restrict__ perm, float2* __restrict__ dst,
+                             int64_t P, int nCh) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * nCh) return;
+    int64_t s = t / nCh;
+    int c = (int)(t - s * nCh);
+    dst[t] = src[(int64_t)perm[s] * nCh + c];
+}
+// dst[perm[s]][c] = src[s][c]  (tile order -> caller order)
+__global__ void k_scatter_pts(const float2* __restrict__ src, const int* __restrict__ perm, float2* __restrict__ dst,
+                              int64_t P, int nCh) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * nCh) return;
+    int64_t s = t / nCh;
+    int c = (int)(t - s * nCh);
+    dst[(int64_t)perm[s] * nCh + c] = src[t];
+}
+__global__ void k_gather_w(const float* __restrict__ src, const int* __restrict__ perm, float* __restrict__ dst,
+                           int64_t P) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P) dst[s] = src[perm[s]];
+}
+
 __global__ void k_hy(const float* ve, float vmax, float* hy, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) hy[i] = fminf(ve[i], vmax);
@@ -367,9 +391,95 @@ void Plan::set_coils(const float2* C, hipStream_t s) {
 }
 
 const Frame& Plan::frame(int f) const {
-    MONA_CHECK(f >= 0 && f < (int)frames.size() && frames[f].Gu && frames[f].Gut, MONA_ERR_STATE,
+    MONA_CHECK(f >= 0 && f < (int)frames.size() && frames[f].Gu_s && frames[f].Gut_s, MONA_ERR_STATE,
                "frame " + std::to_string(f) + " has no gridding matrices");
     return frames[f];
+}
+
+// Tile-order copies of a frame's matrices (host decode of the reference
+// encoding, stable sort of the points by 8^3 cell tile, upload).
+void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s) {
+    MONA_CHECK(f >= 0, MONA_ERR_ARG, "negative frame index");
+    MONA_CHECK(!Gu->squeezed && Gu->l_nJump == Gu->l_size, MONA_ERR_ARG,
+               "Gu must have one stored row per point (not squeezed)");
+    MONA_CHECK((int64_t)Gu->h_r_jump.size() == Gu->nnz && (int64_t)Gut->h_r_jump.size() == Gut->nnz,
+               MONA_ERR_STATE, "gridding matrices without their host encoding");
+    const int64_t P = Gu->l_size;
+    // decode Gu's columns (running deltas, first absolute)
+    std::vector<int64_t> rowptr(P + 1, 0);
+    for (int64_t p = 0; p < P; p++) rowptr[p + 1] = rowptr[p] + Gu->h_r_nJump[p];
+    std::vector<int> col(Gu->nnz);
+    {
+        int64_t c = 0;
+        for (int64_t k = 0; k < Gu->nnz; k++) {
+            c += Gu->h_r_jump[k];
+            col[k] = (int)c;
+        }
+    }
+    // tile key of each point: the tile of its first (lowest) cell
+    const int tx = (N[0] + 7) / 8, ty = (N[1] + 7) / 8;
+    std::vector<int64_t> key(P);
+    for (int64_t p = 0; p < P; p++) {
+        if (rowptr[p + 1] == rowptr[p]) { key[p] = INT64_MAX; continue; }
+        int64_t cell = col[rowptr[p]];
+        int64_t x = cell % N[0], y = (cell / N[0]) % N[1], z = cell / ((int64_t)N[0] * N[1]);
+        key[p] = ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
+    }
+    std::vector<int> perm(P), inv(P);
+    for (int64_t p = 0; p < P; p++) perm[p] = (int)p;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    for (int64_t s2 = 0; s2 < P; s2++) inv[perm[s2]] = (int)s2;
+    // Gu in tile order
+    std::vector<int> scol(Gu->nnz), snj(P);
+    std::vector<float> sval(Gu->nnz);
+    {
+        int64_t o = 0;
+        for (int64_t s2 = 0; s2 < P; s2++) {
+            int64_t p = perm[s2];
+            int n = Gu->h_r_nJump[p];
+            snj[s2] = n;
+            for (int64_t k = rowptr[p]; k < rowptr[p + 1]; k++, o++) {
+                scol[o] = col[k];
+                sval[o] = Gu->h_m_val[k];
+            }
+        }
+    }
+    auto Gu_s = std::make_shared<SparseDev>();
+    sparse_from_decoded(*Gu_s, Gu->r_size, Gu->l_size, scol, sval, snj, nullptr, s);
+    // Gut with relabelled point columns (entries of each cell keep their order)
+    std::vector<int> tcol(Gut->nnz);
+    {
+        int64_t c = 0;
+        for (int64_t k = 0; k < Gut->nnz; k++) {
+            c += Gut->h_r_jump[k];
+            MONA_CHECK(c >= 0 && c < P, MONA_ERR_ARG, "Gut column outside [0, nPt)");
+            tcol[k] = inv[c];
+        }
+    }
+    std::vector<int> rows;
+    const std::vector<int>* rp = nullptr;
+    if (Gut->squeezed) {
+        rows.resize(Gut->l_nJump);
+        int64_t r = 0;
+        for (int i = 0; i < Gut->l_nJump; i++) {
+            r += Gut->h_l_jump[i];
+            rows[i] = (int)r;
+        }
+        rp = &rows;
+    }
+    auto Gut_s = std::make_shared<SparseDev>();
+    sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, tcol, Gut->h_m_val, Gut->h_r_nJump, rp, s);
+    auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
+    MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipStreamSynchronize(s));
+    if ((int)frames.size() <= f) frames.resize(f + 1);
+    Frame& fr = frames[f];
+    fr.Gu = Gu;
+    fr.Gut = Gut;
+    fr.Gu_s = Gu_s;
+    fr.Gut_s = Gut_s;
+    fr.perm = dperm;
+    fr.P = P;
 }
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
@@ -390,7 +500,7 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
     }
     {
         TimedRegion t(0, s);
-        spmv_c2_rowmajor(*fr.Gu, gr, y, nCh, nullptr, s);
+        spmv_c2_rowmajor(*fr.Gu_s, gr, y, nCh, nullptr, s);
     }
 }
 
@@ -489,7 +599,7 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     float2* gr = grid.as<float2>();
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*fr.Gut, y, gr, nCh, hy, s);
+        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s);
     }
     {
         TimedRegion t(2, s);
@@ -501,6 +611,33 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
             launch_check();
         }
     }
+}
+
+void Plan::to_tile_order(int f, const float2* src, float2* dst, hipStream_t s) {
+    const Frame& fr = frame(f);
+    k_gather_pts<<<nb256(fr.P * nCh), 256, 0, s>>>(src, fr.perm->as<int>(), dst, fr.P, nCh);
+    launch_check();
+}
+
+void Plan::weights_to_tile_order(int f, const float* src, float* dst, hipStream_t s) {
+    const Frame& fr = frame(f);
+    k_gather_w<<<nb256(fr.P), 256, 0, s>>>(src, fr.perm->as<int>(), dst, fr.P);
+    launch_check();
+}
+
+void Plan::forward_public(int f, const float2* x, float2* y, hipStream_t s) {
+    const Frame& fr = frame(f);
+    work.kpub.ensure((size_t)fr.P * nCh * sizeof(float2));
+    forward(f, x, work.kpub.as<float2>(), s);
+    k_scatter_pts<<<nb256(fr.P * nCh), 256, 0, s>>>(work.kpub.as<float2>(), fr.perm->as<int>(), y, fr.P, nCh);
+    launch_check();
+}
+
+void Plan::adjoint_public(int f, const float2* y, float2* x, bool combine, hipStream_t s) {
+    const Frame& fr = frame(f);
+    work.kpub.ensure((size_t)fr.P * nCh * sizeof(float2));
+    to_tile_order(f, y, work.kpub.as<float2>(), s);
+    adjoint(f, work.kpub.as<float2>(), x, combine, nullptr, 1.0f, s);
 }
 
 // ---------------------------------------------------------------- vector helpers
@@ -584,7 +721,13 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
     float2* p = w.i2.as<float2>();
     CgScalars* sc = w.sc.as<CgScalars>();
     double* part = w.partial.as<double>();
-    compute_hy(ve, ve_max, hy, P, s);
+    // k-space data and weights in the frame's tile order for the whole solve
+    w.hy0.ensure((size_t)P * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    compute_hy(ve, ve_max, w.hy0.as<float>(), P, s);
+    weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
+    to_tile_order(f, y, w.ys.as<float2>(), s);
+    y = w.ys.as<float2>();
     const float invHX = (float)(1.0 / HX);
 
     for (int it = 0; it < nIter; it++) {
@@ -663,11 +806,7 @@ int mona_plan_set_frame(mona_plan P, int frame, mona_sparse Gu, mona_sparse Gut)
             std::lock_guard<std::mutex> lk(C.mu);
             C.init();
         }
-        sparse_upload(Gu->S, C.stream);
-        sparse_upload(Gut->S, C.stream);
-        if ((int)P->P.frames.size() <= frame) P->P.frames.resize(frame + 1);
-        P->P.frames[frame].Gu = &Gu->S;
-        P->P.frames[frame].Gut = &Gut->S;
+        P->P.set_frame(frame, &Gu->S, &Gut->S, C.stream);
     });
 }
 
@@ -690,15 +829,14 @@ int mona_plan_get_KFC(mona_plan P, void* KFC, void* KFCc, hipStream_t stream) {
 int mona_shanna(mona_plan P, int frame, const void* x, void* y, hipStream_t stream) {
     return guarded([&] {
         MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
-        P->P.forward(frame, static_cast<const float2*>(x), static_cast<float2*>(y), stream);
+        P->P.forward_public(frame, static_cast<const float2*>(x), static_cast<float2*>(y), stream);
     });
 }
 
 int mona_nakatsha(mona_plan P, int frame, const void* y, void* x, int C_flag, hipStream_t stream) {
     return guarded([&] {
         MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
-        P->P.adjoint(frame, static_cast<const float2*>(y), static_cast<float2*>(x), C_flag != 0, nullptr,
-                     1.0f, stream);
+        P->P.adjoint_public(frame, static_cast<const float2*>(y), static_cast<float2*>(x), C_flag != 0, stream);
     });
 }
 

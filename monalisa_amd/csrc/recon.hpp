@@ -1,5 +1,6 @@
 // Reconstruction plan and operator declarations (recon.hip, admm.hip).
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "mona_internal.hpp"
@@ -20,9 +21,18 @@ struct CgScalars {
     int done, pad;
 };
 
+// A frame's gridding matrices.  The operators run on copies whose points
+// (rows of Gu, columns of Gut) are in tile order -- points sorted by the 8^3
+// cell tile that holds their first neighbour, ties kept in input order -- so
+// that consecutive work-items touch neighbouring grid cells and k-space
+// points.  Every row keeps its entries in the caller's order, so the sums
+// are unchanged.  perm[s] is the caller's index of sorted point s.
 struct Frame {
-    const SparseDev* Gu = nullptr;
+    const SparseDev* Gu = nullptr;   // caller's matrices
     const SparseDev* Gut = nullptr;
+    std::shared_ptr<SparseDev> Gu_s, Gut_s;
+    std::shared_ptr<DevBuf> perm;    // int32[P]
+    int64_t P = 0;
 };
 
 // Records HIP events around a region when kernel timing is enabled.
@@ -39,6 +49,8 @@ class TimedRegion {
 
 struct Work {
     DevBuf hy, k1, k2, i1, i2, sc, partial, hist;
+    DevBuf ys, hy0;             // k-space data and weights in tile order
+    DevBuf kpub;                // staging of the public operator calls
     DevBuf z1, z2, z3, z4, z5;  // ADMM image-space buffers
 };
 
@@ -68,11 +80,20 @@ struct Plan {
     void forward_fft_pruned(const float2* x, float2* grid, hipStream_t s);
     void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, hipStream_t s);
 
-    // y = M x  (x [nu], y [P][nCh])
+    void set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s);
+
+    // y = M x  (x [nu], y [P][nCh] in the frame's tile order)
     void forward(int f, const float2* x, float2* y, hipStream_t s);
-    // x = scale * sum_c KFC_conj .* crop(cifft(Gut (hy .* y)))   (hy may be NULL)
+    // x = scale * sum_c KFC_conj .* crop(cifft(Gut (hy .* y)))   (hy may be
+    // NULL; y and hy in tile order)
     void adjoint(int f, const float2* y, float2* x, bool combine, const float* hy, float scale,
                  hipStream_t s);
+    // The same operators with k-space in the caller's point order.
+    void forward_public(int f, const float2* x, float2* y, hipStream_t s);
+    void adjoint_public(int f, const float2* y, float2* x, bool combine, hipStream_t s);
+    // tile-order copies of caller-order k-space data / point weights
+    void to_tile_order(int f, const float2* src, float2* dst, hipStream_t s);
+    void weights_to_tile_order(int f, const float* src, float* dst, hipStream_t s);
     void sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
                double* residual, hipStream_t s);
 };

@@ -338,38 +338,69 @@ void sparse_upload(SparseDev& S, hipStream_t s) {
         MONA_HIP(hipMemcpyAsync(d_lj.p, S.h_l_jump.data(), (size_t)l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
     }
     decode_device(S, d_rj.as<int>(), d_rn.as<int>(), d_mv.as<float>(), d_lj.as<int>(), 1, nullptr, s);
-    // chunk lists of the long rows: chunk c covers entries
-    // [chunk_be[2c], chunk_be[2c+1]); long row r owns chunks
-    // long_first[r] .. long_first[r+1]-1
-    {
-        std::vector<int> lrow, first;
-        std::vector<int64_t> be;
-        int64_t off = 0;
-        for (int i = 0; i < l_nJump; i++) {
-            int n = S.h_r_nJump[i];
-            if (n > kLongRow) {
-                lrow.push_back(i);
-                first.push_back((int)(be.size() / 2));
-                for (int j = 0; j < n; j += kLongRow) {
-                    be.push_back(off + j);
-                    be.push_back(off + std::min(n, j + kLongRow));
-                }
+    build_chunks(S, S.h_r_nJump.data(), s);
+    MONA_HIP(hipStreamSynchronize(s));
+    S.on_device = true;
+}
+
+// Chunk lists of the long rows: chunk c covers entries [chunk_be[2c],
+// chunk_be[2c+1]); long row r owns chunks long_first[r] .. long_first[r+1]-1.
+void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s) {
+    std::vector<int> lrow, first;
+    std::vector<int64_t> be;
+    int64_t off = 0;
+    for (int i = 0; i < S.l_nJump; i++) {
+        int n = r_nJump[i];
+        if (n > kLongRow) {
+            lrow.push_back(i);
+            first.push_back((int)(be.size() / 2));
+            for (int j = 0; j < n; j += kLongRow) {
+                be.push_back(off + j);
+                be.push_back(off + std::min(n, j + kLongRow));
             }
-            off += n;
         }
-        first.push_back((int)(be.size() / 2));
-        S.n_long = (int)lrow.size();
-        S.n_chunk = (int64_t)(be.size() / 2);
-        if (S.n_long > 0) {
-            S.long_row.alloc(lrow.size() * sizeof(int));
-            S.long_first.alloc(first.size() * sizeof(int));
-            S.chunk_begin.alloc(be.size() * sizeof(int64_t));
-            MONA_HIP(hipMemcpyAsync(S.long_row.p, lrow.data(), lrow.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            MONA_HIP(hipMemcpyAsync(S.long_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            MONA_HIP(hipMemcpyAsync(S.chunk_begin.p, be.data(), be.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        }
+        off += n;
+    }
+    first.push_back((int)(be.size() / 2));
+    S.n_long = (int)lrow.size();
+    S.n_chunk = (int64_t)(be.size() / 2);
+    if (S.n_long > 0) {
+        S.long_row.alloc(lrow.size() * sizeof(int));
+        S.long_first.alloc(first.size() * sizeof(int));
+        S.chunk_begin.alloc(be.size() * sizeof(int64_t));
+        MONA_HIP(hipMemcpyAsync(S.long_row.p, lrow.data(), lrow.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(S.long_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(S.chunk_begin.p, be.data(), be.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
     }
     MONA_HIP(hipStreamSynchronize(s));
+}
+
+void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector<int>& col,
+                         const std::vector<float>& val, const std::vector<int>& r_nJump,
+                         const std::vector<int>* rowidx, hipStream_t s) {
+    S.r_size = r_size;
+    S.l_size = l_size;
+    S.l_nJump = (int)r_nJump.size();
+    S.nnz = (int64_t)col.size();
+    S.squeezed = rowidx != nullptr;
+    std::vector<int64_t> rowptr(S.l_nJump + 1, 0);
+    int mx = 0;
+    for (int i = 0; i < S.l_nJump; i++) {
+        rowptr[i + 1] = rowptr[i] + r_nJump[i];
+        mx = std::max(mx, r_nJump[i]);
+    }
+    S.max_row = mx;
+    S.col.alloc(std::max<int64_t>(S.nnz, 1) * sizeof(int));
+    S.val.alloc(std::max<int64_t>(S.nnz, 1) * sizeof(float));
+    S.rowptr.alloc(rowptr.size() * sizeof(int64_t));
+    MONA_HIP(hipMemcpyAsync(S.col.p, col.data(), S.nnz * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.val.p, val.data(), S.nnz * sizeof(float), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.rowptr.p, rowptr.data(), rowptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (rowidx) {
+        S.rowidx.alloc(rowidx->size() * sizeof(int));
+        MONA_HIP(hipMemcpyAsync(S.rowidx.p, rowidx->data(), rowidx->size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    build_chunks(S, r_nJump.data(), s);
     S.on_device = true;
 }
 
