@@ -136,6 +136,38 @@ template <> struct Dft<8> {
     }
 };
 
+// Radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
+template <> struct Dft<16> {
+    __device__ static void run(float2* a, float sg) {
+        // cos / sin of 2 pi m / 16, m = 0..9
+        const float c[10] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                             0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+                             -1.0f, -0.92387953251128675613f};
+        const float sn[10] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+                              1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                              0.0f, -0.38268343236508977173f};
+        float2 b[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            float2 t[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+            Dft<4>::run(t, sg);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) {
+                int m = n2 * k1;  // w16^m, m <= 9
+                float2 w = make_float2(c[m], sg * sn[m]);
+                b[n2][k1] = (m == 0) ? t[k1] : cmul(t[k1], w);
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            float2 t[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+            Dft<4>::run(t, sg);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) a[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
 // One Stockham stage of radix R over L lines of length N held transposed
 // in LDS ([n][line]).  j in [0, N/R): inputs at rows j + r*N/R, twiddle
 // w_(Ns R)^(j%Ns * r), outputs at rows (j/Ns)*Ns*R + j%Ns + r*Ns.
@@ -182,9 +214,18 @@ __device__ void stage(float2* lds, int L, int N, int Ns, const float2* __restric
 // MODE 0: plain pass; 1: forward prologue (input = img .* KFC);
 // 2: adjoint epilogue with coil sum; 3: adjoint epilogue per coil.
 template <int TILE, int NT, int MODE>
-__global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* __restrict__ tw) {
-    extern __shared__ float2 lds[];
+__global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* __restrict__ twg) {
+    // tiles up to 32 KiB also stage the twiddle table in LDS (in front of
+    // the tile); the 64 KiB fused tiles read it from global memory
+    constexpr bool TWL = TILE <= 4096;
+    extern __shared__ float2 smem[];
     const int N = a.N;
+    float2* lds = smem + (TWL ? N : 0);
+    const float2* tw = twg;
+    if (TWL) {
+        for (int i = threadIdx.x; i < N; i += NT) smem[i] = twg[i];
+        tw = smem;  // visible after the barrier that ends the load phase
+    }
     const int64_t tile_w = blockIdx.x % a.ntile_w;
     const int64_t to = blockIdx.x / a.ntile_w;
     const int64_t w0 = tile_w * a.W;
@@ -268,6 +309,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
     for (int s = 0; s < a.nstage; s++) {
         int R = a.radix[s];
         switch (R) {
+            case 16: stage<TILE, NT, 16>(lds, L, N, Ns, tw, sg); break;
             case 8: stage<TILE, NT, 8>(lds, L, N, Ns, tw, sg); break;
             case 5: stage<TILE, NT, 5>(lds, L, N, Ns, tw, sg); break;
             case 4: stage<TILE, NT, 4>(lds, L, N, Ns, tw, sg); break;
@@ -398,6 +440,7 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     PassArgs a{};
     a.N = N;
     int n = N, ns = 0;
+    while (n % 16 == 0) { a.radix[ns++] = 16; n /= 16; }
     while (n % 8 == 0 && n > 4) { a.radix[ns++] = 8; n /= 8; }
     while (n % 4 == 0) { a.radix[ns++] = 4; n /= 4; }
     while (n % 2 == 0) { a.radix[ns++] = 2; n /= 2; }
@@ -440,7 +483,9 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
     size_t lds = (size_t)a.W * a.Bo * N * sizeof(float2);
-    MONA_CHECK(lds <= (size_t)TILE * sizeof(float2), MONA_ERR_INTERNAL, "FFT tile too large");
+    if (TILE <= 4096) lds += (size_t)N * sizeof(float2);  // twiddles staged in LDS
+    MONA_CHECK(lds <= (size_t)(TILE + (TILE <= 4096 ? N : 0)) * sizeof(float2), MONA_ERR_INTERNAL,
+               "FFT tile too large");
     const float2* tw = twiddles().get(N);
     if (TILE == 2048) launch<2048, 256>(a, p.mode, (unsigned)nblocks, lds, tw, s);
     else if (TILE == 4096) launch<4096, 256>(a, p.mode, (unsigned)nblocks, lds, tw, s);
