@@ -264,6 +264,18 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
 int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x,
                hipStream_t stream);
 
+/* Plan options (no reference counterpart: execution choices of this
+ * implementation). */
+#define MONA_OPT_BATCH_FRAMES 1 /* temporal-TV solver: all frames in one launch per gridding op
+                                   (block-diagonal matrices, frame-batched FFT passes) */
+#define MONA_OPT_GRAPH 2        /* temporal-TV solver: ADMM iterations replayed from a HIP graph
+                                   (constant delta/rho schedules) */
+#define MONA_OPT_FFT_TILE 3     /* plain FFT pass tile: 2048, 4096 or 8192 complex values, 0 = default */
+int mona_plan_set_option(mona_plan P, int option, long long value);
+/* Autotuner: times forward + adjoint of one frame for each FFT tile and keeps
+ * the fastest; *chosen (may be NULL) receives it. */
+int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* chosen);
+
 /* Kernel timing: when enabled, every gridding launch (forward Gu product,
  * adjoint Gut product) records HIP events on its stream; the accumulated
  * launch count and total milliseconds per kind are read with

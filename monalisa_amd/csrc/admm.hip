@@ -180,6 +180,13 @@ void abs1(const float2* a, int64_t n, double scale, double* out, double* partial
 // objective = 0.5 * dafi + 0.5 * delta * regul
 __global__ void k_objective(double* h, double delta) { h[0] = 0.5 * h[1] + 0.5 * delta * h[2]; }
 
+// hist[n*ctr .. n*ctr+n-1] = slot[0..n-1]; ctr += 1
+__global__ void k_hist_commit(double* hist, const double* slot, int* ctr, int n) {
+    int c = *ctr;
+    for (int i = 0; i < n; i++) hist[(int64_t)n * c + i] = slot[i];
+    *ctr = c + 1;
+}
+
 // Gridded-recon epilogue (bmNasha.m:66-87): per-coil image
 // v_c = grid[crop(i)][c] * F * K[i]; combined sum(conj(C) v)/sum|C|^2.
 __global__ void k_nasha_epilogue(const float2* __restrict__ grid, const float* __restrict__ K,
@@ -364,20 +371,33 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
     }
     const float2* ysorted = w.ys.as<float2>();
     auto yF = [&](int f) { return ysorted + koff[f]; };
+    // operators: one launch per gridding op over all frames (batched form), or
+    // frame by frame
+    if (P.use_batch && (!P.Gu_b || (int)P.koff_b.size() != nFr + 1)) P.build_batch(s);
+    hipStream_t cs = s;  // stream the iteration body is issued on
     auto M = [&](const float2* xx, float2* out) {
-        for (int f = 0; f < nFr; f++) P.forward(f, xx + (int64_t)f * nu, out + koff[f], s);
+        if (P.use_batch) { P.forward_batch(xx, out, cs); return; }
+        for (int f = 0; f < nFr; f++) P.forward(f, xx + (int64_t)f * nu, out + koff[f], cs);
     };
     auto Mh = [&](const float2* r, float2* out) {
+        if (P.use_batch) { P.adjoint_batch(r, out, hy, invHX, cs); return; }
         for (int f = 0; f < nFr; f++)
-            P.adjoint(f, r + koff[f], out + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, s);
+            P.adjoint(f, r + koff[f], out + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, cs);
     };
     if (!zu_given) {
         k_tdiff<<<nb(nx), 256, 0, s>>>(x, z, nullptr, nullptr, nFr, nu);
         MONA_HIP(hipGetLastError());
         fill_f32(reinterpret_cast<float*>(u), 0.0f, nx * 2, s);
     }
-    for (int c = 0; c < nIter; c++) {
-        float dl = delta[c], rc = rho[c];
+    // monitoring values go to a fixed slot and are committed to hist[3*iter]
+    // by a kernel that reads and advances a device counter, so the body has
+    // no per-iteration pointers and can be replayed from a graph
+    w.z5.ensure(4 * sizeof(double) + sizeof(int));
+    double* hslot = w.z5.as<double>();
+    int* hctr = reinterpret_cast<int*>(hslot + 4);
+    fill_i32(hctr, 0, 1, s);
+    auto body = [&](float dl, float rc) {
+        hipStream_t s = cs;
         cg_reset(sc, s);
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
@@ -413,12 +433,64 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
         // monitoring: dafi, regul (private_dafi_regul), objective
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
-        sqnorm(res_y, ny, hy, P.nCh, 1.0, hist + 3 * c + 1, false, part, s);
+        sqnorm(res_y, ny, hy, P.nCh, 1.0, hslot + 1, false, part, s);
         k_tdiff<<<nb(nx), 256, 0, s>>>(x, Fp, nullptr, nullptr, nFr, nu);
         MONA_HIP(hipGetLastError());
-        abs1(Fp, nx, (double)HZ, hist + 3 * c + 2, part, s);
-        k_objective<<<1, 1, 0, s>>>(hist + 3 * c, (double)dl);
+        abs1(Fp, nx, (double)HZ, hslot + 2, part, s);
+        k_objective<<<1, 1, 0, s>>>(hslot, (double)dl);
         MONA_HIP(hipGetLastError());
+        k_hist_commit<<<1, 1, 0, s>>>(hist, hslot, hctr, 3);
+        MONA_HIP(hipGetLastError());
+    };
+    bool constant = true;
+    for (int c = 1; c < nIter; c++) constant = constant && delta[c] == delta[0] && rho[c] == rho[0];
+    if (P.use_graph && constant && nIter > 1) {
+        // iteration 0 eagerly (allocates every work buffer), then one
+        // captured iteration replayed nIter - 1 times, on a private stream
+        // ordered after / before the caller's stream
+        struct GraphRun {  // releases everything on every exit path
+            hipStream_t gs = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            ~GraphRun() {
+                if (gs) {
+                    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+                    if (hipStreamIsCapturing(gs, &st) == hipSuccess && st != hipStreamCaptureStatusNone) {
+                        hipGraph_t g = nullptr;
+                        (void)hipStreamEndCapture(gs, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                    }
+                    (void)hipStreamSynchronize(gs);
+                }
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (graph) (void)hipGraphDestroy(graph);
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                if (gs) (void)hipStreamDestroy(gs);
+            }
+        } g;
+        MONA_HIP(hipStreamCreateWithFlags(&g.gs, hipStreamNonBlocking));
+        MONA_HIP(hipEventCreateWithFlags(&g.e0, hipEventDisableTiming));
+        MONA_HIP(hipEventCreateWithFlags(&g.e1, hipEventDisableTiming));
+        MONA_HIP(hipEventRecord(g.e0, s));
+        MONA_HIP(hipStreamWaitEvent(g.gs, g.e0, 0));
+        cs = g.gs;
+        struct RestoreStream {
+            hipStream_t& r;
+            hipStream_t v;
+            ~RestoreStream() { r = v; }
+        } restore{cs, s};
+        body(delta[0], rho[0]);
+        MONA_HIP(hipStreamBeginCapture(g.gs, hipStreamCaptureModeThreadLocal));
+        body(delta[0], rho[0]);
+        MONA_HIP(hipStreamEndCapture(g.gs, &g.graph));
+        MONA_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        for (int c = 1; c < nIter; c++) MONA_HIP(hipGraphLaunch(g.exec, g.gs));
+        MONA_HIP(hipEventRecord(g.e1, g.gs));
+        MONA_HIP(hipStreamWaitEvent(s, g.e1, 0));
+    } else {
+        for (int c = 0; c < nIter; c++) body(delta[c], rho[c]);
     }
     if (hist_out && nIter > 0) {
         MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * 3 * sizeof(double), hipMemcpyDeviceToHost, s));

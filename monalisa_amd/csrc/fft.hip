@@ -51,6 +51,7 @@ struct PassArgs {
     float scale, epi_scale;
     const float2* img;   // prologue: image, [outer][in_cnt]
     const float2* kfc;   // prologue: KFC [outer*in_cnt][S]; epilogue: KFC_conj [outer*out_cnt][S]
+    int64_t kfc_period;  // KFC index taken modulo this many pixels (frames share the maps)
     float2* img_out;     // epilogue (combine): [outer][out_cnt]
 };
 
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
                 int64_t o = o0 + ob;
                 if (MODE == 1) {
                     int64_t ii = o * a.in_cnt + r;
-                    v[i] = cmul(a.img[ii], a.kfc[ii * a.S + w0 + w]);
+                    v[i] = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + w0 + w]);
                 } else {
                     v[i] = a.in[o * a.in_os + (int64_t)r * a.S + w0 + w];
                 }
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
             float sc = a.scale;
             if (a.postsign && ((k + half) & 1)) sc = -sc;
             int64_t i = (o0 + ob) * a.out_cnt + kl;
-            const float2* kc = a.kfc + i * a.S;
+            const float2* kc = a.kfc + (i % a.kfc_period) * a.S;
             float2 acc = make_float2(0.f, 0.f);
             for (int c = 0; c < Wt; c++) {
                 float2 v = lds[k * L + ob * Wt + c];
@@ -356,7 +357,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
         v = make_float2(v.x * sc, v.y * sc);
         if (MODE == 3) {
             int64_t i = (o0 + ob) * a.out_cnt + kl;
-            float2 p = cmul(v, a.kfc[i * a.S + w0 + w]);
+            float2 p = cmul(v, a.kfc[(i % a.kfc_period) * a.S + w0 + w]);
             a.out[i * a.S + w0 + w] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
         } else {
             a.out[(o0 + ob) * a.out_os + (int64_t)kl * a.S + w0 + w] = v;
@@ -436,7 +437,8 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     MONA_CHECK(p.out_off >= 0 && p.out_cnt >= 1 && p.out_off + p.out_cnt <= N, MONA_ERR_INTERNAL,
                "bad output window");
     MONA_CHECK(p.S >= 1 && p.outer >= 1, MONA_ERR_INTERNAL, "empty FFT pass");
-    const int TILE = p.mode == 0 ? plain_tile() : kFusedTile;
+    const int TILE = p.mode != 0 ? kFusedTile
+                     : (p.tile == 2048 || p.tile == 4096 || p.tile == 8192) ? p.tile : plain_tile();
     PassArgs a{};
     a.N = N;
     int n = N, ns = 0;
@@ -478,6 +480,7 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     a.epi_scale = p.epi_scale;
     a.img = p.img;
     a.kfc = p.kfc;
+    a.kfc_period = p.kfc_period > 0 ? p.kfc_period : INT64_MAX;
     a.img_out = p.img_out;
     int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     int64_t nblocks = a.ntile_w * ntile_o;

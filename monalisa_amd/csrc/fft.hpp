@@ -31,6 +31,8 @@ struct AxisPass {
     int mode = 0;
     const float2* img = nullptr;
     const float2* kfc = nullptr;
+    int64_t kfc_period = 0;  // > 0: kfc pixel index taken modulo this (batched frames)
+    int tile = 0;            // plain-pass tile (2048 / 4096 / 8192; 0 = default)
     float2* img_out = nullptr;
 };
 

@@ -46,6 +46,8 @@ void timing_flush_locked() {
 
 TimedRegion::TimedRegion(int kind, hipStream_t s) : kind_(kind), s_(s) {
     if (!g_timing) return;
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone) return;
     if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) return;
     (void)hipEventRecord(a_, s_);
     on_ = true;
@@ -490,7 +492,8 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
     {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
-            forward_fft_pruned(x, gr, s);
+            ensure_fft_scratch();
+            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s);
         } else {
             Geom g = geom();
             prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
@@ -506,29 +509,35 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
 
 // Scratch for the pruned passes: the largest intermediate is
 // prod(N_d, d <= a) * prod(n_d, d > a) * nCh over the axes a < imDim - 1.
-void Plan::ensure_fft_scratch() {
+int64_t Plan::fft_scratch_elems() const {
     int64_t mx = 1;
     for (int a = 0; a + 1 < imDim; a++) {
         int64_t v = nCh;
         for (int d = 0; d < imDim; d++) v *= (d <= a) ? N[d] : n[d];
         mx = std::max(mx, v);
     }
+    return mx;
+}
+
+void Plan::ensure_fft_scratch() {
+    int64_t mx = fft_scratch_elems();
     fft_a.ensure((size_t)mx * sizeof(float2));
     fft_b.ensure((size_t)mx * sizeof(float2));
 }
 
 // Forward: x .* KFC zero-filled to N_u and transformed, one pass per axis;
-// pass a reads the compact [prod(n_d, d>a)][n_a][S_a] array, S_a =
-// nCh*prod(N_d, d<a), and writes [prod(n_d, d>a)][N_a][S_a].
-void Plan::forward_fft_pruned(const float2* x, float2* gr, hipStream_t s) {
-    ensure_fft_scratch();
+// pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
+// nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
+void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s) {
     const float2* src = nullptr;
-    float2* bufs[2] = {fft_a.as<float2>(), fft_b.as<float2>()};
+    float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     int64_t S = nCh;
     for (int a = 0; a < imDim; a++) {
-        int64_t outer = 1;
+        int64_t outer = nb;
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         AxisPass p;
+        p.tile = fft_tile;
+        p.kfc_period = nu;
         p.N = N[a];
         p.S = S;
         p.outer = outer;
@@ -555,15 +564,17 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, hipStream_t s) {
 // Adjoint: the grid transformed back one axis at a time from the last,
 // each pass keeping only the central n_a rows; the pass along x ends with
 // .* KFC_conj (and the coil sum when combine).
-void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, hipStream_t s) {
-    ensure_fft_scratch();
+void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, int nb, DevBuf& ba,
+                              DevBuf& bb, hipStream_t s) {
     const float2* src = gr;
-    float2* bufs[2] = {fft_a.as<float2>(), fft_b.as<float2>()};
+    float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     for (int a = imDim - 1; a >= 0; a--) {
-        int64_t S = nCh, outer = 1;
+        int64_t S = nCh, outer = nb;
         for (int d = 0; d < a; d++) S *= N[d];
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         AxisPass p;
+        p.tile = fft_tile;
+        p.kfc_period = nu;
         p.N = N[a];
         p.S = S;
         p.outer = outer;
@@ -604,7 +615,8 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
-            adjoint_fft_pruned(gr, x, combine, scale, s);
+            ensure_fft_scratch();
+            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s);
         } else {
             cfft_nd(gr, imDim, N, nCh, true, s);
             epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
@@ -638,6 +650,93 @@ void Plan::adjoint_public(int f, const float2* y, float2* x, bool combine, hipSt
     work.kpub.ensure((size_t)fr.P * nCh * sizeof(float2));
     to_tile_order(f, y, work.kpub.as<float2>(), s);
     adjoint(f, work.kpub.as<float2>(), x, combine, nullptr, 1.0f, s);
+}
+
+// ---------------------------------------------------------------- batched form
+namespace {
+template <class T>
+std::vector<T> d2h(const DevBuf& b, int64_t n, hipStream_t s) {
+    std::vector<T> h(n);
+    if (n > 0) MONA_HIP(hipMemcpyAsync(h.data(), b.p, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return h;
+}
+}  // namespace
+
+// Block-diagonal stacking of every frame's tile-order matrices: frame f's
+// grid occupies columns [f*Nu, (f+1)*Nu) of Gu_b and rows of Gut_b, its
+// points follow those of frames 0..f-1.
+void Plan::build_batch(hipStream_t s) {
+    const int nFr = (int)frames.size();
+    MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
+    MONA_CHECK((int64_t)nFr * Nu < 0x7fffffff, MONA_ERR_ARG, "nFrames * prod(N_u) too large for one launch");
+    std::vector<int> ucol, unj, tcol, tnj, trow;
+    std::vector<float> uval, tval;
+    koff_b.assign(nFr + 1, 0);
+    int64_t poff = 0;
+    for (int f = 0; f < nFr; f++) {
+        const Frame& fr = frame(f);
+        const SparseDev& U = *fr.Gu_s;
+        const SparseDev& T = *fr.Gut_s;
+        auto c = d2h<int>(U.col, U.nnz, s);
+        auto v = d2h<float>(U.val, U.nnz, s);
+        auto rp = d2h<int64_t>(U.rowptr, U.l_nJump + 1, s);
+        auto tc = d2h<int>(T.col, T.nnz, s);
+        auto tv = d2h<float>(T.val, T.nnz, s);
+        auto trp = d2h<int64_t>(T.rowptr, T.l_nJump + 1, s);
+        std::vector<int> tr = T.squeezed ? d2h<int>(T.rowidx, T.l_nJump, s) : std::vector<int>();
+        MONA_HIP(hipStreamSynchronize(s));
+        for (int64_t k = 0; k < U.nnz; k++) ucol.push_back(c[k] + (int)(f * Nu));
+        uval.insert(uval.end(), v.begin(), v.end());
+        for (int i = 0; i < U.l_nJump; i++) unj.push_back((int)(rp[i + 1] - rp[i]));
+        for (int64_t k = 0; k < T.nnz; k++) tcol.push_back(tc[k] + (int)poff);
+        tval.insert(tval.end(), tv.begin(), tv.end());
+        for (int i = 0; i < T.l_nJump; i++) {
+            tnj.push_back((int)(trp[i + 1] - trp[i]));
+            trow.push_back((T.squeezed ? tr[i] : i) + (int)(f * Nu));
+        }
+        poff += fr.P;
+        koff_b[f + 1] = koff_b[f] + fr.P * nCh;
+    }
+    auto gu = std::make_shared<SparseDev>();
+    sparse_from_decoded(*gu, (int)(nFr * Nu), (int)poff, ucol, uval, unj, nullptr, s);
+    auto gut = std::make_shared<SparseDev>();
+    sparse_from_decoded(*gut, (int)poff, (int)(nFr * Nu), tcol, tval, tnj, &trow, s);
+    Gu_b = gu;
+    Gut_b = gut;
+}
+
+void Plan::forward_batch(const float2* x, float2* y, hipStream_t s) {
+    MONA_CHECK(coils_set && Gu_b, MONA_ERR_STATE, "batched operators not built");
+    const int nFr = (int)frames.size();
+    MONA_CHECK(fft_pass_fusable(N[0], nCh), MONA_ERR_ARG, "batched form needs N_x * nCh <= 8192");
+    grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
+    fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
+    fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
+    {
+        TimedRegion t(2, s);
+        forward_fft_pruned(x, grid_b.as<float2>(), nFr, fft_ab, fft_bb, s);
+    }
+    {
+        TimedRegion t(0, s);
+        spmv_c2_rowmajor(*Gu_b, grid_b.as<float2>(), y, nCh, nullptr, s);
+    }
+}
+
+void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s) {
+    MONA_CHECK(coils_set && Gut_b, MONA_ERR_STATE, "batched operators not built");
+    const int nFr = (int)frames.size();
+    MONA_CHECK(fft_pass_fusable(N[0], nCh), MONA_ERR_ARG, "batched form needs N_x * nCh <= 8192");
+    grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
+    fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
+    fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
+    {
+        TimedRegion t(1, s);
+        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s);
+    }
+    {
+        TimedRegion t(2, s);
+        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nFr, fft_ab, fft_bb, s);
+    }
 }
 
 // ---------------------------------------------------------------- vector helpers
@@ -846,6 +945,59 @@ int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, 
         MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
         P->P.sensa(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nIter, nCGD,
                    residual, stream);
+    });
+}
+
+int mona_plan_set_option(mona_plan P, int option, long long value) {
+    return guarded([&] {
+        MONA_CHECK(P, MONA_ERR_ARG, "NULL plan");
+        switch (option) {
+            case MONA_OPT_BATCH_FRAMES: P->P.use_batch = value != 0; break;
+            case MONA_OPT_GRAPH: P->P.use_graph = value != 0; break;
+            case MONA_OPT_FFT_TILE:
+                MONA_CHECK(value == 0 || value == 2048 || value == 4096 || value == 8192, MONA_ERR_ARG,
+                           "FFT tile must be 0, 2048, 4096 or 8192");
+                P->P.fft_tile = (int)value;
+                break;
+            default: MONA_CHECK(false, MONA_ERR_ARG, "unknown option");
+        }
+    });
+}
+
+// Times one forward + adjoint of frame `frame` for every plain FFT tile and
+// keeps the fastest (two timed repetitions after one warm-up each).
+int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* chosen) {
+    return guarded([&] {
+        MONA_CHECK(P, MONA_ERR_ARG, "NULL plan");
+        Plan& p = P->P;
+        const Frame& fr = p.frame(frame);
+        DevBuf xi((size_t)p.nu * sizeof(float2)), xo((size_t)p.nu * sizeof(float2)),
+            yk((size_t)fr.P * p.nCh * sizeof(float2));
+        fill_f32(xi.as<float>(), 1.0f, p.nu * 2, stream);
+        hipEvent_t a, b;
+        MONA_HIP(hipEventCreate(&a));
+        MONA_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        int best_tile = 4096;
+        for (int tile : {2048, 4096, 8192}) {
+            p.fft_tile = tile;
+            p.forward(frame, xi.as<float2>(), yk.as<float2>(), stream);
+            p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, nullptr, 1.0f, stream);
+            MONA_HIP(hipEventRecord(a, stream));
+            for (int r = 0; r < 2; r++) {
+                p.forward(frame, xi.as<float2>(), yk.as<float2>(), stream);
+                p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, nullptr, 1.0f, stream);
+            }
+            MONA_HIP(hipEventRecord(b, stream));
+            MONA_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            MONA_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) { best = ms; best_tile = tile; }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        p.fft_tile = best_tile;
+        if (chosen) *chosen = best_tile;
     });
 }
 

@@ -76,9 +76,25 @@ struct Plan {
     void launch_check();
     float kmax() const { return (float)kparam[ktype_ == MONA_KERNEL_GAUSS ? 1 : 2]; }
 
+    // FFT stages of the operators over nb frames stored back to back
+    // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
+    void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s);
+    void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, int nb, DevBuf& a, DevBuf& b,
+                            hipStream_t s);
+    int64_t fft_scratch_elems() const;
     void ensure_fft_scratch();
-    void forward_fft_pruned(const float2* x, float2* grid, hipStream_t s);
-    void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, hipStream_t s);
+
+    // Batched form over all frames: one block-diagonal gridding launch and
+    // frame-batched FFT passes per operator application.
+    std::shared_ptr<SparseDev> Gu_b, Gut_b;
+    std::vector<int64_t> koff_b;       // k-space offset (float2) of each frame
+    DevBuf grid_b, fft_ab, fft_bb;
+    bool use_batch = false;            // teva_chain uses the batched operators
+    bool use_graph = false;            // ADMM iterations replayed from a HIP graph
+    int fft_tile = 0;                  // plain FFT pass tile (0 = default)
+    void build_batch(hipStream_t s);
+    void forward_batch(const float2* x, float2* y, hipStream_t s);
+    void adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s);
 
     void set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s);
 

@@ -67,6 +67,26 @@ class Plan:
     def nPt(self, f=0):
         return self._frames[f][0].l_size
 
+    # -- execution options (no reference counterpart) ----------------------
+    OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE = 1, 2, 3
+
+    def set_options(self, batch_frames=None, graph=None, fft_tile=None):
+        """batch_frames: the temporal-TV solver applies its operators to all
+        frames in one launch per gridding op; graph: its ADMM iterations are
+        replayed from a HIP graph (constant delta/rho); fft_tile: plain FFT
+        pass tile (2048, 4096, 8192; 0 = default)."""
+        for key, v in ((self.OPT_BATCH_FRAMES, batch_frames), (self.OPT_GRAPH, graph),
+                       (self.OPT_FFT_TILE, fft_tile)):
+            if v is not None:
+                check(lib.mona_plan_set_option(self._h, key, int(v)))
+
+    def autotune(self, f=0, stream=None):
+        """Times forward + adjoint of frame f for each FFT tile, keeps the
+        fastest and returns it."""
+        chosen = C.c_longlong()
+        check(lib.mona_plan_autotune(self._h, int(f), stream_of(stream), C.byref(chosen)))
+        return chosen.value
+
     def K(self):
         import torch
         out = torch.empty(self.nu, dtype=torch.float32, device="cuda")

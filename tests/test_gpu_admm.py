@@ -43,6 +43,45 @@ def test_steva_vs_oracle(kind):
     assert np.allclose(hist, np.array(hr), rtol=2e-3)
 
 
+def _teva_problem(nFr=3):
+    import torch
+    from monalisa_amd import synth
+    from monalisa_amd.nufft import Plan
+    from monalisa_amd.sparse import bmTraj2SparseMat
+    rng = np.random.default_rng(3)
+    n_u, N_u, dK_u, nCh = [16, 16], [32, 32], [0.5, 0.5], 2
+    C = synth.coil_maps(n_u, nCh, rng)
+    P = Plan(N_u, n_u, dK_u, nCh, "kaiser")
+    P.set_coils(torch.from_numpy(C).cuda())
+    ves, Rs = [], []
+    for f in range(nFr):
+        t = synth.radial2d_golden(12 + 4 * f, 32, k_max=8.0)
+        t = np.roll(t, f, axis=1)
+        ve = synth.ve_radial2(t, 32, 12 + 4 * f)
+        G = bmTraj2SparseMat(t, ve, N_u, dK_u, "kaiser")
+        P.set_frame(f, G["Gu"], G["Gut"])
+        ves.append(ve.astype(np.float32))
+        Rs.append((t, ve))
+    xs = [synth.ellipsoid_phantom(n_u, np.random.default_rng(10 + f)) for f in range(nFr)]
+    ys = []
+    for f in range(nFr):
+        y = P.bmShanna(torch.from_numpy(xs[f]).cuda(), f=f).cpu().numpy()
+        ys.append((y + 0.01 * (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape))).astype(np.complex64))
+    return dict(P=P, C=C, ys=ys, ves=ves, Rs=Rs, n_u=n_u, N_u=N_u, dK_u=dK_u, nCh=nCh, nFr=nFr)
+
+
+def _teva_run(q, nIter=3, delta=0.05, rho=0.5, nCGD=3):
+    import torch
+    P = q["P"]
+    ve_max = float(max(v.max() for v in q["ves"]))
+    x = torch.zeros((q["nFr"], P.nu), dtype=torch.complex64, device="cuda")
+    z, u, hist = P.bmTevaMorphosia_chain(x, None, None, [torch.from_numpy(v).cuda() for v in q["ys"]],
+                                         [torch.from_numpy(v).cuda() for v in q["ves"]], delta, rho, nCGD,
+                                         ve_max, nIter=nIter)
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), z.cpu().numpy(), u.cpu().numpy(), hist
+
+
 def test_teva_chain_vs_oracle():
     import torch
     from monalisa_amd import synth
@@ -83,6 +122,65 @@ def test_teva_chain_vs_oracle():
     ref = np.stack([flat(v, n_u) for v in xr])
     assert rel(got, ref) < 1e-4
     assert np.allclose(hist, np.array(hr), rtol=2e-3)
+
+
+def test_teva_batched_equals_per_frame():
+    """All frames in one launch per gridding op (block-diagonal matrices,
+    frame-batched FFT passes) give the per-frame solver's results bit for bit:
+    every row keeps its entries and order, every FFT line its arithmetic."""
+    print("[gpu] teva batched vs per-frame", flush=True)
+    q = _teva_problem()
+    ref = _teva_run(q)
+    q["P"].set_options(batch_frames=True)
+    got = _teva_run(q)
+    for a, b in zip(got[:3], ref[:3]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_teva_graph_replays_equal_eager(batch):
+    """ADMM iterations replayed from a HIP graph equal the eager loop; the
+    graph run is repeated with eager operator calls and synchronisations in
+    between, and every replay (one history row each) is checked."""
+    import torch
+    print(f"[gpu] teva graph batch={batch}", flush=True)
+    q = _teva_problem()
+    P = q["P"]
+    P.set_options(batch_frames=batch)
+    ref = _teva_run(q, nIter=4)
+    P.set_options(graph=True)
+    for rep in range(3):
+        got = _teva_run(q, nIter=4)
+        for a, b in zip(got[:3], ref[:3]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(got[3], ref[3])
+        # eager work on the same plan between graph runs
+        xe = torch.ones(P.nu, dtype=torch.complex64, device="cuda")
+        y0 = P.bmShanna(xe, f=rep % q["nFr"])
+        torch.cuda.synchronize()
+        assert torch.isfinite(torch.view_as_real(y0)).all()
+    P.set_options(graph=False, batch_frames=False)
+    again = _teva_run(q, nIter=4)
+    np.testing.assert_array_equal(again[0], ref[0])
+
+
+def test_autotune_keeps_results():
+    import torch
+    print("[gpu] autotune", flush=True)
+    q = _teva_problem(nFr=1)
+    P = q["P"]
+    x = torch.from_numpy(synth_x(P.nu)).cuda()
+    y_ref = P.bmShanna(x).cpu().numpy()
+    tile = P.autotune()
+    assert tile in (2048, 4096, 8192)
+    y = P.bmShanna(x).cpu().numpy()
+    np.testing.assert_allclose(y, y_ref, rtol=0, atol=1e-5 * np.abs(y_ref).max())
+
+
+def synth_x(nu, seed=5):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(size=nu) + 1j * rng.normal(size=nu)).astype(np.complex64)
 
 
 @pytest.mark.parametrize("with_coils", [True, False])
