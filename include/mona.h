@@ -271,9 +271,12 @@ int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* 
 #define MONA_OPT_GRAPH 2        /* temporal-TV solver: ADMM iterations replayed from a HIP graph
                                    (constant delta/rho schedules) */
 #define MONA_OPT_FFT_TILE 3     /* plain FFT pass tile: 2048, 4096 or 8192 complex values, 0 = default */
+#define MONA_OPT_FFT_LINES 4    /* register-resident FFT passes (N = 16 * N2, N2 <= 16): lines per tile
+                                   on strided axes 16, 32 or 64, 0 = default, -1 = LDS kernel only */
 int mona_plan_set_option(mona_plan P, int option, long long value);
-/* Autotuner: times forward + adjoint of one frame for each FFT tile and keeps
- * the fastest; *chosen (may be NULL) receives it. */
+/* Autotuner: times forward + adjoint of one frame for each FFT pass variant
+ * and keeps the fastest; *chosen (may be NULL) receives the register kernel's
+ * lines per tile, or minus the LDS kernel's tile. */
 int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* chosen);
 
 /* Kernel timing: when enabled, every gridding launch (forward Gu product,

@@ -53,6 +53,7 @@ struct PassArgs {
     const float2* kfc;   // prologue: KFC [outer*in_cnt][S]; epilogue: KFC_conj [outer*out_cnt][S]
     int64_t kfc_period;  // KFC index taken modulo this many pixels (frames share the maps)
     float2* img_out;     // epilogue (combine): [outer][out_cnt]
+    int xs;              // register kernel: LDS stride between the N1 exchange planes
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -168,6 +169,68 @@ template <> struct Dft<16> {
         }
     }
 };
+
+// Composite register DFTs R = A * B (Cooley-Tukey, n = B n1 + n2,
+// k = k1 + A k2) with twiddles exp(sg 2 pi i m / R) folded to constants.
+constexpr double kPi = 3.14159265358979323846;
+constexpr double ce_sin(double x) {  // |x| <= pi, Taylor series to x^27
+    double term = x, sum = x;
+    for (int k = 1; k < 14; k++) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ce_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 14; k++) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+template <int R> struct TwConst {
+    float c[R], s[R];
+    constexpr TwConst() : c(), s() {
+        for (int m = 0; m < R; m++) {
+            double x = 2.0 * kPi * m / R;
+            if (x > kPi) x -= 2.0 * kPi;
+            c[m] = (float)ce_cos(x);
+            s[m] = (float)ce_sin(x);
+        }
+    }
+};
+
+template <int A, int B> struct DftCT {
+    __device__ static void run(float2* a, float sg) {
+        constexpr TwConst<A * B> W{};
+        float2 y[B][A];
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+            float2 t[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) t[n1] = a[B * n1 + n2];
+            Dft<A>::run(t, sg);
+#pragma unroll
+            for (int k1 = 0; k1 < A; k1++) {
+                const int m = n2 * k1;
+                y[n2][k1] = (m == 0) ? t[k1] : cmul(t[k1], make_float2(W.c[m], sg * W.s[m]));
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            float2 t[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; n2++) t[n2] = y[n2][k1];
+            Dft<B>::run(t, sg);
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) a[k1 + A * k2] = t[k2];
+        }
+    }
+};
+template <> struct Dft<6> : DftCT<2, 3> {};
+template <> struct Dft<10> : DftCT<2, 5> {};
+template <> struct Dft<12> : DftCT<4, 3> {};
 
 // One Stockham stage of radix R over L lines of length N held transposed
 // in LDS ([n][line]).  j in [0, N/R): inputs at rows j + r*N/R, twiddle
@@ -365,6 +428,131 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
     }
 }
 
+// Register-resident pass for N = 16 * N2 (N2 <= 16): the four-step split
+// n = n2 + N2 n1, k = k1 + 16 k2.  A tile holds L = Bo * W lines, 16
+// work-items per line.  Work-item (line, j): step 1 (j < N2) loads rows
+// j + N2 n1 of its line straight into registers (rows outside the input
+// window are zero), runs the 16-point DFT over n1 and multiplies by
+// w_N^(j k1); one LDS exchange ([k1][n2][line], planes padded so that the
+// reads of a wave spread over all banks); step 2 runs the N2-point DFT over
+// n2 for k1 = j and stores rows j + 16 k2 straight from registers.  A wave's
+// loads and stores cover runs of consecutive lines (inner positions) in
+// consecutive rows.  MODE as for the LDS kernel; the coil sum of MODE 2 goes
+// through a second LDS pass and adds the coils in order.
+template <int N2, int MODE>
+__global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
+    constexpr int N1 = 16;
+    constexpr int N = N1 * N2;
+    extern __shared__ float2 smem[];
+    float2* tw = smem;      // N twiddles exp(-2 pi i t / N)
+    float2* ex = smem + N;  // exchange planes
+    const int W = a.W, L = a.W * a.Bo, XS = a.xs;
+    const int t = threadIdx.x;
+    const int line = t % L, j = t / L;
+    const int ob = line / W, w = line - ob * W;
+    const int64_t tile_w = blockIdx.x % a.ntile_w, to = blockIdx.x / a.ntile_w;
+    const int64_t o = to * a.Bo + ob;
+    const int64_t ws = tile_w * W + w;
+    const bool live = o < a.outer && ws < a.S;
+    const float sg = a.inverse ? 1.0f : -1.0f;
+
+    float2 v[N1];
+    if (j < N2) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const int r = j + N2 * n1 - a.in_off;
+            float2 x = make_float2(0.f, 0.f);
+            if (live && r >= 0 && r < a.in_cnt) {
+                if (MODE == 1) {
+                    const int64_t ii = o * a.in_cnt + r;
+                    x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
+                } else {
+                    x = a.in[o * a.in_os + (int64_t)r * a.S + ws];
+                }
+            }
+            v[n1] = x;
+        }
+    }
+    for (int i = t; i < N; i += blockDim.x) tw[i] = twg[i];
+    __syncthreads();
+    if (j < N2) {
+        if (a.presign) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++)
+                if ((j + N2 * n1) & 1) v[n1] = make_float2(-v[n1].x, -v[n1].y);
+        }
+        Dft<N1>::run(v, sg);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; k1++) {
+            float2 wv = tw[j * k1];
+            wv.y *= -sg;  // table holds exp(-2 pi i t / N); conj for the inverse
+            v[k1] = cmul(v[k1], wv);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) ex[k1 * XS + j * L + line] = v[k1];
+    }
+    __syncthreads();
+    float2 u[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[j * XS + n2 * L + line];
+    Dft<N2>::run(u, sg);
+    const int half = N / 2;
+    if (MODE == 2) {
+        // products with KFC_conj into LDS as [ob][kept row][coil] (odd
+        // stride), then one work-item per (ob, kept row) adds the coils
+        const int Wp = W | 1;
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) {
+                const int k = j + N1 * k2;
+                const int kl = k - a.out_off;
+                if (kl >= 0 && kl < a.out_cnt) {
+                    float sc = a.scale;
+                    if (a.postsign && ((k + half) & 1)) sc = -sc;
+                    const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
+                    const int64_t i = o * a.out_cnt + kl;
+                    ex[(ob * a.out_cnt + kl) * Wp + w] = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
+                }
+            }
+        }
+        __syncthreads();
+        const int nred = a.Bo * a.out_cnt;
+        for (int e = t; e < nred; e += blockDim.x) {
+            const int obb = e / a.out_cnt, kl = e - obb * a.out_cnt;
+            const int64_t oo = to * a.Bo + obb;
+            if (oo < a.outer) {
+                float2 acc = make_float2(0.f, 0.f);
+                for (int c = 0; c < W; c++) {
+                    const float2 p = ex[(obb * a.out_cnt + kl) * Wp + c];
+                    acc.x += p.x;
+                    acc.y += p.y;
+                }
+                a.img_out[oo * a.out_cnt + kl] = make_float2(acc.x * a.epi_scale, acc.y * a.epi_scale);
+            }
+        }
+        return;
+    }
+    if (!live) return;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+        const int k = j + N1 * k2;
+        const int kl = k - a.out_off;
+        if (kl >= 0 && kl < a.out_cnt) {
+            float sc = a.scale;
+            if (a.postsign && ((k + half) & 1)) sc = -sc;
+            const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
+            if (MODE == 3) {
+                const int64_t i = o * a.out_cnt + kl;
+                const float2 p = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
+                a.out[i * a.S + ws] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
+            } else {
+                a.out[o * a.out_os + (int64_t)kl * a.S + ws] = y;
+            }
+        }
+    }
+}
+
 struct TwiddleCache {
     std::mutex mu;
     std::map<int, std::unique_ptr<DevBuf>> tables;
@@ -415,6 +603,50 @@ void launch(const PassArgs& a, int mode, unsigned nblocks, size_t lds, const flo
     }
 }
 
+// Register kernel: lines per tile on strided axes (16, 32 or 64;
+// MONA_FFT_LINES, default 16); -1 disables the register kernel.
+int reg_lines_default() {
+    static int t = [] {
+        const char* e = std::getenv("MONA_FFT_LINES");
+        int v = e ? std::atoi(e) : 16;
+        return (v == -1 || v == 32 || v == 64) ? v : 16;
+    }();
+    return t;
+}
+
+// N = 16 * N2 with a register DFT of length N2.
+int reg_n2(int N) {
+    if (N % 16) return 0;
+    const int n2 = N / 16;
+    switch (n2) {
+        case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: case 16: return n2;
+        default: return 0;
+    }
+}
+
+template <int N2, int MODE>
+void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+    auto kern = fft_reg_kernel<N2, MODE>;
+    if (lds > 65536) {
+        static std::once_flag once;
+        std::call_once(once, [&] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        });
+    }
+    kern<<<nblocks, nt, lds, s>>>(a, tw);
+}
+
+template <int N2>
+void launch_reg(const PassArgs& a, int mode, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+    switch (mode) {
+        case 1: launch_reg_mode<N2, 1>(a, nblocks, nt, lds, tw, s); break;
+        case 2: launch_reg_mode<N2, 2>(a, nblocks, nt, lds, tw, s); break;
+        case 3: launch_reg_mode<N2, 3>(a, nblocks, nt, lds, tw, s); break;
+        default: launch_reg_mode<N2, 0>(a, nblocks, nt, lds, tw, s); break;
+    }
+}
+
 }  // namespace
 
 bool fft_size_supported(int N) {
@@ -425,7 +657,73 @@ bool fft_size_supported(int N) {
     return n == 1;
 }
 
-bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= kFusedTile; }
+bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= kFusedTile || (reg_n2(N) && S <= 64); }
+
+namespace {
+void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
+    const int N = p.N;
+    PassArgs a{};
+    a.N = N;
+    a.S = p.S;
+    a.outer = p.outer;
+    if (p.S <= 64) {
+        // whole inner dimension in a tile, >= 16 lines where the outer
+        // dimension allows
+        a.W = (int)p.S;
+        int bo = (int)std::max<int64_t>(1, (16 + p.S - 1) / p.S);
+        while (bo > 1 && bo * p.S > 64) bo--;
+        a.Bo = (int)std::min<int64_t>(bo, p.outer);
+    } else {
+        a.W = lines;
+        a.Bo = 1;
+    }
+    const int L = a.W * a.Bo;
+    // exchange plane stride: n2 * L rounded so that consecutive planes
+    // start 128 bytes apart modulo the 256-byte bank row
+    int xs = n2 * L;
+    xs += ((16 - xs % 32) + 32) % 32;
+    a.xs = xs;
+    a.ntile_w = (p.S + a.W - 1) / a.W;
+    a.in = p.in;
+    a.in_os = p.in_os;
+    a.in_off = p.in_off;
+    a.in_cnt = p.in_cnt;
+    a.out = p.out;
+    a.out_os = p.out_os;
+    a.out_off = p.out_off;
+    a.out_cnt = p.out_cnt;
+    a.inverse = p.inverse;
+    a.presign = p.presign;
+    a.postsign = p.postsign;
+    a.scale = p.scale;
+    a.epi_scale = p.epi_scale;
+    a.img = p.img;
+    a.kfc = p.kfc;
+    a.kfc_period = p.kfc_period > 0 ? p.kfc_period : INT64_MAX;
+    a.img_out = p.img_out;
+    const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
+    const int64_t nblocks = a.ntile_w * ntile_o;
+    MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
+    const int nt = 16 * L;
+    MONA_CHECK(nt <= 1024 && (p.mode != 2 || a.W == p.S), MONA_ERR_INTERNAL, "bad register FFT tile");
+    size_t ex = std::max<size_t>((size_t)16 * xs, (size_t)a.Bo * p.out_cnt * (a.W | 1));
+    size_t lds = ((size_t)N + ex) * sizeof(float2);
+    MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
+    const float2* tw = twiddles().get(N);
+    switch (n2) {
+        case 2: launch_reg<2>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 3: launch_reg<3>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 4: launch_reg<4>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 5: launch_reg<5>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 6: launch_reg<6>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 8: launch_reg<8>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 10: launch_reg<10>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 12: launch_reg<12>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        default: launch_reg<16>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+    }
+    MONA_HIP(hipGetLastError());
+}
+}  // namespace
 
 void fft_pass(const AxisPass& p, hipStream_t s) {
     const int N = p.N;
@@ -439,6 +737,14 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     MONA_CHECK(p.S >= 1 && p.outer >= 1, MONA_ERR_INTERNAL, "empty FFT pass");
     const int TILE = p.mode != 0 ? kFusedTile
                      : (p.tile == 2048 || p.tile == 4096 || p.tile == 8192) ? p.tile : plain_tile();
+    const int lines = p.lines != 0 ? p.lines : reg_lines_default();
+    const int n2 = reg_n2(N);
+    const bool old_fits = (int64_t)N * p.S <= kFusedTile;
+    if (n2 && ((p.mode == 0 && lines > 0) || (p.mode != 0 && p.S <= 64 && (lines > 0 || !old_fits)))) {
+        fft_pass_reg(p, n2, lines > 0 ? lines : 16, s);
+        return;
+    }
+    MONA_CHECK(p.mode == 0 || old_fits, MONA_ERR_INTERNAL, "fused FFT pass does not fit a tile");
     PassArgs a{};
     a.N = N;
     int n = N, ns = 0;

@@ -33,6 +33,8 @@ struct AxisPass {
     const float2* kfc = nullptr;
     int64_t kfc_period = 0;  // > 0: kfc pixel index taken modulo this (batched frames)
     int tile = 0;            // plain-pass tile (2048 / 4096 / 8192; 0 = default)
+    int lines = 0;           // register kernel (N = 16 * N2): lines per tile on strided axes
+                             // (16 / 32 / 64), 0 = default, -1 = LDS Stockham kernel only
     float2* img_out = nullptr;
 };
 

@@ -537,6 +537,7 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         AxisPass p;
         p.tile = fft_tile;
+        p.lines = fft_lines;
         p.kfc_period = nu;
         p.N = N[a];
         p.S = S;
@@ -574,6 +575,7 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         AxisPass p;
         p.tile = fft_tile;
+        p.lines = fft_lines;
         p.kfc_period = nu;
         p.N = N[a];
         p.S = S;
@@ -959,13 +961,21 @@ int mona_plan_set_option(mona_plan P, int option, long long value) {
                            "FFT tile must be 0, 2048, 4096 or 8192");
                 P->P.fft_tile = (int)value;
                 break;
+            case MONA_OPT_FFT_LINES:
+                MONA_CHECK(value == -1 || value == 0 || value == 16 || value == 32 || value == 64, MONA_ERR_ARG,
+                           "FFT lines per tile must be -1, 0, 16, 32 or 64");
+                P->P.fft_lines = (int)value;
+                break;
             default: MONA_CHECK(false, MONA_ERR_ARG, "unknown option");
         }
     });
 }
 
-// Times one forward + adjoint of frame `frame` for every plain FFT tile and
-// keeps the fastest (two timed repetitions after one warm-up each).
+// Times one forward + adjoint of frame `frame` for each FFT pass variant
+// (register kernel with 16 / 32 / 64 lines per tile; LDS Stockham kernel
+// with tiles 2048 / 4096 / 8192) and keeps the fastest; two timed
+// repetitions after one warm-up each.  *chosen = lines per tile of the
+// winner, or minus its Stockham tile.
 int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* chosen) {
     return guarded([&] {
         MONA_CHECK(P, MONA_ERR_ARG, "NULL plan");
@@ -977,27 +987,37 @@ int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* ch
         hipEvent_t a, b;
         MONA_HIP(hipEventCreate(&a));
         MONA_HIP(hipEventCreate(&b));
+        struct Variant { int tile, lines; };
+        const Variant vars[] = {{4096, 16}, {4096, 32}, {4096, 64}, {2048, -1}, {4096, -1}, {8192, -1}};
         float best = 1e30f;
-        int best_tile = 4096;
-        for (int tile : {2048, 4096, 8192}) {
-            p.fft_tile = tile;
-            p.forward(frame, xi.as<float2>(), yk.as<float2>(), stream);
-            p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, nullptr, 1.0f, stream);
-            MONA_HIP(hipEventRecord(a, stream));
-            for (int r = 0; r < 2; r++) {
+        Variant win = {4096, 16};
+        try {
+            for (const Variant& v : vars) {
+                p.fft_tile = v.tile;
+                p.fft_lines = v.lines;
                 p.forward(frame, xi.as<float2>(), yk.as<float2>(), stream);
                 p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, nullptr, 1.0f, stream);
+                MONA_HIP(hipEventRecord(a, stream));
+                for (int r = 0; r < 2; r++) {
+                    p.forward(frame, xi.as<float2>(), yk.as<float2>(), stream);
+                    p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, nullptr, 1.0f, stream);
+                }
+                MONA_HIP(hipEventRecord(b, stream));
+                MONA_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                MONA_HIP(hipEventElapsedTime(&ms, a, b));
+                if (ms < best) { best = ms; win = v; }
             }
-            MONA_HIP(hipEventRecord(b, stream));
-            MONA_HIP(hipEventSynchronize(b));
-            float ms = 0;
-            MONA_HIP(hipEventElapsedTime(&ms, a, b));
-            if (ms < best) { best = ms; best_tile = tile; }
+        } catch (...) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            throw;
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        p.fft_tile = best_tile;
-        if (chosen) *chosen = best_tile;
+        p.fft_tile = win.tile;
+        p.fft_lines = win.lines;
+        if (chosen) *chosen = win.lines > 0 ? win.lines : -(long long)win.tile;
     });
 }
 

@@ -92,6 +92,7 @@ struct Plan {
     bool use_batch = false;            // teva_chain uses the batched operators
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)
+    int fft_lines = 0;                 // register FFT kernel lines per tile (16/32/64; 0 default, -1 off)
     void build_batch(hipStream_t s);
     void forward_batch(const float2* x, float2* y, hipStream_t s);
     void adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s);

@@ -68,21 +68,24 @@ class Plan:
         return self._frames[f][0].l_size
 
     # -- execution options (no reference counterpart) ----------------------
-    OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE = 1, 2, 3
+    OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE, OPT_FFT_LINES = 1, 2, 3, 4
 
-    def set_options(self, batch_frames=None, graph=None, fft_tile=None):
+    def set_options(self, batch_frames=None, graph=None, fft_tile=None, fft_lines=None):
         """batch_frames: the temporal-TV solver applies its operators to all
         frames in one launch per gridding op; graph: its ADMM iterations are
         replayed from a HIP graph (constant delta/rho); fft_tile: plain FFT
-        pass tile (2048, 4096, 8192; 0 = default)."""
+        pass tile of the LDS kernel (2048, 4096, 8192; 0 = default);
+        fft_lines: register-resident passes (N = 16 * N2) with 16 / 32 / 64
+        lines per tile on strided axes, 0 = default, -1 = LDS kernel only."""
         for key, v in ((self.OPT_BATCH_FRAMES, batch_frames), (self.OPT_GRAPH, graph),
-                       (self.OPT_FFT_TILE, fft_tile)):
+                       (self.OPT_FFT_TILE, fft_tile), (self.OPT_FFT_LINES, fft_lines)):
             if v is not None:
                 check(lib.mona_plan_set_option(self._h, key, int(v)))
 
     def autotune(self, f=0, stream=None):
-        """Times forward + adjoint of frame f for each FFT tile, keeps the
-        fastest and returns it."""
+        """Times forward + adjoint of frame f for each FFT pass variant, keeps
+        the fastest and returns the register kernel's lines per tile (or minus
+        the LDS kernel's tile)."""
         chosen = C.c_longlong()
         check(lib.mona_plan_autotune(self._h, int(f), stream_of(stream), C.byref(chosen)))
         return chosen.value

@@ -173,7 +173,7 @@ def test_autotune_keeps_results():
     x = torch.from_numpy(synth_x(P.nu)).cuda()
     y_ref = P.bmShanna(x).cpu().numpy()
     tile = P.autotune()
-    assert tile in (2048, 4096, 8192)
+    assert tile in (16, 32, 64, -2048, -4096, -8192)
     y = P.bmShanna(x).cpu().numpy()
     np.testing.assert_allclose(y, y_ref, rtol=0, atol=1e-5 * np.abs(y_ref).max())
 

@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("N,nb", [((8,), 3), ((16, 12), 2), ((8, 6, 10), 2), ((64, 64), 1),
-                                  ((32, 32, 32), 2), ((192,), 5), ((24, 40), 20), ((30, 20), 1)])
+                                  ((32, 32, 32), 2), ((192,), 5), ((24, 40), 20), ((30, 20), 1),
+                                  ((64, 48, 40), 3), ((16, 256), 4), ((18, 160), 3), ((96, 96, 96), 1)])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_cfft_vs_numpy(N, nb, inverse):
     import torch
@@ -51,11 +52,12 @@ def test_deapodisation_vs_oracle(kind, kernel):
     assert rel(KFCc, flat(O.kf_conj(np.conj(Cn), p["N_u"], p["n_u"], p["dK_u"], kernel), p["n_u"])) < 1e-6
 
 
-@pytest.mark.parametrize("kind", ["2d", "3d", "2d_rect"])
-def test_forward_adjoint_vs_oracle(kind):
+@pytest.mark.parametrize("kind,nCh", [("2d", 3), ("3d", 3), ("2d_rect", 3), ("3d32", 3), ("3d32", 20),
+                                      ("2d", 1)])
+def test_forward_adjoint_vs_oracle(kind, nCh):
     import torch
-    p = make_problem(kind)
-    print(f"[gpu] forward/adjoint {kind}", flush=True)
+    p = make_problem(kind, nCh=nCh)
+    print(f"[gpu] forward/adjoint {kind} nCh={nCh}", flush=True)
     P = _plan(p)
     R = oracle_mats(p)
     Cn = nd(p["C"], p["n_u"], p["nCh"])
@@ -103,3 +105,34 @@ def test_operators_deterministic():
     ya = P.bmNakatsha(torch.from_numpy(a).cuda()).cpu().numpy()
     yb = P.bmNakatsha(torch.from_numpy(a).cuda()).cpu().numpy()
     assert np.array_equal(ya.view(np.int32), yb.view(np.int32))
+
+
+@pytest.mark.parametrize("kind", ["3d", "3d32"])
+def test_fft_pass_variants(kind):
+    """FFT pass kernels: the LDS Stockham kernel with tiles 2048 / 4096 /
+    8192 runs the same butterflies in the same order for every tile (bit-
+    identical), so does the register kernel for 16 / 32 / 64 lines per tile;
+    the two kernels agree to float rounding."""
+    import torch
+    print(f"[gpu] fft pass variants {kind}", flush=True)
+    p = make_problem(kind, seed=7, nCh=4)
+    P = _plan(p)
+    x = torch.from_numpy(p["x"]).cuda()
+
+    def run(tile, lines):
+        P.set_options(fft_tile=tile, fft_lines=lines)
+        y = P.bmShanna(x)
+        xa = P.bmNakatsha(y)
+        torch.cuda.synchronize()
+        return y.cpu().numpy(), xa.cpu().numpy()
+
+    lds = [run(t, -1) for t in (4096, 2048, 8192)]
+    reg = [run(4096, l) for l in (16, 32, 64)]
+    for y, xa in lds[1:]:
+        np.testing.assert_array_equal(y, lds[0][0])
+        np.testing.assert_array_equal(xa, lds[0][1])
+    for y, xa in reg[1:]:
+        np.testing.assert_array_equal(y, reg[0][0])
+        np.testing.assert_array_equal(xa, reg[0][1])
+    assert rel(reg[0][0], lds[0][0]) < 1e-6
+    assert rel(reg[0][1], lds[0][1]) < 1e-6
