@@ -604,12 +604,12 @@ void launch(const PassArgs& a, int mode, unsigned nblocks, size_t lds, const flo
 }
 
 // Register kernel: lines per tile on strided axes (16, 32 or 64;
-// MONA_FFT_LINES, default 16); -1 disables the register kernel.
+// MONA_FFT_LINES, default 32); -1 disables the register kernel.
 int reg_lines_default() {
     static int t = [] {
         const char* e = std::getenv("MONA_FFT_LINES");
-        int v = e ? std::atoi(e) : 16;
-        return (v == -1 || v == 32 || v == 64) ? v : 16;
+        int v = e ? std::atoi(e) : 32;
+        return (v == -1 || v == 16 || v == 64) ? v : 32;
     }();
     return t;
 }
