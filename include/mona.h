@@ -264,6 +264,25 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
 int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x,
                hipStream_t stream);
 
+/* Mixed precision (no reference counterpart).  CG-SENSE by iterative
+ * refinement with reliable updates: CG on the normal equations of bmSensa's
+ * system (same HX / HY inner products) whose products use the operators
+ * with the grid-side arrays and k-space vector stored as bf16 (bf16 = 1) or
+ * float (bf16 = 0); every nInner steps the recursive residual is replaced
+ * by the true one computed in float, g = (1/HX) M^H HY (y - M x), and CG
+ * continues.  rel = ||g|| / ||b||, b = (1/HX) M^H HY y (the residual of
+ * x = 0); stops when rel < tol or after nOuter replacements.  x in place
+ * (any start); rel_res receives rel at the start and after every
+ * replacement (nOuter + 1 doubles), *n_outer the replacements done.
+ * bf16 storage needs every N_u = 16 * {2,3,4,5,6,8,10,12,16} and
+ * nCh <= 64. */
+int mona_sensa_ir(mona_plan P, int frame, void* x, const void* y, const float* ve, float ve_max, int nInner,
+                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, hipStream_t stream);
+/* bmShanna / bmNakatsha (C_flag = 1) with bf16 storage of the grid-side
+ * arrays and of the k-space vector; caller-order float k-space in / out. */
+int mona_shanna_bf16(mona_plan P, int frame, const void* x, void* y, hipStream_t stream);
+int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream_t stream);
+
 /* Plan options (no reference counterpart: execution choices of this
  * implementation). */
 #define MONA_OPT_BATCH_FRAMES 1 /* temporal-TV solver: all frames in one launch per gridding op

@@ -439,8 +439,10 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
 // loads and stores cover runs of consecutive lines (inner positions) in
 // consecutive rows.  MODE as for the LDS kernel; the coil sum of MODE 2 goes
 // through a second LDS pass and adds the coils in order.
-template <int N2, int MODE>
+template <int N2, int MODE, class TI, class TO>
 __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
+    const TI* __restrict__ in = reinterpret_cast<const TI*>(a.in);
+    TO* __restrict__ out = reinterpret_cast<TO*>(a.out);
     constexpr int N1 = 16;
     constexpr int N = N1 * N2;
     extern __shared__ float2 smem[];
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
                     const int64_t ii = o * a.in_cnt + r;
                     x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
                 } else {
-                    x = a.in[o * a.in_os + (int64_t)r * a.S + ws];
+                    x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
                 }
             }
             v[n1] = x;
@@ -545,9 +547,9 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             if (MODE == 3) {
                 const int64_t i = o * a.out_cnt + kl;
                 const float2 p = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
-                a.out[i * a.S + ws] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
+                st_c(out + i * a.S + ws, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
             } else {
-                a.out[o * a.out_os + (int64_t)kl * a.S + ws] = y;
+                st_c(out + o * a.out_os + (int64_t)kl * a.S + ws, y);
             }
         }
     }
@@ -624,9 +626,9 @@ int reg_n2(int N) {
     }
 }
 
-template <int N2, int MODE>
+template <int N2, int MODE, class TI, class TO>
 void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
-    auto kern = fft_reg_kernel<N2, MODE>;
+    auto kern = fft_reg_kernel<N2, MODE, TI, TO>;
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
@@ -637,13 +639,34 @@ void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, co
     kern<<<nblocks, nt, lds, s>>>(a, tw);
 }
 
+// Storage variants: float2 throughout, or bf16x2 for the grid-side arrays
+// (MODE 1 writes, MODE 0 reads and writes, MODE 2 reads bf16x2).
 template <int N2>
-void launch_reg(const PassArgs& a, int mode, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+void launch_reg(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsigned nblocks, int nt, size_t lds,
+                const float2* tw, hipStream_t s) {
+    if (bf_in || bf_out) {
+        switch (mode) {
+            case 1:
+                MONA_CHECK(bf_out, MONA_ERR_INTERNAL, "bad bf16 pass");
+                launch_reg_mode<N2, 1, float2, bf16x2>(a, nblocks, nt, lds, tw, s);
+                break;
+            case 2:
+                MONA_CHECK(bf_in, MONA_ERR_INTERNAL, "bad bf16 pass");
+                launch_reg_mode<N2, 2, bf16x2, float2>(a, nblocks, nt, lds, tw, s);
+                break;
+            case 0:
+                MONA_CHECK(bf_in && bf_out, MONA_ERR_INTERNAL, "bad bf16 pass");
+                launch_reg_mode<N2, 0, bf16x2, bf16x2>(a, nblocks, nt, lds, tw, s);
+                break;
+            default: MONA_CHECK(false, MONA_ERR_INTERNAL, "bf16 storage has no per-coil epilogue");
+        }
+        return;
+    }
     switch (mode) {
-        case 1: launch_reg_mode<N2, 1>(a, nblocks, nt, lds, tw, s); break;
-        case 2: launch_reg_mode<N2, 2>(a, nblocks, nt, lds, tw, s); break;
-        case 3: launch_reg_mode<N2, 3>(a, nblocks, nt, lds, tw, s); break;
-        default: launch_reg_mode<N2, 0>(a, nblocks, nt, lds, tw, s); break;
+        case 1: launch_reg_mode<N2, 1, float2, float2>(a, nblocks, nt, lds, tw, s); break;
+        case 2: launch_reg_mode<N2, 2, float2, float2>(a, nblocks, nt, lds, tw, s); break;
+        case 3: launch_reg_mode<N2, 3, float2, float2>(a, nblocks, nt, lds, tw, s); break;
+        default: launch_reg_mode<N2, 0, float2, float2>(a, nblocks, nt, lds, tw, s); break;
     }
 }
 
@@ -656,6 +679,8 @@ bool fft_size_supported(int N) {
         while (n % f == 0) n /= f;
     return n == 1;
 }
+
+bool fft_reg_supported(int N) { return reg_n2(N) != 0; }
 
 bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= kFusedTile || (reg_n2(N) && S <= 64); }
 
@@ -711,15 +736,15 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
     const float2* tw = twiddles().get(N);
     switch (n2) {
-        case 2: launch_reg<2>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 3: launch_reg<3>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 4: launch_reg<4>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 5: launch_reg<5>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 6: launch_reg<6>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 8: launch_reg<8>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 10: launch_reg<10>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 12: launch_reg<12>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
-        default: launch_reg<16>(a, p.mode, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 2: launch_reg<2>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 3: launch_reg<3>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 4: launch_reg<4>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 5: launch_reg<5>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 6: launch_reg<6>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 8: launch_reg<8>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 10: launch_reg<10>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 12: launch_reg<12>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        default: launch_reg<16>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
     }
     MONA_HIP(hipGetLastError());
 }
@@ -740,6 +765,12 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     const int lines = p.lines != 0 ? p.lines : reg_lines_default();
     const int n2 = reg_n2(N);
     const bool old_fits = (int64_t)N * p.S <= kFusedTile;
+    if (p.bf_in || p.bf_out) {
+        MONA_CHECK(n2 && (p.mode == 0 || p.S <= 64), MONA_ERR_ARG,
+                   "bf16 storage needs FFT lengths 16 * {2,3,4,5,6,8,10,12,16} and <= 64 coils");
+        fft_pass_reg(p, n2, lines > 0 ? lines : 32, s);
+        return;
+    }
     if (n2 && ((p.mode == 0 && lines > 0) || (p.mode != 0 && p.S <= 64 && (lines > 0 || !old_fits)))) {
         fft_pass_reg(p, n2, lines > 0 ? lines : 16, s);
         return;

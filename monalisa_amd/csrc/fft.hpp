@@ -6,6 +6,9 @@ namespace mona {
 
 // True when N = 2^a 3^b 5^c <= 2048.
 bool fft_size_supported(int N);
+// True when N = 16 * N2 with N2 in {2,3,4,5,6,8,10,12,16}: passes of this
+// length run register-resident (and support bf16x2 storage).
+bool fft_reg_supported(int N);
 
 // One pass along an axis of length N over an [outer][rows][S] array (S
 // contiguous).  Input rows in_off .. in_off+in_cnt-1 of the length-N line
@@ -36,6 +39,7 @@ struct AxisPass {
     int lines = 0;           // register kernel (N = 16 * N2): lines per tile on strided axes
                              // (16 / 32 / 64), 0 = default, -1 = LDS Stockham kernel only
     float2* img_out = nullptr;
+    bool bf_in = false, bf_out = false;  // in / out stored as bf16x2 (register kernel only)
 };
 
 void fft_pass(const AxisPass& p, hipStream_t s);

@@ -115,6 +115,27 @@ void scan_i32_to_i64_inclusive(const int* in, int64_t* out, int64_t n, hipStream
 void fill_f32(float* p, float v, int64_t n, hipStream_t s);
 void fill_i32(int* p, int v, int64_t n, hipStream_t s);
 
+// ---------------------------------------------------------------- storage types
+// A complex value stored as two bfloat16 (real in the low half), the
+// reduced-precision storage of the mixed-precision solver.  Arithmetic is
+// always float: ld_c widens exactly, st_c rounds to nearest even.
+struct bf16x2 {
+    uint32_t v;
+};
+__device__ __forceinline__ float2 ld_c(const float2* p) { return *p; }
+__device__ __forceinline__ float2 ld_c(const bf16x2* p) {
+    const uint32_t u = p->v;
+    return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+__device__ __forceinline__ uint32_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ void st_c(float2* p, float2 v) { *p = v; }
+__device__ __forceinline__ void st_c(bf16x2* p, float2 v) { p->v = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16); }
+
 // ---------------------------------------------------------------- sparse
 // Device-resident, decoded form of a cpp_prepared one-block bmSparseMat.
 struct SparseDev {
@@ -169,6 +190,9 @@ constexpr int kLongRow = 64;
 // (deterministic; differs from the sequential sum only by rounding).
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
                       const float* yscale, hipStream_t s);
+// The same product with v and / or w stored as bf16x2 (sums in float).
+void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
+                         const float* yscale, hipStream_t s);
 
 }  // namespace mona
 

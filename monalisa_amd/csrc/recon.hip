@@ -203,6 +203,23 @@ __global__ void k_scatter_pts(const float2* __restrict__ src, const int* __restr
     int c = (int)(t - s * nCh);
     dst[(int64_t)perm[s] * nCh + c] = src[t];
 }
+// bf16x2 tile order <-> float2 caller order
+__global__ void k_scatter_pts_bf(const bf16x2* __restrict__ src, const int* __restrict__ perm,
+                                 float2* __restrict__ dst, int64_t P, int nCh) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * nCh) return;
+    int64_t s = t / nCh;
+    int c = (int)(t - s * nCh);
+    dst[(int64_t)perm[s] * nCh + c] = ld_c(src + t);
+}
+__global__ void k_gather_pts_bf(const float2* __restrict__ src, const int* __restrict__ perm,
+                                bf16x2* __restrict__ dst, int64_t P, int nCh) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * nCh) return;
+    int64_t s = t / nCh;
+    int c = (int)(t - s * nCh);
+    st_c(dst + t, src[(int64_t)perm[s] * nCh + c]);
+}
 __global__ void k_gather_w(const float* __restrict__ src, const int* __restrict__ perm, float* __restrict__ dst,
                            int64_t P) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,6 +247,26 @@ k_sqnorm_partial(const float2* a, int64_t n, const float* w, int nvec, double cw
         double m = (double)v.x * v.x + (double)v.y * v.y;
         if (w) m *= (double)w[i / nvec];
         s += m;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] * cw;
+}
+
+// partial[block] = cw * sum_i Re(conj(a_i) b_i), FP64 accumulation in a
+// fixed order (deterministic)
+__global__ void __launch_bounds__(kRedThreads)
+k_dotre_partial(const float2* a, const float2* b, int64_t n, double cw, double* partial) {
+    __shared__ double sh[kRedThreads];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedThreads) {
+        float2 u = a[i], v = b[i];
+        s += (double)u.x * v.x + (double)u.y * v.y;
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -528,7 +565,8 @@ void Plan::ensure_fft_scratch() {
 // Forward: x .* KFC zero-filled to N_u and transformed, one pass per axis;
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
-void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s) {
+void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s,
+                              bool bf) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     int64_t S = nCh;
@@ -551,6 +589,8 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
         p.out_off = 0;
         p.out_cnt = N[a];
         p.inverse = false;
+        p.bf_in = bf && a > 0;
+        p.bf_out = bf;
         if (a == 0) {
             p.mode = 1;
             p.img = x;
@@ -566,7 +606,7 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
 // each pass keeping only the central n_a rows; the pass along x ends with
 // .* KFC_conj (and the coil sum when combine).
 void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, int nb, DevBuf& ba,
-                              DevBuf& bb, hipStream_t s) {
+                              DevBuf& bb, hipStream_t s, bool bf) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     for (int a = imDim - 1; a >= 0; a--) {
@@ -589,6 +629,8 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         p.out_os = (int64_t)n[a] * S;
         p.inverse = true;
         p.scale = 1.0f / (float)N[a];
+        p.bf_in = bf;
+        p.bf_out = bf && a > 0;
         if (a == 0) {
             p.mode = combine ? 2 : 3;
             p.kfc = KFCc.as<float2>();
@@ -770,8 +812,8 @@ void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s
     k_hy<<<nb256(n), 256, 0, s>>>(ve, vmax, hy, n);
     MONA_HIP(hipGetLastError());
 }
-void cg_alpha(CgScalars* sc, hipStream_t s) {
-    k_cg_alpha<<<1, 1, 0, s>>>(sc, 10.0 * 1.1920928955078125e-07);  // 10*eps('single')
+void cg_alpha(CgScalars* sc, hipStream_t s, double eps) {
+    k_cg_alpha<<<1, 1, 0, s>>>(sc, eps);
     MONA_HIP(hipGetLastError());
 }
 void cg_beta(CgScalars* sc, hipStream_t s) {
@@ -861,6 +903,151 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
     }
 }
 
+// ================================================================ mixed precision
+// Operators with the grid-side arrays (FFT intermediates, oversampled grid)
+// and the k-space vector stored as bf16x2; gridding and FFT arithmetic in
+// float.  k-space in the frame's tile order.
+void Plan::check_bf16() const {
+    MONA_CHECK(nCh <= 64, MONA_ERR_ARG, "bf16 storage needs nCh <= 64");
+    for (int d = 0; d < imDim; d++)
+        MONA_CHECK(fft_reg_supported(N[d]), MONA_ERR_ARG,
+                   "bf16 storage needs N_u = 16 * {2,3,4,5,6,8,10,12,16} in every dimension");
+}
+
+void Plan::forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s) {
+    MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
+    check_bf16();
+    const Frame& fr = frame(f);
+    ensure_grid();
+    ensure_fft_scratch();
+    {
+        TimedRegion t(2, s);
+        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true);
+    }
+    {
+        TimedRegion t(0, s);
+        spmv_c2_rowmajor_st(*fr.Gu_s, grid.p, true, y, true, nCh, nullptr, s);
+    }
+}
+
+void Plan::adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, float scale, hipStream_t s) {
+    MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
+    check_bf16();
+    const Frame& fr = frame(f);
+    ensure_grid();
+    ensure_fft_scratch();
+    {
+        TimedRegion t(1, s);
+        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s);
+    }
+    {
+        TimedRegion t(2, s);
+        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true);
+    }
+}
+
+void dotre(const float2* a, const float2* b, int64_t n, double cw, double* out, double* partial, hipStream_t s) {
+    int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+    k_dotre_partial<<<nb, kRedThreads, 0, s>>>(a, b, n, cw, partial);
+    MONA_HIP(hipGetLastError());
+    k_reduce_final<<<1, kRedThreads, 0, s>>>(partial, nb, out, 0);
+    MONA_HIP(hipGetLastError());
+}
+
+// CG-SENSE by iterative refinement with reliable updates.  CG on the normal
+// equations of bmSensa.m:6-96, (1/HX) M^H HY M x = (1/HX) M^H HY y, with the
+// same HX / HY inner products, in image space.  Its products A p use the
+// operators with bf16x2 storage (bf16) or float storage; every nInner steps
+// the recursively updated residual is replaced by the true residual
+//   g_k = (1/HX) M^H HY (y - M x_k)     (float operators)
+// and CG continues with its search direction (no restart), so the
+// reduced-precision products steer the Krylov iteration while the float
+// residual sets where it converges.  rel_k = ||g_k||_X / ||b||_X with
+// b = (1/HX) M^H HY y (the residual of x = 0); stop when rel_k < tol or
+// after nOuter replacements.  Scalars in FP64.
+void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve_max, int nInner, int nOuter,
+                    double tol, bool bf16, double* rel_out, int* n_outer, hipStream_t s) {
+    MONA_CHECK(nInner >= 1 && nOuter >= 0, MONA_ERR_ARG, "nInner must be >= 1 and nOuter >= 0");
+    if (bf16) check_bf16();
+    const Frame& fr = frame(f);
+    int64_t P = fr.Gu->l_size;
+    int64_t ny = P * nCh;
+    if (!(ve_max > 0)) ve_max = device_max(ve, P, s);
+    Work& w = work;
+    w.hy.ensure((size_t)P * sizeof(float));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.i1.ensure((size_t)nu * sizeof(float2));
+    w.i2.ensure((size_t)nu * sizeof(float2));
+    w.i4.ensure((size_t)nu * sizeof(float2));
+    w.sc.ensure(sizeof(CgScalars));
+    w.partial.ensure(kRedBlocks * sizeof(double));
+    w.hist.ensure((size_t)(nOuter + 2) * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2* res = w.k1.as<float2>();
+    float2* r = w.i1.as<float2>();
+    float2* p = w.i2.as<float2>();
+    float2* q = w.i4.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    double* hist = w.hist.as<double>();
+    w.hy0.ensure((size_t)P * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    compute_hy(ve, ve_max, w.hy0.as<float>(), P, s);
+    weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
+    to_tile_order(f, y, w.ys.as<float2>(), s);
+    y = w.ys.as<float2>();
+    const float invHX = (float)(1.0 / HX);
+    // true residual r = (1/HX) M^H HY (y - M x) into r, its squared norm into dst
+    auto true_residual = [&](double* dst) {
+        forward(f, x, res, s);
+        vsub(y, res, res, ny, s);
+        adjoint(f, res, r, true, hy, invHX, s);
+        sqnorm(r, nu, nullptr, 1, HX, dst, false, part, s);
+    };
+    std::vector<double> rr(nOuter + 1, 0.0);
+    double bb = 0.0;
+    adjoint(f, y, r, true, hy, invHX, s);                    // b = (1/HX) M^H HY y
+    sqnorm(r, nu, nullptr, 1, HX, hist + nOuter + 1, false, part, s);
+    MONA_HIP(hipMemcpyAsync(&bb, hist + nOuter + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    cg_reset(sc, s);
+    true_residual(&sc->sqn);
+    dstore(hist, &sc->sqn, s);
+    vcopy(r, p, nu, s);
+    int k = 0;
+    for (;; k++) {
+        MONA_HIP(hipMemcpyAsync(&rr[k], hist + k, sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+        const double rel = bb > 0 ? std::sqrt(rr[k] / bb) : 0.0;
+        if (rel_out) rel_out[k] = rel;
+        if (rel < tol || k == nOuter || !(rr[k] > 0)) break;
+        for (int i = 1; i <= nInner; i++) {
+            if (bf16) {
+                bf16x2* Mp = reinterpret_cast<bf16x2*>(w.k2.p);
+                forward_bf16(f, p, Mp, s);
+                adjoint_bf16(f, Mp, q, hy, invHX, s);
+            } else {
+                float2* Mp = w.k2.as<float2>();
+                forward(f, p, Mp, s);
+                adjoint(f, Mp, q, true, hy, invHX, s);
+            }
+            dotre(p, q, nu, HX, &sc->den, part, s);          // <p, A p>_X
+            cg_alpha(sc, s, 2.2250738585072014e-308);        // relative solve: stop only at 0
+            vaxpy(x, p, &sc->a, 1.0f, nu, s);                // x += a p
+            if (i < nInner) {
+                vaxpy(r, q, &sc->a, -1.0f, nu, s);           // r -= a A p
+                sqnorm(r, nu, nullptr, 1, HX, &sc->sqn_new, false, part, s);
+            } else {
+                true_residual(&sc->sqn_new);                  // reliable update
+                dstore(hist + k + 1, &sc->sqn_new, s);
+            }
+            cg_beta(sc, s);
+            vxpby(p, r, &sc->b, nu, s);                      // p = r + b p
+        }
+    }
+    if (n_outer) *n_outer = k;
+}
+
 }  // namespace mona
 
 using namespace mona;
@@ -947,6 +1134,43 @@ int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, 
         MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
         P->P.sensa(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nIter, nCGD,
                    residual, stream);
+    });
+}
+
+int mona_sensa_ir(mona_plan P, int frame, void* x, const void* y, const float* ve, float ve_max, int nInner,
+                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
+        P->P.sensa_ir(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nInner, nOuter,
+                      tol, bf16 != 0, rel_res, n_outer, stream);
+    });
+}
+
+int mona_shanna_bf16(mona_plan P, int frame, const void* x, void* y, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
+        Plan& p = P->P;
+        const Frame& fr = p.frame(frame);
+        p.work.kpub.ensure((size_t)fr.P * p.nCh * sizeof(float2));
+        bf16x2* t = reinterpret_cast<bf16x2*>(p.work.kpub.p);
+        p.forward_bf16(frame, static_cast<const float2*>(x), t, stream);
+        k_scatter_pts_bf<<<nb256(fr.P * p.nCh), 256, 0, stream>>>(t, fr.perm->as<int>(), static_cast<float2*>(y),
+                                                                   fr.P, p.nCh);
+        MONA_HIP(hipGetLastError());
+    });
+}
+
+int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y, MONA_ERR_ARG, "NULL argument");
+        Plan& p = P->P;
+        const Frame& fr = p.frame(frame);
+        p.work.kpub.ensure((size_t)fr.P * p.nCh * sizeof(float2));
+        bf16x2* t = reinterpret_cast<bf16x2*>(p.work.kpub.p);
+        k_gather_pts_bf<<<nb256(fr.P * p.nCh), 256, 0, stream>>>(static_cast<const float2*>(y), fr.perm->as<int>(),
+                                                                  t, fr.P, p.nCh);
+        MONA_HIP(hipGetLastError());
+        p.adjoint_bf16(frame, t, static_cast<float2*>(x), nullptr, 1.0f, stream);
     });
 }
 

@@ -48,7 +48,7 @@ class TimedRegion {
 };
 
 struct Work {
-    DevBuf hy, k1, k2, i1, i2, sc, partial, hist;
+    DevBuf hy, k1, k2, i1, i2, i3, i4, sc, partial, hist;
     DevBuf ys, hy0;             // k-space data and weights in tile order
     DevBuf kpub;                // staging of the public operator calls
     DevBuf z1, z2, z3, z4, z5;  // ADMM image-space buffers
@@ -78,9 +78,11 @@ struct Plan {
 
     // FFT stages of the operators over nb frames stored back to back
     // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
-    void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s);
+    // (bf: grid-side arrays stored as bf16x2)
+    void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s,
+                            bool bf = false);
     void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, int nb, DevBuf& a, DevBuf& b,
-                            hipStream_t s);
+                            hipStream_t s, bool bf = false);
     int64_t fft_scratch_elems() const;
     void ensure_fft_scratch();
 
@@ -113,6 +115,12 @@ struct Plan {
     void weights_to_tile_order(int f, const float* src, float* dst, hipStream_t s);
     void sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
                double* residual, hipStream_t s);
+    // mixed precision: grid-side arrays and k-space in bf16x2 storage
+    void check_bf16() const;
+    void forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s);
+    void adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, float scale, hipStream_t s);
+    void sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve_max, int nInner, int nOuter,
+                  double tol, bool bf16, double* rel_out, int* n_outer, hipStream_t s);
 };
 
 // vector helpers (recon.hip)
@@ -122,8 +130,11 @@ void vaxpy(float2* y, const float2* x, const double* a, float sign, int64_t n, h
 void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s);
 void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
             double* partial, hipStream_t s);
+void dotre(const float2* a, const float2* b, int64_t n, double cw, double* out, double* partial, hipStream_t s);
 void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s);
-void cg_alpha(CgScalars* sc, hipStream_t s);
+// a = sqn / den; the sticky done flag is raised when sqn < eps (bmSensa.m:54
+// uses 10 * eps('single'))
+void cg_alpha(CgScalars* sc, hipStream_t s, double eps = 10.0 * 1.1920928955078125e-07);
 void cg_beta(CgScalars* sc, hipStream_t s);
 void cg_reset(CgScalars* sc, hipStream_t s);
 void dstore(double* dst, const double* src, hipStream_t s);

@@ -119,10 +119,10 @@ spmv_planes(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
 // in stored order.  Entries are taken eight at a time: their column indices
 // and values, then the eight gathers, are issued before the adds so that
 // the loads are in flight together; the additions stay in entry order.
-template <bool SCALE>
+template <bool SCALE, class TV>
 __device__ __forceinline__ void row_sum(const int* __restrict__ col, const float* __restrict__ val,
                                         int64_t jb, int64_t je, int n_vec, int k,
-                                        const float2* __restrict__ v, const float* __restrict__ scale,
+                                        const TV* __restrict__ v, const float* __restrict__ scale,
                                         float& ar, float& ai) {
     constexpr int U = 8;
     for (int64_t j = jb; j < je; j += U) {
@@ -136,7 +136,7 @@ __device__ __forceinline__ void row_sum(const int* __restrict__ col, const float
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (u < cnt) {
-                x[u] = v[(int64_t)cj[u] * n_vec + k];
+                x[u] = ld_c(v + (int64_t)cj[u] * n_vec + k);
                 if (SCALE) { float s = scale[cj[u]]; x[u].x = s * x[u].x; x[u].y = s * x[u].y; }
             }
 #pragma unroll
@@ -149,14 +149,15 @@ __device__ __forceinline__ void row_sum(const int* __restrict__ col, const float
     }
 }
 
-// Complex interleaved rows, vector index fastest: v[col][n_vec] float2.
-// Optional per-column scale s[col] (a k-space weight HY applied to v before
-// the product, rounded exactly as a separate elementwise product would be).
-template <bool SCALE>
+// Complex interleaved rows, vector index fastest: v[col][n_vec] (float2 or
+// bf16x2 storage).  Optional per-column scale s[col] (a k-space weight HY
+// applied to v before the product, rounded exactly as a separate
+// elementwise product would be).
+template <bool SCALE, class TV, class TW>
 __global__ void __launch_bounds__(256)
 spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
         const float* __restrict__ val, const int* __restrict__ rowidx, int nrows, int n_vec,
-        const float2* __restrict__ v, float2* __restrict__ w, const float* __restrict__ scale) {
+        const TV* __restrict__ v, TW* __restrict__ w, const float* __restrict__ scale) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t total = (int64_t)nrows * n_vec;
     if (t >= total) return;
@@ -166,15 +167,15 @@ spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
     float ar = 0.0f, ai = 0.0f;
     row_sum<SCALE>(col, val, jb, je, n_vec, k, v, scale, ar, ai);
     int row = rowidx ? rowidx[i] : i;
-    w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
+    st_c(w + (int64_t)row * n_vec + k, make_float2(ar, ai));
 }
 
 // Partial sums of the chunks of long rows: chunk c covers entries
 // [chunk_be[2c], chunk_be[2c+1]).
-template <bool SCALE>
+template <bool SCALE, class TV>
 __global__ void __launch_bounds__(256)
 spmv_c2_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const int* __restrict__ col,
-               const float* __restrict__ val, int n_vec, const float2* __restrict__ v,
+               const float* __restrict__ val, int n_vec, const TV* __restrict__ v,
                const float* __restrict__ scale, float2* __restrict__ partial) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_chunk * n_vec) return;
@@ -186,10 +187,11 @@ spmv_c2_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const int*
 }
 
 // Adds the chunk partials of each long row in chunk order.
+template <class TW>
 __global__ void __launch_bounds__(256)
 spmv_c2_combine(const int* __restrict__ long_row, const int* __restrict__ long_first, int n_long,
                 const int* __restrict__ rowidx, int n_vec, const float2* __restrict__ partial,
-                float2* __restrict__ w) {
+                TW* __restrict__ w) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n_long * n_vec) return;
     int r = (int)(t / n_vec), k = (int)(t - (int64_t)r * n_vec);
@@ -201,7 +203,7 @@ spmv_c2_combine(const int* __restrict__ long_row, const int* __restrict__ long_f
     }
     int i = long_row[r];
     int row = rowidx ? rowidx[i] : i;
-    w[(int64_t)row * n_vec + k] = make_float2(ar, ai);
+    st_c(w + (int64_t)row * n_vec + k, make_float2(ar, ai));
 }
 
 __global__ void expand_rows(const int64_t* rowptr, int nrows, int* ind) {
@@ -404,11 +406,12 @@ void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector
     S.on_device = true;
 }
 
-void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s) {
+namespace {
+template <class TV, class TW>
+void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
     if (S.squeezed || S.l_nJump != S.l_size)
-        fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * 2, s);
+        fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
     int64_t total = (int64_t)S.l_nJump * n_vec;
     if (total == 0) return;
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
@@ -438,6 +441,24 @@ void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
                                                             S.n_long, ri, n_vec, part, w);
         MONA_HIP(hipGetLastError());
     }
+}
+}  // namespace
+
+void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
+                      const float* yscale, hipStream_t s) {
+    spmv_c2_typed(S, v, w, n_vec, yscale, s);
+}
+
+void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
+                         const float* yscale, hipStream_t s) {
+    if (v_bf16 && w_bf16)
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s);
+    else if (v_bf16)
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s);
+    else if (w_bf16)
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s);
+    else
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s);
 }
 
 // -------------------------------------------------------------- host-pointer products

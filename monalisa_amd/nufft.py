@@ -136,6 +136,41 @@ class Plan:
                              stream_of(stream)))
         return list(res)[:nIter]
 
+    def sensa_refined(self, x, y, ve, nInner, nOuter, tol=1e-5, bf16=True, ve_max=None, f=0, stream=None):
+        """Mixed-precision CG-SENSE (no reference counterpart): CG on the
+        normal equations with bf16-storage (or float) operator products;
+        every nInner steps the residual is replaced by the true float one,
+        until its norm relative to that of x = 0 is < tol, or nOuter
+        replacements.  x in place (any start; nOuter = 0 only evaluates the
+        relative residual of x); returns the relative residual at the start
+        and after every replacement."""
+        _require_cuda(x, "x")
+        _require_cuda(y, "y")
+        _require_cuda(ve, "ve")
+        rel = (C.c_double * (nOuter + 1))()
+        n_outer = C.c_int()
+        check(lib.mona_sensa_ir(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
+                                float(ve_max) if ve_max else 0.0, int(nInner), int(nOuter), float(tol),
+                                1 if bf16 else 0, rel, C.byref(n_outer), stream_of(stream)))
+        return list(rel)[:n_outer.value + 1]
+
+    def bmShanna_bf16(self, x, f=0, stream=None):
+        """bmShanna with bf16 storage of the grid-side arrays and k-space."""
+        import torch
+        _require_cuda(x, "x")
+        out = torch.empty((self.nPt(f), self.nCh), dtype=torch.complex64, device=x.device)
+        check(lib.mona_shanna_bf16(self._h, int(f), ptr(x), ptr(out), stream_of(stream)))
+        return out
+
+    def bmNakatsha_bf16(self, y, f=0, stream=None):
+        """bmNakatsha (coil-combined) with bf16 storage of the grid-side
+        arrays and k-space."""
+        import torch
+        _require_cuda(y, "y")
+        out = torch.empty((self.nu,), dtype=torch.complex64, device=y.device)
+        check(lib.mona_nakatsha_bf16(self._h, int(f), ptr(y), ptr(out), stream_of(stream)))
+        return out
+
     def bmSteva(self, x, z, u, y, ve, delta, rho, nCGD, ve_max=None, nIter=1, f=0, stream=None):
         """Spatial-TV ADMM in place on x, z, u (bmSteva.m:6-138).  z, u are
         [imDim, nu] complex64 (given) or None (z = grad x, u = 0)."""
