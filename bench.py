@@ -263,8 +263,8 @@ def bench_mixed(args, cfg):
         "target_rel_residual": tol,
         "fp32_cg_rel_residual": rho32,
         "fp32_cg_recons_per_s": 1.0 / s_cg,
-        "cg_steps_bf16": (len(h16) - 1) * cfg["nInner"],
-        "cg_steps_fp32_refined": (len(h32) - 1) * cfg["nInner"],
+        "cg_steps_bf16": h16.steps,
+        "cg_steps_fp32_refined": h32.steps,
         "final_rel_residual_bf16": h16[-1],
         "final_rel_residual_fp32_refined": h32[-1],
         "fp32_refined_recons_per_s": 1.0 / s32,

@@ -271,13 +271,17 @@ int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* 
  * float (bf16 = 0); every nInner steps the recursive residual is replaced
  * by the true one computed in float, g = (1/HX) M^H HY (y - M x), and CG
  * continues.  rel = ||g|| / ||b||, b = (1/HX) M^H HY y (the residual of
- * x = 0); stops when rel < tol or after nOuter replacements.  x in place
- * (any start); rel_res receives rel at the start and after every
- * replacement (nOuter + 1 doubles), *n_outer the replacements done.
+ * x = 0).  When the recursive residual falls below tol the true one is
+ * computed at once (at most once per nInner steps); the solve stops when a
+ * true rel < tol or after nOuter periodic replacements.  x in place (any
+ * start); rel_res receives rel at the start and after every replacement
+ * (room for 2 * nOuter + 1 doubles), *n_outer the replacements done and
+ * *n_steps the CG steps (either may be NULL).
  * bf16 storage needs every N_u = 16 * {2,3,4,5,6,8,10,12,16} and
  * nCh <= 64. */
 int mona_sensa_ir(mona_plan P, int frame, void* x, const void* y, const float* ve, float ve_max, int nInner,
-                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, hipStream_t stream);
+                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, int* n_steps,
+                  hipStream_t stream);
 /* bmShanna / bmNakatsha (C_flag = 1) with bf16 storage of the grid-side
  * arrays and of the k-space vector; caller-order float k-space in / out. */
 int mona_shanna_bf16(mona_plan P, int frame, const void* x, void* y, hipStream_t stream);

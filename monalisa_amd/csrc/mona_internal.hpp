@@ -137,6 +137,12 @@ __device__ __forceinline__ void st_c(float2* p, float2 v) { *p = v; }
 __device__ __forceinline__ void st_c(bf16x2* p, float2 v) { p->v = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16); }
 
 // ---------------------------------------------------------------- sparse
+// One stored entry, column and value side by side (one 8-byte load).
+struct SpEntry {
+    int col;
+    float val;
+};
+
 // Device-resident, decoded form of a cpp_prepared one-block bmSparseMat.
 struct SparseDev {
     int r_size = 0, l_size = 0, l_nJump = 0;
@@ -147,6 +153,7 @@ struct SparseDev {
     DevBuf val;               // float[nnz]
     DevBuf rowptr;            // int64[l_nJump+1]
     DevBuf rowidx;            // int32[l_nJump] output row (only if squeezed)
+    DevBuf cv;                // SpEntry[nnz] (column, value) interleaved (operator matrices)
     // rows longer than kLongRow entries, split into chunks for the operator
     // products (fixed-order partial sums; see spmv_c2_rowmajor)
     int n_long = 0;

@@ -962,11 +962,14 @@ void dotre(const float2* a, const float2* b, int64_t n, double cw, double* out, 
 //   g_k = (1/HX) M^H HY (y - M x_k)     (float operators)
 // and CG continues with its search direction (no restart), so the
 // reduced-precision products steer the Krylov iteration while the float
-// residual sets where it converges.  rel_k = ||g_k||_X / ||b||_X with
-// b = (1/HX) M^H HY y (the residual of x = 0); stop when rel_k < tol or
-// after nOuter replacements.  Scalars in FP64.
+// residual sets where it converges.  rel = ||g||_X / ||b||_X with
+// b = (1/HX) M^H HY y (the residual of x = 0).  After every step the
+// recursive residual's norm is read; when it falls below tol the true
+// residual is computed at once (at most once per period) and the solve stops
+// if that is below tol too; it also stops after nOuter periodic
+// replacements.  Scalars in FP64.
 void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve_max, int nInner, int nOuter,
-                    double tol, bool bf16, double* rel_out, int* n_outer, hipStream_t s) {
+                    double tol, bool bf16, double* rel_out, int* n_outer, int* n_steps, hipStream_t s) {
     MONA_CHECK(nInner >= 1 && nOuter >= 0, MONA_ERR_ARG, "nInner must be >= 1 and nOuter >= 0");
     if (bf16) check_bf16();
     const Frame& fr = frame(f);
@@ -982,7 +985,7 @@ void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve
     w.i4.ensure((size_t)nu * sizeof(float2));
     w.sc.ensure(sizeof(CgScalars));
     w.partial.ensure(kRedBlocks * sizeof(double));
-    w.hist.ensure((size_t)(nOuter + 2) * sizeof(double));
+    w.hist.ensure(sizeof(double));
     float* hy = w.hy.as<float>();
     float2* res = w.k1.as<float2>();
     float2* r = w.i1.as<float2>();
@@ -1005,47 +1008,67 @@ void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve
         adjoint(f, res, r, true, hy, invHX, s);
         sqnorm(r, nu, nullptr, 1, HX, dst, false, part, s);
     };
-    std::vector<double> rr(nOuter + 1, 0.0);
     double bb = 0.0;
     adjoint(f, y, r, true, hy, invHX, s);                    // b = (1/HX) M^H HY y
-    sqnorm(r, nu, nullptr, 1, HX, hist + nOuter + 1, false, part, s);
-    MONA_HIP(hipMemcpyAsync(&bb, hist + nOuter + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    sqnorm(r, nu, nullptr, 1, HX, hist, false, part, s);
+    MONA_HIP(hipMemcpyAsync(&bb, hist, sizeof(double), hipMemcpyDeviceToHost, s));
     cg_reset(sc, s);
     true_residual(&sc->sqn);
-    dstore(hist, &sc->sqn, s);
     vcopy(r, p, nu, s);
-    int k = 0;
-    for (;; k++) {
-        MONA_HIP(hipMemcpyAsync(&rr[k], hist + k, sizeof(double), hipMemcpyDeviceToHost, s));
+    // host copy of a device FP64 value; the stream is synchronised
+    auto read = [&](const double* d) {
+        double h = 0.0;
+        MONA_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
         MONA_HIP(hipStreamSynchronize(s));
-        const double rel = bb > 0 ? std::sqrt(rr[k] / bb) : 0.0;
-        if (rel_out) rel_out[k] = rel;
-        if (rel < tol || k == nOuter || !(rr[k] > 0)) break;
-        for (int i = 1; i <= nInner; i++) {
-            if (bf16) {
-                bf16x2* Mp = reinterpret_cast<bf16x2*>(w.k2.p);
-                forward_bf16(f, p, Mp, s);
-                adjoint_bf16(f, Mp, q, hy, invHX, s);
-            } else {
-                float2* Mp = w.k2.as<float2>();
-                forward(f, p, Mp, s);
-                adjoint(f, Mp, q, true, hy, invHX, s);
-            }
-            dotre(p, q, nu, HX, &sc->den, part, s);          // <p, A p>_X
-            cg_alpha(sc, s, 2.2250738585072014e-308);        // relative solve: stop only at 0
-            vaxpy(x, p, &sc->a, 1.0f, nu, s);                // x += a p
-            if (i < nInner) {
-                vaxpy(r, q, &sc->a, -1.0f, nu, s);           // r -= a A p
-                sqnorm(r, nu, nullptr, 1, HX, &sc->sqn_new, false, part, s);
-            } else {
-                true_residual(&sc->sqn_new);                  // reliable update
-                dstore(hist + k + 1, &sc->sqn_new, s);
-            }
-            cg_beta(sc, s);
-            vxpby(p, r, &sc->b, nu, s);                      // p = r + b p
+        return h;
+    };
+    auto rel_of = [&](double sq) { return bb > 0 ? std::sqrt(sq / bb) : 0.0; };
+    int k = 0;  // replacements recorded
+    double rel = rel_of(read(&sc->sqn));
+    if (rel_out) rel_out[0] = rel;
+    int periodic = 0, in_period = 0, steps = 0;
+    bool armed = true;  // an early replacement may be tried (once per period)
+    while (rel >= tol && periodic < nOuter && bb > 0) {
+        in_period++;
+        steps++;
+        if (bf16) {
+            bf16x2* Mp = reinterpret_cast<bf16x2*>(w.k2.p);
+            forward_bf16(f, p, Mp, s);
+            adjoint_bf16(f, Mp, q, hy, invHX, s);
+        } else {
+            float2* Mp = w.k2.as<float2>();
+            forward(f, p, Mp, s);
+            adjoint(f, Mp, q, true, hy, invHX, s);
         }
+        dotre(p, q, nu, HX, &sc->den, part, s);              // <p, A p>_X
+        cg_alpha(sc, s, 2.2250738585072014e-308);            // relative solve: stop only at 0
+        vaxpy(x, p, &sc->a, 1.0f, nu, s);                    // x += a p
+        bool replaced = false;
+        if (in_period < nInner) {
+            vaxpy(r, q, &sc->a, -1.0f, nu, s);               // r -= a A p
+            sqnorm(r, nu, nullptr, 1, HX, &sc->sqn_new, false, part, s);
+            if (armed && rel_of(read(&sc->sqn_new)) < tol) {  // converged by the estimate:
+                true_residual(&sc->sqn_new);                  // confirm with the true one
+                armed = false;
+                replaced = true;
+            }
+        } else {
+            true_residual(&sc->sqn_new);                      // periodic reliable update
+            periodic++;
+            in_period = 0;
+            armed = true;
+            replaced = true;
+        }
+        if (replaced) {
+            rel = rel_of(read(&sc->sqn_new));
+            k++;
+            if (rel_out) rel_out[k] = rel;
+        }
+        cg_beta(sc, s);
+        vxpby(p, r, &sc->b, nu, s);                          // p = r + b p
     }
     if (n_outer) *n_outer = k;
+    if (n_steps) *n_steps = steps;
 }
 
 }  // namespace mona
@@ -1138,11 +1161,12 @@ int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, 
 }
 
 int mona_sensa_ir(mona_plan P, int frame, void* x, const void* y, const float* ve, float ve_max, int nInner,
-                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, hipStream_t stream) {
+                  int nOuter, double tol, int bf16, double* rel_res, int* n_outer, int* n_steps,
+                  hipStream_t stream) {
     return guarded([&] {
         MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
         P->P.sensa_ir(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nInner, nOuter,
-                      tol, bf16 != 0, rel_res, n_outer, stream);
+                      tol, bf16 != 0, rel_res, n_outer, n_steps, stream);
     });
 }
 

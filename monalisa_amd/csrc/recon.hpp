@@ -120,7 +120,7 @@ struct Plan {
     void forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s);
     void adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, float scale, hipStream_t s);
     void sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve_max, int nInner, int nOuter,
-                  double tol, bool bf16, double* rel_out, int* n_outer, hipStream_t s);
+                  double tol, bool bf16, double* rel_out, int* n_outer, int* n_steps, hipStream_t s);
 };
 
 // vector helpers (recon.hip)

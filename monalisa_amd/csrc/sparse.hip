@@ -9,6 +9,7 @@
 // (src/sparseMat/cpp/bmSparseMat_cC_oBlock_omp.cpp:99-110), hence
 // bit-identical results with no atomics.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "mona_internal.hpp"
@@ -168,6 +169,137 @@ spmv_c2(const int64_t* __restrict__ rowptr, const int* __restrict__ col,
     row_sum<SCALE>(col, val, jb, je, n_vec, k, v, scale, ar, ai);
     int row = rowidx ? rowidx[i] : i;
     st_c(w + (int64_t)row * n_vec + k, make_float2(ar, ai));
+}
+
+// CPT consecutive complex values (16-byte aligned for CPT > 1).
+template <int CPT>
+__device__ __forceinline__ void ld_n(const float2* p, float2* x) {
+    if constexpr (CPT == 1) {
+        x[0] = *p;
+    } else {
+#pragma unroll
+        for (int h = 0; h < CPT / 2; h++) {
+            const float4 a = reinterpret_cast<const float4*>(p)[h];
+            x[2 * h] = make_float2(a.x, a.y);
+            x[2 * h + 1] = make_float2(a.z, a.w);
+        }
+    }
+}
+template <int CPT>
+__device__ __forceinline__ void ld_n(const bf16x2* p, float2* x) {
+    if constexpr (CPT == 4) {
+        const uint4 a = *reinterpret_cast<const uint4*>(p);
+        const bf16x2 b[4] = {{a.x}, {a.y}, {a.z}, {a.w}};
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = ld_c(&b[c]);
+    } else if constexpr (CPT == 2) {
+        const uint2 a = *reinterpret_cast<const uint2*>(p);
+        const bf16x2 b[2] = {{a.x}, {a.y}};
+        x[0] = ld_c(&b[0]);
+        x[1] = ld_c(&b[1]);
+    } else {
+        x[0] = ld_c(p);
+    }
+}
+template <int CPT>
+__device__ __forceinline__ void st_n(float2* p, const float* ar, const float* ai) {
+    if constexpr (CPT == 1) {
+        *p = make_float2(ar[0], ai[0]);
+    } else {
+#pragma unroll
+        for (int h = 0; h < CPT / 2; h++)
+            reinterpret_cast<float4*>(p)[h] = make_float4(ar[2 * h], ai[2 * h], ar[2 * h + 1], ai[2 * h + 1]);
+    }
+}
+template <int CPT>
+__device__ __forceinline__ void st_n(bf16x2* p, const float* ar, const float* ai) {
+    uint32_t u[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        bf16x2 b;
+        st_c(&b, make_float2(ar[c], ai[c]));
+        u[c] = b.v;
+    }
+    if constexpr (CPT == 4) *reinterpret_cast<uint4*>(p) = make_uint4(u[0], u[1], u[2], u[3]);
+    else if constexpr (CPT == 2) *reinterpret_cast<uint2*>(p) = make_uint2(u[0], u[1]);
+    else p->v = u[0];
+}
+
+// Row sum over CPT vectors at once: v[col][k0 .. k0+CPT-1], entries packed
+// as (col, val).  Per vector the entries are added one by one in stored
+// order, exactly as row_sum does.
+template <bool SCALE, int CPT, int U, class TV>
+__device__ __forceinline__ void row_sum_cv(const SpEntry* __restrict__ cv, int64_t jb, int64_t je, int n_vec,
+                                           int k0, const TV* __restrict__ v, const float* __restrict__ scale,
+                                           float* ar, float* ai) {
+    for (int64_t j = jb; j < je; j += U) {
+        const int cnt = (int)((je - j) < U ? (je - j) : U);
+        SpEntry e[U];
+        float2 x[U][CPT];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) e[u] = cv[j + u];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) {
+                ld_n<CPT>(v + (int64_t)e[u].col * n_vec + k0, x[u]);
+                if (SCALE) {
+                    const float sc = scale[e[u].col];
+#pragma unroll
+                    for (int c = 0; c < CPT; c++) {
+                        x[u][c].x = sc * x[u][c].x;
+                        x[u][c].y = sc * x[u][c].y;
+                    }
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (u < cnt) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const float pr = e[u].val * x[u][c].x, pi = e[u].val * x[u][c].y;
+                    ar[c] = ar[c] + pr;
+                    ai[c] = ai[c] + pi;
+                }
+            }
+    }
+}
+
+// spmv_c2 with CPT vectors per work-item and packed entries: a quarter of
+// the vector-memory instructions of spmv_c2 at CPT = 4 (the metadata load
+// is shared by CPT vectors and the gathers are 16 bytes wide).
+template <bool SCALE, int CPT, int U, class TV, class TW>
+__global__ void __launch_bounds__(256)
+spmv_cv(const int64_t* __restrict__ rowptr, const SpEntry* __restrict__ cv, const int* __restrict__ rowidx,
+        int nrows, int n_vec, const TV* __restrict__ v, TW* __restrict__ w, const float* __restrict__ scale) {
+    const int tpr = n_vec / CPT;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrows * tpr) return;
+    const int i = (int)(t / tpr), kq = (int)(t - (int64_t)i * tpr);
+    const int64_t jb = rowptr[i], je = rowptr[i + 1];
+    if (je - jb > kLongRow) return;  // summed by spmv_cv_chunks + spmv_c2_combine
+    float ar[CPT], ai[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) ar[c] = ai[c] = 0.0f;
+    row_sum_cv<SCALE, CPT, U>(cv, jb, je, n_vec, kq * CPT, v, scale, ar, ai);
+    const int row = rowidx ? rowidx[i] : i;
+    st_n<CPT>(w + (int64_t)row * n_vec + kq * CPT, ar, ai);
+}
+
+template <bool SCALE, int CPT, int U, class TV>
+__global__ void __launch_bounds__(256)
+spmv_cv_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const SpEntry* __restrict__ cv, int n_vec,
+               const TV* __restrict__ v, const float* __restrict__ scale, float2* __restrict__ partial) {
+    const int tpr = n_vec / CPT;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_chunk * tpr) return;
+    const int64_t c = t / tpr;
+    const int kq = (int)(t - c * tpr);
+    float ar[CPT], ai[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) ar[q] = ai[q] = 0.0f;
+    row_sum_cv<SCALE, CPT, U>(cv, chunk_be[2 * c], chunk_be[2 * c + 1], n_vec, kq * CPT, v, scale, ar, ai);
+    st_n<CPT>(partial + c * n_vec + kq * CPT, ar, ai);
 }
 
 // Partial sums of the chunks of long rows: chunk c covers entries
@@ -402,27 +534,82 @@ void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector
         S.rowidx.alloc(rowidx->size() * sizeof(int));
         MONA_HIP(hipMemcpyAsync(S.rowidx.p, rowidx->data(), rowidx->size() * sizeof(int), hipMemcpyHostToDevice, s));
     }
+    {
+        std::vector<SpEntry> cv(S.nnz);
+        for (int64_t k = 0; k < S.nnz; k++) cv[k] = SpEntry{col[k], val[k]};
+        S.cv.alloc(std::max<int64_t>(S.nnz, 1) * sizeof(SpEntry));
+        MONA_HIP(hipMemcpyAsync(S.cv.p, cv.data(), S.nnz * sizeof(SpEntry), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipStreamSynchronize(s));  // cv is a temporary
+    }
     build_chunks(S, r_nJump.data(), s);
     S.on_device = true;
 }
 
 namespace {
+// Packed-entry products with CPT vectors per work-item (CPT | n_vec, v and w
+// 16-byte aligned, operator matrices only).
+template <int CPT, int U, class TV, class TW>
+void spmv_cv_launch(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s) {
+    const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
+    const int64_t total = (int64_t)S.l_nJump * (n_vec / CPT);
+    const SpEntry* cv = S.cv.as<SpEntry>();
+    if (yscale)
+        spmv_cv<true, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, S.l_nJump,
+                                                                  n_vec, v, w, yscale);
+    else
+        spmv_cv<false, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, S.l_nJump,
+                                                                   n_vec, v, w, nullptr);
+    MONA_HIP(hipGetLastError());
+    if (S.n_long > 0) {
+        S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+        const int64_t tc = S.n_chunk * (n_vec / CPT);
+        const int64_t* cb = S.chunk_begin.as<int64_t>();
+        float2* part = S.partial.as<float2>();
+        if (yscale)
+            spmv_cv_chunks<true, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, cv, n_vec, v, yscale, part);
+        else
+            spmv_cv_chunks<false, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, cv, n_vec, v, nullptr, part);
+        MONA_HIP(hipGetLastError());
+        const int64_t tl = (int64_t)S.n_long * n_vec;
+        spmv_c2_combine<<<blocks_for(tl, 256), 256, 0, s>>>(S.long_row.as<int>(), S.long_first.as<int>(),
+                                                            S.n_long, ri, n_vec, part, w);
+        MONA_HIP(hipGetLastError());
+    }
+}
+
 template <class TV, class TW>
 void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
+    if (S.cv.p && S.nnz > 0) {
+        const bool al = ((uintptr_t)v % 16 == 0) && ((uintptr_t)w % 16 == 0);
+        const int cpt = (al && n_vec % 4 == 0) ? 4 : (al && n_vec % 2 == 0) ? 2 : 1;
+        if (S.squeezed || S.l_nJump != S.l_size)
+            fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
+        if (S.l_nJump == 0) return;
+        // measured on 3D grids at 16 and 20 coils: the gridding gather
+        // (~27 entries a row) runs best with two vectors per work-item and
+        // 16 gathers in flight; the weighted spreading product (rows of ~8
+        // entries) with four vectors and 4 entries a batch
+        if (!yscale && cpt >= 2) spmv_cv_launch<2, 16>(S, v, w, n_vec, yscale, s);
+        else if (cpt == 4) spmv_cv_launch<4, 4>(S, v, w, n_vec, yscale, s);
+        else if (cpt == 2) spmv_cv_launch<2, 8>(S, v, w, n_vec, yscale, s);
+        else spmv_cv_launch<1, 8>(S, v, w, n_vec, yscale, s);
+        return;
+    }
     if (S.squeezed || S.l_nJump != S.l_size)
         fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
     int64_t total = (int64_t)S.l_nJump * n_vec;
     if (total == 0) return;
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
-    if (yscale)
+    if (yscale) {
         spmv_c2<true><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(),
                                                              S.val.as<float>(), ri, S.l_nJump, n_vec,
                                                              v, w, yscale);
-    else
+    } else {
         spmv_c2<false><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), S.col.as<int>(),
                                                               S.val.as<float>(), ri, S.l_nJump, n_vec,
                                                               v, w, nullptr);
+    }
     MONA_HIP(hipGetLastError());
     if (S.n_long > 0) {
         S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));

@@ -139,20 +139,22 @@ class Plan:
     def sensa_refined(self, x, y, ve, nInner, nOuter, tol=1e-5, bf16=True, ve_max=None, f=0, stream=None):
         """Mixed-precision CG-SENSE (no reference counterpart): CG on the
         normal equations with bf16-storage (or float) operator products;
-        every nInner steps the residual is replaced by the true float one,
-        until its norm relative to that of x = 0 is < tol, or nOuter
-        replacements.  x in place (any start; nOuter = 0 only evaluates the
-        relative residual of x); returns the relative residual at the start
-        and after every replacement."""
+        every nInner steps, and as soon as the recursive estimate says
+        converged, the residual is replaced by the true float one; stops
+        when that, relative to the residual of x = 0, is < tol, or after
+        nOuter periodic replacements.  x in place (any start; nOuter = 0 only
+        evaluates the relative residual of x); returns the relative residual
+        at the start and after every replacement."""
         _require_cuda(x, "x")
         _require_cuda(y, "y")
         _require_cuda(ve, "ve")
-        rel = (C.c_double * (nOuter + 1))()
+        rel = (C.c_double * (2 * nOuter + 1))()
         n_outer = C.c_int()
+        n_steps = C.c_int()
         check(lib.mona_sensa_ir(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
                                 float(ve_max) if ve_max else 0.0, int(nInner), int(nOuter), float(tol),
-                                1 if bf16 else 0, rel, C.byref(n_outer), stream_of(stream)))
-        return list(rel)[:n_outer.value + 1]
+                                1 if bf16 else 0, rel, C.byref(n_outer), C.byref(n_steps), stream_of(stream)))
+        return RefineHistory(list(rel)[:n_outer.value + 1], n_steps.value)
 
     def bmShanna_bf16(self, x, f=0, stream=None):
         """bmShanna with bf16 storage of the grid-side arrays and k-space."""
@@ -222,6 +224,15 @@ class Plan:
         out = torch.empty(shape, dtype=torch.complex64, device=y.device)
         check(lib.mona_nasha(self._h, Gn.handle(), ptr(y), ptr(Cmaps), ptr(out), stream_of(stream)))
         return out
+
+
+class RefineHistory(list):
+    """Relative residuals of a refined solve (start, then every residual
+    replacement); .steps = CG steps taken."""
+
+    def __init__(self, values, steps):
+        super().__init__(values)
+        self.steps = steps
 
 
 def _sched(v):

@@ -80,6 +80,7 @@ def test_sensa_refined_restores_fp32_level(bf16):
     hist = P.sensa_refined(x, yd, ved, nInner=10, nOuter=40, tol=tol, bf16=bf16)
     assert hist[0] == 1.0
     assert hist[-1] < tol, (tol, hist)
+    assert hist.steps <= 40 * 10
     assert _normal_residual(x.cpu().numpy(), y, ve, p, R, KFC, KFCc) < 1.05 * tol
 
 
@@ -98,4 +99,4 @@ def test_sensa_refined_deterministic():
         outs.append((x.cpu().numpy(), h))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     assert outs[0][1] == outs[1][1]
-    assert len(outs[0][1]) == 4
+    assert len(outs[0][1]) == 4 and outs[0][1].steps == 15
