@@ -443,6 +443,17 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
                "Gu must have one stored row per point (not squeezed)");
     MONA_CHECK((int64_t)Gu->h_r_jump.size() == Gu->nnz && (int64_t)Gut->h_r_jump.size() == Gut->nnz,
                MONA_ERR_STATE, "gridding matrices without their host encoding");
+    // a frame with the same matrices shares their tile-order copies
+    for (const Frame& o : frames)
+        if (o.Gu == Gu && o.Gut == Gut && o.Gu_s) {
+            Frame copy = o;
+            if ((int)frames.size() <= f) frames.resize(f + 1);
+            frames[f] = copy;
+            Gu_b.reset();
+            Gut_b.reset();
+            koff_b.clear();
+            return;
+        }
     const int64_t P = Gu->l_size;
     // decode Gu's columns (running deltas, first absolute)
     std::vector<int64_t> rowptr(P + 1, 0);
@@ -519,6 +530,9 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     fr.Gut_s = Gut_s;
     fr.perm = dperm;
     fr.P = P;
+    Gu_b.reset();  // the batched matrices are rebuilt on next use
+    Gut_b.reset();
+    koff_b.clear();
 }
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
@@ -903,6 +917,125 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
     }
 }
 
+// ================================================================ memory
+namespace {
+size_t sparse_bytes(const SparseDev& S) {
+    return S.col.bytes + S.val.bytes + S.rowptr.bytes + S.rowidx.bytes + S.cv.bytes + S.long_row.bytes +
+           S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes;
+}
+}  // namespace
+
+size_t Plan::device_bytes() const {
+    size_t b = K.bytes + KFC.bytes + KFCc.bytes + grid.bytes + fft_a.bytes + fft_b.bytes + grid_b.bytes +
+               fft_ab.bytes + fft_bb.bytes;
+    const Work& w = work;
+    for (const DevBuf* d : {&w.hy, &w.k1, &w.k2, &w.i1, &w.i2, &w.i3, &w.i4, &w.sc, &w.partial, &w.hist, &w.ys,
+                            &w.hy0, &w.kpub, &w.z1, &w.z2, &w.z3, &w.z4, &w.z5})
+        b += d->bytes;
+    std::vector<const void*> seen;
+    auto once = [&](const void* q) {
+        if (!q || std::find(seen.begin(), seen.end(), q) != seen.end()) return false;
+        seen.push_back(q);
+        return true;
+    };
+    for (const Frame& fr : frames) {
+        if (once(fr.Gu_s.get())) b += sparse_bytes(*fr.Gu_s);
+        if (once(fr.Gut_s.get())) b += sparse_bytes(*fr.Gut_s);
+        if (once(fr.perm.get())) b += fr.perm->bytes;
+    }
+    if (Gu_b) b += sparse_bytes(*Gu_b);
+    if (Gut_b) b += sparse_bytes(*Gut_b);
+    return b;
+}
+
+// ================================================================ batched CG-SENSE
+// bmSensa over every frame of the plan at once: frame f solves its own
+// system (own data, weights and CG scalars); the operators run as one
+// block-diagonal gridding launch and frame-batched FFT passes over all
+// frames, the reductions and vector updates frame by frame with the same
+// launches as sensa(), so each frame's result equals sensa() on that frame
+// alone.  x [nFr][nu]; y / ve packed per frame at y_off / ve_off (caller
+// point order); residual [nFr][nIter].
+void Plan::sensa_batch(float2* x, const float2* y, const long long* y_off, const float* ve, const long long* ve_off,
+                       const float* ve_max_in, int nIter, int nCGD, double* residual, hipStream_t s) {
+    MONA_CHECK(nIter >= 0 && nCGD >= 0, MONA_ERR_ARG, "nIter and nCGD must be >= 0");
+    const int nFr = (int)frames.size();
+    MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
+    if (!Gu_b || (int)koff_b.size() != nFr + 1) build_batch(s);
+    std::vector<int64_t> Pf(nFr), koff(nFr + 1, 0);
+    for (int f = 0; f < nFr; f++) {
+        Pf[f] = frame(f).Gu->l_size;
+        koff[f + 1] = koff[f] + Pf[f] * nCh;
+    }
+    const int64_t ny = koff[nFr], nx = (int64_t)nFr * nu;
+    Work& w = work;
+    w.hy.ensure((size_t)(ny / nCh) * sizeof(float));
+    w.hy0.ensure((size_t)(ny / nCh) * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.i1.ensure((size_t)nx * sizeof(float2));
+    w.i2.ensure((size_t)nx * sizeof(float2));
+    w.sc.ensure((size_t)nFr * sizeof(CgScalars));
+    w.partial.ensure(kRedBlocks * sizeof(double));
+    w.hist.ensure((size_t)std::max(nIter, 1) * nFr * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2* ys = w.ys.as<float2>();
+    float2* res = w.k1.as<float2>();
+    float2* Mp = w.k2.as<float2>();
+    float2* dag = w.i1.as<float2>();
+    float2* p = w.i2.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    double* hist = w.hist.as<double>();
+    for (int f = 0; f < nFr; f++) {
+        float vmax = ve_max_in ? ve_max_in[f] : 0.0f;
+        if (!(vmax > 0)) vmax = device_max(ve + ve_off[f], Pf[f], s);
+        compute_hy(ve + ve_off[f], vmax, w.hy0.as<float>() + koff[f] / nCh, Pf[f], s);
+        weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / nCh, hy + koff[f] / nCh, s);
+        to_tile_order(f, y + y_off[f], ys + koff[f], s);
+    }
+    const float invHX = (float)(1.0 / HX);
+    auto K = [&](float2* a, int f) { return a + koff[f]; };
+    auto X = [&](float2* a, int f) { return a + (int64_t)f * nu; };
+    for (int it = 0; it < nIter; it++) {
+        for (int f = 0; f < nFr; f++) cg_reset(sc + f, s);
+        forward_batch(x, Mp, s);
+        for (int f = 0; f < nFr; f++) vsub(ys + koff[f], K(Mp, f), K(res, f), koff[f + 1] - koff[f], s);
+        adjoint_batch(res, dag, hy, invHX, s);
+        for (int f = 0; f < nFr; f++) {
+            sqnorm(X(dag, f), nu, nullptr, 1, HX, &sc[f].sqn, false, part, s);
+            vcopy(X(dag, f), X(p, f), nu, s);
+        }
+        for (int i = 1; i <= nCGD; i++) {
+            forward_batch(p, Mp, s);
+            for (int f = 0; f < nFr; f++) {
+                sqnorm(K(Mp, f), koff[f + 1] - koff[f], hy + koff[f] / nCh, nCh, 1.0, &sc[f].den, false, part, s);
+                cg_alpha(sc + f, s);
+                vaxpy(X(x, f), X(p, f), &sc[f].a, 1.0f, nu, s);
+            }
+            if (i == nCGD) break;
+            for (int f = 0; f < nFr; f++) vaxpy(K(res, f), K(Mp, f), &sc[f].a, -1.0f, koff[f + 1] - koff[f], s);
+            adjoint_batch(res, dag, hy, invHX, s);
+            for (int f = 0; f < nFr; f++) {
+                sqnorm(X(dag, f), nu, nullptr, 1, HX, &sc[f].sqn_new, false, part, s);
+                cg_beta(sc + f, s);
+                vxpby(X(p, f), X(dag, f), &sc[f].b, nu, s);
+            }
+        }
+        forward_batch(x, Mp, s);
+        for (int f = 0; f < nFr; f++) {
+            vsub(ys + koff[f], K(Mp, f), K(res, f), koff[f + 1] - koff[f], s);
+            sqnorm(K(res, f), koff[f + 1] - koff[f], hy + koff[f] / nCh, nCh, 1.0, hist + (int64_t)f * nIter + it,
+                   false, part, s);
+        }
+    }
+    if (residual && nIter > 0) {
+        MONA_HIP(hipMemcpyAsync(residual, hist, (size_t)nIter * nFr * sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+}
+
 // ================================================================ mixed precision
 // Operators with the grid-side arrays (FFT intermediates, oversampled grid)
 // and the k-space vector stored as bf16x2; gridding and FFT arithmetic in
@@ -1157,6 +1290,23 @@ int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, 
         MONA_CHECK(P && x && y && ve, MONA_ERR_ARG, "NULL argument");
         P->P.sensa(frame, static_cast<float2*>(x), static_cast<const float2*>(y), ve, ve_max, nIter, nCGD,
                    residual, stream);
+    });
+}
+
+int mona_sensa_batch(mona_plan P, void* x, const void* y, const long long* y_off, const float* ve,
+                     const long long* ve_off, const float* ve_max, int nIter, int nCGD, double* residual,
+                     hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y && y_off && ve && ve_off, MONA_ERR_ARG, "NULL argument");
+        P->P.sensa_batch(static_cast<float2*>(x), static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, nIter,
+                         nCGD, residual, stream);
+    });
+}
+
+int mona_plan_memory(mona_plan P, long long* bytes) {
+    return guarded([&] {
+        MONA_CHECK(P && bytes, MONA_ERR_ARG, "NULL argument");
+        *bytes = (long long)P->P.device_bytes();
     });
 }
 

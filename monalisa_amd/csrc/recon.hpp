@@ -115,6 +115,11 @@ struct Plan {
     void weights_to_tile_order(int f, const float* src, float* dst, hipStream_t s);
     void sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
                double* residual, hipStream_t s);
+    // all frames at once (independent systems, batched operators)
+    void sensa_batch(float2* x, const float2* y, const long long* y_off, const float* ve, const long long* ve_off,
+                     const float* ve_max, int nIter, int nCGD, double* residual, hipStream_t s);
+    // device memory held by the plan (its own buffers and frame matrices)
+    size_t device_bytes() const;
     // mixed precision: grid-side arrays and k-space in bf16x2 storage
     void check_bf16() const;
     void forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s);

@@ -136,6 +136,35 @@ class Plan:
                              stream_of(stream)))
         return list(res)[:nIter]
 
+    def bmSensa_batch(self, x, y_list, ve_list, nCGD, ve_max=None, nIter=1, stream=None):
+        """bmSensa on every frame at once (batched operators, per-frame CG):
+        x [nFr, nu] in place; y_list / ve_list per frame; returns the
+        residuals [nFr][nIter].  Each frame's result equals bmSensa on that
+        frame alone."""
+        import torch
+        _require_cuda(x, "x")
+        nFr = x.shape[0]
+        ycat = y_list if isinstance(y_list, torch.Tensor) else torch.cat([v.reshape(-1) for v in y_list])
+        vcat = ve_list if isinstance(ve_list, torch.Tensor) else torch.cat([v.reshape(-1) for v in ve_list])
+        yo = np.cumsum([0] + [self.nPt(f) * self.nCh for f in range(nFr)])[:-1]
+        vo = np.cumsum([0] + [self.nPt(f) for f in range(nFr)])[:-1]
+        yoff = (C.c_longlong * nFr)(*[int(v) for v in yo])
+        voff = (C.c_longlong * nFr)(*[int(v) for v in vo])
+        vm = None
+        if ve_max is not None:
+            vm = (C.c_float * nFr)(*[float(v) for v in np.broadcast_to(np.asarray(ve_max, dtype=np.float64), (nFr,))])
+        res = (C.c_double * (max(nIter, 1) * nFr))()
+        ycat, vcat = ycat.contiguous(), vcat.contiguous()   # held until the call returns
+        check(lib.mona_sensa_batch(self._h, ptr(x), ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)),
+                                   voff, vm, int(nIter), int(nCGD), res, stream_of(stream)))
+        return np.array(list(res)[:nIter * nFr]).reshape(nFr, nIter)
+
+    def device_bytes(self):
+        """Device memory held by the plan (buffers and frame matrices)."""
+        b = C.c_longlong()
+        check(lib.mona_plan_memory(self._h, C.byref(b)))
+        return b.value
+
     def sensa_refined(self, x, y, ve, nInner, nOuter, tol=1e-5, bf16=True, ve_max=None, f=0, stream=None):
         """Mixed-precision CG-SENSE (no reference counterpart): CG on the
         normal equations with bf16-storage (or float) operator products;
