@@ -264,14 +264,15 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
 int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x,
                hipStream_t stream);
 
-/* bmSensa over every frame of the plan at once (no reference counterpart:
- * the batched form of the serving loop).  Frame f solves its own system
+/* bmSensa over frames 0 .. nFrames-1 of the plan at once (nFrames <= 0: all;
+ * no reference counterpart: the batched form of the serving loop).  Frame
+ * f solves its own system
  * with data y + y_off[f] ([P_f][nCh], caller point order), weights
  * ve + ve_off[f] and ve_max[f] (<= 0: max of its ve; ve_max may be NULL);
  * x is [nFr][prod(n_u)].  The operators run as one block-diagonal gridding
  * launch and frame-batched FFT passes; each frame's result equals
  * mona_sensa on that frame alone.  residual: [nFr][nIter] (may be NULL). */
-int mona_sensa_batch(mona_plan P, void* x, const void* y, const long long* y_off, const float* ve,
+int mona_sensa_batch(mona_plan P, int nFrames, void* x, const void* y, const long long* y_off, const float* ve,
                      const long long* ve_off, const float* ve_max, int nIter, int nCGD, double* residual,
                      hipStream_t stream);
 /* Device memory held by a plan: its buffers and frame matrices. */

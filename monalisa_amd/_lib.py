@@ -69,7 +69,7 @@ SIGNATURES = {
     "mona_steva": (I, [P, I, P, P, P, I, P, P, F, FP, FP, I, I, DP, P]),
     "mona_teva_chain": (I, [P, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_nasha": (I, [P, P, P, P, P, P]),
-    "mona_sensa_batch": (I, [P, P, P, LLP, FP, LLP, FP, I, I, DP, P]),
+    "mona_sensa_batch": (I, [P, I, P, P, LLP, FP, LLP, FP, I, I, DP, P]),
     "mona_plan_memory": (I, [P, LLP]),
     "mona_sensa_ir": (I, [P, I, P, P, FP, F, I, I, D, I, DP, IP, IP, P]),
     "mona_shanna_bf16": (I, [P, I, P, P, P]),

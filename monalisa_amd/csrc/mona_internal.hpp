@@ -195,11 +195,26 @@ constexpr int kLongRow = 64;
 // reference CPU kernel); longer rows are summed as consecutive chunks of
 // kLongRow entries whose partial sums are then added in chunk order
 // (deterministic; differs from the sequential sum only by rounding).
+// A leading part of a matrix: its first `rows` stored rows (with their
+// long-row chunks) writing output rows [0, l_size).
+struct SpView {
+    int rows = 0;
+    int64_t l_size = 0;
+    int n_long = 0;
+    int64_t n_chunk = 0;
+};
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s);
+                      const float* yscale, hipStream_t s, const SpView* view = nullptr);
 // The same product with v and / or w stored as bf16x2 (sums in float).
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
-                         const float* yscale, hipStream_t s);
+                         const float* yscale, hipStream_t s, const SpView* view = nullptr);
+// Block-diagonal stacking on the device: part f's columns are shifted by
+// col_off[f] and its output rows by row_off[f] (out has r_size columns and
+// l_size output rows; its stored rows are the parts' in order).  views[f]
+// (n + 1 entries) is the leading part made of parts 0 .. f-1.
+void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& parts,
+                       const std::vector<int64_t>& col_off, const std::vector<int64_t>& row_off, int r_size,
+                       int l_size, std::vector<SpView>* views, hipStream_t s);
 
 }  // namespace mona
 

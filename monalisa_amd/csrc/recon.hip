@@ -711,89 +711,72 @@ void Plan::adjoint_public(int f, const float2* y, float2* x, bool combine, hipSt
 }
 
 // ---------------------------------------------------------------- batched form
-namespace {
-template <class T>
-std::vector<T> d2h(const DevBuf& b, int64_t n, hipStream_t s) {
-    std::vector<T> h(n);
-    if (n > 0) MONA_HIP(hipMemcpyAsync(h.data(), b.p, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, s));
-    return h;
-}
-}  // namespace
-
-// Block-diagonal stacking of every frame's tile-order matrices: frame f's
-// grid occupies columns [f*Nu, (f+1)*Nu) of Gu_b and rows of Gut_b, its
-// points follow those of frames 0..f-1.
+// Block-diagonal stacking of every frame's tile-order matrices, on the
+// device: frame f's grid occupies columns [f*Nu, (f+1)*Nu) of Gu_b and
+// output rows of Gut_b, its points follow those of frames 0..f-1.  The
+// views of the leading frames let a batch use only frames 0..nA-1.
 void Plan::build_batch(hipStream_t s) {
     const int nFr = (int)frames.size();
     MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
     MONA_CHECK((int64_t)nFr * Nu < 0x7fffffff, MONA_ERR_ARG, "nFrames * prod(N_u) too large for one launch");
-    std::vector<int> ucol, unj, tcol, tnj, trow;
-    std::vector<float> uval, tval;
+    std::vector<const SparseDev*> us, ts;
+    std::vector<int64_t> ucol, urow, tcol, trow;
     koff_b.assign(nFr + 1, 0);
     int64_t poff = 0;
     for (int f = 0; f < nFr; f++) {
         const Frame& fr = frame(f);
-        const SparseDev& U = *fr.Gu_s;
-        const SparseDev& T = *fr.Gut_s;
-        auto c = d2h<int>(U.col, U.nnz, s);
-        auto v = d2h<float>(U.val, U.nnz, s);
-        auto rp = d2h<int64_t>(U.rowptr, U.l_nJump + 1, s);
-        auto tc = d2h<int>(T.col, T.nnz, s);
-        auto tv = d2h<float>(T.val, T.nnz, s);
-        auto trp = d2h<int64_t>(T.rowptr, T.l_nJump + 1, s);
-        std::vector<int> tr = T.squeezed ? d2h<int>(T.rowidx, T.l_nJump, s) : std::vector<int>();
-        MONA_HIP(hipStreamSynchronize(s));
-        for (int64_t k = 0; k < U.nnz; k++) ucol.push_back(c[k] + (int)(f * Nu));
-        uval.insert(uval.end(), v.begin(), v.end());
-        for (int i = 0; i < U.l_nJump; i++) unj.push_back((int)(rp[i + 1] - rp[i]));
-        for (int64_t k = 0; k < T.nnz; k++) tcol.push_back(tc[k] + (int)poff);
-        tval.insert(tval.end(), tv.begin(), tv.end());
-        for (int i = 0; i < T.l_nJump; i++) {
-            tnj.push_back((int)(trp[i + 1] - trp[i]));
-            trow.push_back((T.squeezed ? tr[i] : i) + (int)(f * Nu));
-        }
+        us.push_back(fr.Gu_s.get());
+        ts.push_back(fr.Gut_s.get());
+        ucol.push_back((int64_t)f * Nu);
+        urow.push_back(poff);
+        tcol.push_back(poff);
+        trow.push_back((int64_t)f * Nu);
         poff += fr.P;
         koff_b[f + 1] = koff_b[f] + fr.P * nCh;
     }
     auto gu = std::make_shared<SparseDev>();
-    sparse_from_decoded(*gu, (int)(nFr * Nu), (int)poff, ucol, uval, unj, nullptr, s);
+    sparse_block_diag(*gu, us, ucol, urow, (int)(nFr * Nu), (int)poff, &view_u, s);
     auto gut = std::make_shared<SparseDev>();
-    sparse_from_decoded(*gut, (int)poff, (int)(nFr * Nu), tcol, tval, tnj, &trow, s);
+    sparse_block_diag(*gut, ts, tcol, trow, (int)poff, (int)(nFr * Nu), &view_t, s);
     Gu_b = gu;
     Gut_b = gut;
 }
 
-void Plan::forward_batch(const float2* x, float2* y, hipStream_t s) {
+void Plan::forward_batch(const float2* x, float2* y, hipStream_t s, int nA) {
     MONA_CHECK(coils_set && Gu_b, MONA_ERR_STATE, "batched operators not built");
     const int nFr = (int)frames.size();
+    if (nA <= 0) nA = nFr;
+    MONA_CHECK(nA <= nFr, MONA_ERR_ARG, "more batch frames than the plan has");
     MONA_CHECK(fft_pass_fusable(N[0], nCh), MONA_ERR_ARG, "batched form needs N_x * nCh <= 8192");
     grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
     fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid_b.as<float2>(), nFr, fft_ab, fft_bb, s);
+        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s);
     }
     {
         TimedRegion t(0, s);
-        spmv_c2_rowmajor(*Gu_b, grid_b.as<float2>(), y, nCh, nullptr, s);
+        spmv_c2_rowmajor(*Gu_b, grid_b.as<float2>(), y, nCh, nullptr, s, &view_u[nA]);
     }
 }
 
-void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s) {
+void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s, int nA) {
     MONA_CHECK(coils_set && Gut_b, MONA_ERR_STATE, "batched operators not built");
     const int nFr = (int)frames.size();
+    if (nA <= 0) nA = nFr;
+    MONA_CHECK(nA <= nFr, MONA_ERR_ARG, "more batch frames than the plan has");
     MONA_CHECK(fft_pass_fusable(N[0], nCh), MONA_ERR_ARG, "batched form needs N_x * nCh <= 8192");
     grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
     fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s);
+        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA]);
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nFr, fft_ab, fft_bb, s);
+        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s);
     }
 }
 
@@ -956,12 +939,15 @@ size_t Plan::device_bytes() const {
 // launches as sensa(), so each frame's result equals sensa() on that frame
 // alone.  x [nFr][nu]; y / ve packed per frame at y_off / ve_off (caller
 // point order); residual [nFr][nIter].
-void Plan::sensa_batch(float2* x, const float2* y, const long long* y_off, const float* ve, const long long* ve_off,
-                       const float* ve_max_in, int nIter, int nCGD, double* residual, hipStream_t s) {
+void Plan::sensa_batch(int nA, float2* x, const float2* y, const long long* y_off, const float* ve,
+                       const long long* ve_off, const float* ve_max_in, int nIter, int nCGD, double* residual,
+                       hipStream_t s) {
     MONA_CHECK(nIter >= 0 && nCGD >= 0, MONA_ERR_ARG, "nIter and nCGD must be >= 0");
-    const int nFr = (int)frames.size();
-    MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
-    if (!Gu_b || (int)koff_b.size() != nFr + 1) build_batch(s);
+    MONA_CHECK(!frames.empty(), MONA_ERR_STATE, "the plan has no frames");
+    if (nA <= 0) nA = (int)frames.size();
+    MONA_CHECK(nA <= (int)frames.size(), MONA_ERR_ARG, "more batch frames than the plan has");
+    if (!Gu_b || koff_b.size() != frames.size() + 1) build_batch(s);
+    const int nFr = nA;  // the leading frames solved in this call
     std::vector<int64_t> Pf(nFr), koff(nFr + 1, 0);
     for (int f = 0; f < nFr; f++) {
         Pf[f] = frame(f).Gu->l_size;
@@ -1000,15 +986,15 @@ void Plan::sensa_batch(float2* x, const float2* y, const long long* y_off, const
     auto X = [&](float2* a, int f) { return a + (int64_t)f * nu; };
     for (int it = 0; it < nIter; it++) {
         for (int f = 0; f < nFr; f++) cg_reset(sc + f, s);
-        forward_batch(x, Mp, s);
+        forward_batch(x, Mp, s, nA);
         for (int f = 0; f < nFr; f++) vsub(ys + koff[f], K(Mp, f), K(res, f), koff[f + 1] - koff[f], s);
-        adjoint_batch(res, dag, hy, invHX, s);
+        adjoint_batch(res, dag, hy, invHX, s, nA);
         for (int f = 0; f < nFr; f++) {
             sqnorm(X(dag, f), nu, nullptr, 1, HX, &sc[f].sqn, false, part, s);
             vcopy(X(dag, f), X(p, f), nu, s);
         }
         for (int i = 1; i <= nCGD; i++) {
-            forward_batch(p, Mp, s);
+            forward_batch(p, Mp, s, nA);
             for (int f = 0; f < nFr; f++) {
                 sqnorm(K(Mp, f), koff[f + 1] - koff[f], hy + koff[f] / nCh, nCh, 1.0, &sc[f].den, false, part, s);
                 cg_alpha(sc + f, s);
@@ -1016,14 +1002,14 @@ void Plan::sensa_batch(float2* x, const float2* y, const long long* y_off, const
             }
             if (i == nCGD) break;
             for (int f = 0; f < nFr; f++) vaxpy(K(res, f), K(Mp, f), &sc[f].a, -1.0f, koff[f + 1] - koff[f], s);
-            adjoint_batch(res, dag, hy, invHX, s);
+            adjoint_batch(res, dag, hy, invHX, s, nA);
             for (int f = 0; f < nFr; f++) {
                 sqnorm(X(dag, f), nu, nullptr, 1, HX, &sc[f].sqn_new, false, part, s);
                 cg_beta(sc + f, s);
                 vxpby(X(p, f), X(dag, f), &sc[f].b, nu, s);
             }
         }
-        forward_batch(x, Mp, s);
+        forward_batch(x, Mp, s, nA);
         for (int f = 0; f < nFr; f++) {
             vsub(ys + koff[f], K(Mp, f), K(res, f), koff[f + 1] - koff[f], s);
             sqnorm(K(res, f), koff[f + 1] - koff[f], hy + koff[f] / nCh, nCh, 1.0, hist + (int64_t)f * nIter + it,
@@ -1293,13 +1279,13 @@ int mona_sensa(mona_plan P, int frame, void* x, const void* y, const float* ve, 
     });
 }
 
-int mona_sensa_batch(mona_plan P, void* x, const void* y, const long long* y_off, const float* ve,
+int mona_sensa_batch(mona_plan P, int nFrames, void* x, const void* y, const long long* y_off, const float* ve,
                      const long long* ve_off, const float* ve_max, int nIter, int nCGD, double* residual,
                      hipStream_t stream) {
     return guarded([&] {
         MONA_CHECK(P && x && y && y_off && ve && ve_off, MONA_ERR_ARG, "NULL argument");
-        P->P.sensa_batch(static_cast<float2*>(x), static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, nIter,
-                         nCGD, residual, stream);
+        P->P.sensa_batch(nFrames, static_cast<float2*>(x), static_cast<const float2*>(y), y_off, ve, ve_off, ve_max,
+                         nIter, nCGD, residual, stream);
     });
 }
 

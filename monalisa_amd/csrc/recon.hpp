@@ -95,9 +95,11 @@ struct Plan {
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)
     int fft_lines = 0;                 // register FFT kernel lines per tile (16/32/64; 0 default, -1 off)
+    std::vector<SpView> view_u, view_t;  // leading frames 0..nA-1 of Gu_b / Gut_b
     void build_batch(hipStream_t s);
-    void forward_batch(const float2* x, float2* y, hipStream_t s);
-    void adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s);
+    // over frames 0..nA-1 (nA <= 0: all)
+    void forward_batch(const float2* x, float2* y, hipStream_t s, int nA = 0);
+    void adjoint_batch(const float2* y, float2* x, const float* hy, float scale, hipStream_t s, int nA = 0);
 
     void set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s);
 
@@ -116,8 +118,9 @@ struct Plan {
     void sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
                double* residual, hipStream_t s);
     // all frames at once (independent systems, batched operators)
-    void sensa_batch(float2* x, const float2* y, const long long* y_off, const float* ve, const long long* ve_off,
-                     const float* ve_max, int nIter, int nCGD, double* residual, hipStream_t s);
+    void sensa_batch(int nA, float2* x, const float2* y, const long long* y_off, const float* ve,
+                     const long long* ve_off, const float* ve_max, int nIter, int nCGD, double* residual,
+                     hipStream_t s);
     // device memory held by the plan (its own buffers and frame matrices)
     size_t device_bytes() const;
     // mixed precision: grid-side arrays and k-space in bf16x2 storage

@@ -350,6 +350,24 @@ __global__ void max_row_len(const int64_t* rowptr, int nrows, int* out) {
     atomicMax(out, (int)(rowptr[i + 1] - rowptr[i]));
 }
 
+// -------------------------------------------------------------- stacking
+__global__ void k_add_i32(int* dst, const int* src, int64_t n, int64_t add) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (int)(src[i] + add);
+}
+__global__ void k_add_i64(int64_t* dst, const int64_t* src, int64_t n, int64_t add) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + add;
+}
+__global__ void k_add_entries(SpEntry* dst, const SpEntry* src, int64_t n, int64_t add) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = SpEntry{(int)(src[i].col + add), src[i].val};
+}
+__global__ void k_iota_i32(int* dst, int64_t n, int64_t start) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (int)(start + i);
+}
+
 // -------------------------------------------------------------- decode
 
 static int64_t checked_nnz(const int* r_nJump, int l_nJump) {
@@ -549,53 +567,67 @@ namespace {
 // Packed-entry products with CPT vectors per work-item (CPT | n_vec, v and w
 // 16-byte aligned, operator matrices only).
 template <int CPT, int U, class TV, class TW>
-void spmv_cv_launch(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s) {
+void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
+                    hipStream_t s) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
-    const int64_t total = (int64_t)S.l_nJump * (n_vec / CPT);
+    const int64_t total = (int64_t)V.rows * (n_vec / CPT);
     const SpEntry* cv = S.cv.as<SpEntry>();
-    if (yscale)
-        spmv_cv<true, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, S.l_nJump,
-                                                                  n_vec, v, w, yscale);
-    else
-        spmv_cv<false, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, S.l_nJump,
-                                                                   n_vec, v, w, nullptr);
-    MONA_HIP(hipGetLastError());
-    if (S.n_long > 0) {
+    if (total > 0) {
+        if (yscale)
+            spmv_cv<true, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, V.rows,
+                                                                         n_vec, v, w, yscale);
+        else
+            spmv_cv<false, CPT, U><<<blocks_for(total, 256), 256, 0, s>>>(S.rowptr.as<int64_t>(), cv, ri, V.rows,
+                                                                          n_vec, v, w, nullptr);
+        MONA_HIP(hipGetLastError());
+    }
+    if (V.n_long > 0) {
         S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
-        const int64_t tc = S.n_chunk * (n_vec / CPT);
+        const int64_t tc = V.n_chunk * (n_vec / CPT);
         const int64_t* cb = S.chunk_begin.as<int64_t>();
         float2* part = S.partial.as<float2>();
         if (yscale)
-            spmv_cv_chunks<true, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, cv, n_vec, v, yscale, part);
+            spmv_cv_chunks<true, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, V.n_chunk, cv, n_vec, v, yscale, part);
         else
-            spmv_cv_chunks<false, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, cv, n_vec, v, nullptr, part);
+            spmv_cv_chunks<false, CPT, U><<<blocks_for(tc, 256), 256, 0, s>>>(cb, V.n_chunk, cv, n_vec, v, nullptr, part);
         MONA_HIP(hipGetLastError());
-        const int64_t tl = (int64_t)S.n_long * n_vec;
+        const int64_t tl = (int64_t)V.n_long * n_vec;
         spmv_c2_combine<<<blocks_for(tl, 256), 256, 0, s>>>(S.long_row.as<int>(), S.long_first.as<int>(),
-                                                            S.n_long, ri, n_vec, part, w);
+                                                            V.n_long, ri, n_vec, part, w);
         MONA_HIP(hipGetLastError());
     }
 }
 
 template <class TV, class TW>
-void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s) {
+void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s,
+                   const SpView* view) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
     if (S.cv.p && S.nnz > 0) {
+        SpView V;
+        if (view) {
+            V = *view;
+        } else {
+            V.rows = S.l_nJump;
+            V.l_size = S.l_size;
+            V.n_long = S.n_long;
+            V.n_chunk = S.n_chunk;
+        }
         const bool al = ((uintptr_t)v % 16 == 0) && ((uintptr_t)w % 16 == 0);
         const int cpt = (al && n_vec % 4 == 0) ? 4 : (al && n_vec % 2 == 0) ? 2 : 1;
         if (S.squeezed || S.l_nJump != S.l_size)
-            fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
-        if (S.l_nJump == 0) return;
+            fill_f32(reinterpret_cast<float*>(w), 0.0f, V.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
+        if (V.rows == 0) return;
         // measured on 3D grids at 16 and 20 coils: the gridding gather
         // (~27 entries a row) runs best with two vectors per work-item and
         // 16 gathers in flight; the weighted spreading product (rows of ~8
         // entries) with four vectors and 4 entries a batch
-        if (!yscale && cpt >= 2) spmv_cv_launch<2, 16>(S, v, w, n_vec, yscale, s);
-        else if (cpt == 4) spmv_cv_launch<4, 4>(S, v, w, n_vec, yscale, s);
-        else if (cpt == 2) spmv_cv_launch<2, 8>(S, v, w, n_vec, yscale, s);
-        else spmv_cv_launch<1, 8>(S, v, w, n_vec, yscale, s);
+        if (!yscale && cpt >= 2) spmv_cv_launch<2, 16>(S, V, v, w, n_vec, yscale, s);
+        else if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s);
+        else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s);
+        else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s);
         return;
     }
+    MONA_CHECK(!view, MONA_ERR_INTERNAL, "matrix views need the packed entries");
     if (S.squeezed || S.l_nJump != S.l_size)
         fill_f32(reinterpret_cast<float*>(w), 0.0f, (int64_t)S.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
     int64_t total = (int64_t)S.l_nJump * n_vec;
@@ -632,20 +664,97 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
 }  // namespace
 
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s) {
-    spmv_c2_typed(S, v, w, n_vec, yscale, s);
+                      const float* yscale, hipStream_t s, const SpView* view) {
+    spmv_c2_typed(S, v, w, n_vec, yscale, s, view);
 }
 
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
-                         const float* yscale, hipStream_t s) {
+                         const float* yscale, hipStream_t s, const SpView* view) {
     if (v_bf16 && w_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view);
     else if (v_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view);
     else if (w_bf16)
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view);
     else
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view);
+}
+
+void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& parts,
+                       const std::vector<int64_t>& col_off, const std::vector<int64_t>& row_off, int r_size,
+                       int l_size, std::vector<SpView>* views, hipStream_t s) {
+    const size_t n = parts.size();
+    int64_t nnz = 0, rows = 0, nch = 0;
+    int nl = 0, mx = 0;
+    bool natural = true;  // output rows = stored rows, in order
+    for (size_t f = 0; f < n; f++) {
+        const SparseDev& P = *parts[f];
+        MONA_CHECK(P.on_device && (P.cv.p || P.nnz == 0), MONA_ERR_INTERNAL, "stacking needs packed device parts");
+        natural = natural && !P.squeezed && P.l_nJump == P.l_size && row_off[f] == rows;
+        nnz += P.nnz;
+        rows += P.l_nJump;
+        nl += P.n_long;
+        nch += P.n_chunk;
+        mx = std::max(mx, P.max_row);
+    }
+    MONA_CHECK(rows < 0x7fffffff && nnz < ((int64_t)1 << 40), MONA_ERR_ARG, "stacked matrix too large");
+    out.r_size = r_size;
+    out.l_size = l_size;
+    out.l_nJump = (int)rows;
+    out.nnz = nnz;
+    out.squeezed = !natural;
+    out.max_row = mx;
+    out.n_long = nl;
+    out.n_chunk = nch;
+    out.col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int));
+    out.val.alloc(std::max<int64_t>(nnz, 1) * sizeof(float));
+    out.cv.alloc(std::max<int64_t>(nnz, 1) * sizeof(SpEntry));
+    out.rowptr.alloc((size_t)(rows + 1) * sizeof(int64_t));
+    if (out.squeezed) out.rowidx.alloc(std::max<int64_t>(rows, 1) * sizeof(int));
+    if (nl > 0) {
+        out.long_row.alloc((size_t)nl * sizeof(int));
+        out.long_first.alloc((size_t)(nl + 1) * sizeof(int));
+        out.chunk_begin.alloc((size_t)nch * 2 * sizeof(int64_t));
+    }
+    if (views) views->assign(n + 1, SpView{});
+    int64_t eo = 0, ro = 0, co = 0;
+    int lo = 0;
+    auto nb = [](int64_t k) { return blocks_for(std::max<int64_t>(k, 1), 256); };
+    for (size_t f = 0; f < n; f++) {
+        const SparseDev& P = *parts[f];
+        if (views) (*views)[f] = SpView{(int)ro, row_off[f], lo, co};
+        if (P.nnz > 0) {
+            k_add_i32<<<nb(P.nnz), 256, 0, s>>>(out.col.as<int>() + eo, P.col.as<int>(), P.nnz, col_off[f]);
+            k_add_entries<<<nb(P.nnz), 256, 0, s>>>(out.cv.as<SpEntry>() + eo, P.cv.as<SpEntry>(), P.nnz,
+                                                    col_off[f]);
+            MONA_HIP(hipMemcpyAsync(out.val.as<float>() + eo, P.val.p, P.nnz * sizeof(float),
+                                    hipMemcpyDeviceToDevice, s));
+        }
+        // l_nJump + 1 pointers: the last equals the next part's first
+        k_add_i64<<<nb(P.l_nJump + 1), 256, 0, s>>>(out.rowptr.as<int64_t>() + ro, P.rowptr.as<int64_t>(),
+                                                    P.l_nJump + 1, eo);
+        if (out.squeezed && P.l_nJump > 0) {
+            if (P.squeezed)
+                k_add_i32<<<nb(P.l_nJump), 256, 0, s>>>(out.rowidx.as<int>() + ro, P.rowidx.as<int>(), P.l_nJump,
+                                                        row_off[f]);
+            else
+                k_iota_i32<<<nb(P.l_nJump), 256, 0, s>>>(out.rowidx.as<int>() + ro, P.l_nJump, row_off[f]);
+        }
+        if (P.n_long > 0) {
+            k_add_i32<<<nb(P.n_long), 256, 0, s>>>(out.long_row.as<int>() + lo, P.long_row.as<int>(), P.n_long, ro);
+            k_add_i32<<<nb(P.n_long + 1), 256, 0, s>>>(out.long_first.as<int>() + lo, P.long_first.as<int>(),
+                                                       P.n_long + 1, co);
+            k_add_i64<<<nb(2 * P.n_chunk), 256, 0, s>>>(out.chunk_begin.as<int64_t>() + 2 * co,
+                                                        P.chunk_begin.as<int64_t>(), 2 * P.n_chunk, eo);
+        }
+        MONA_HIP(hipGetLastError());
+        eo += P.nnz;
+        ro += P.l_nJump;
+        lo += P.n_long;
+        co += P.n_chunk;
+    }
+    if (views) (*views)[n] = SpView{(int)ro, (int64_t)l_size, lo, co};
+    out.on_device = true;
 }
 
 // -------------------------------------------------------------- host-pointer products

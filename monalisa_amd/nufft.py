@@ -137,10 +137,10 @@ class Plan:
         return list(res)[:nIter]
 
     def bmSensa_batch(self, x, y_list, ve_list, nCGD, ve_max=None, nIter=1, stream=None):
-        """bmSensa on every frame at once (batched operators, per-frame CG):
-        x [nFr, nu] in place; y_list / ve_list per frame; returns the
-        residuals [nFr][nIter].  Each frame's result equals bmSensa on that
-        frame alone."""
+        """bmSensa on frames 0 .. nFr-1 at once (batched operators, per-frame
+        CG), nFr = x.shape[0] <= the plan's frames: x [nFr, nu] in place;
+        y_list / ve_list per frame; returns the residuals [nFr][nIter].  Each
+        frame's result equals bmSensa on that frame alone."""
         import torch
         _require_cuda(x, "x")
         nFr = x.shape[0]
@@ -155,7 +155,7 @@ class Plan:
             vm = (C.c_float * nFr)(*[float(v) for v in np.broadcast_to(np.asarray(ve_max, dtype=np.float64), (nFr,))])
         res = (C.c_double * (max(nIter, 1) * nFr))()
         ycat, vcat = ycat.contiguous(), vcat.contiguous()   # held until the call returns
-        check(lib.mona_sensa_batch(self._h, ptr(x), ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)),
+        check(lib.mona_sensa_batch(self._h, int(nFr), ptr(x), ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)),
                                    voff, vm, int(nIter), int(nCGD), res, stream_of(stream)))
         return np.array(list(res)[:nIter * nFr]).reshape(nFr, nIter)
 

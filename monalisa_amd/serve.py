@@ -55,10 +55,12 @@ class Request:
 class ReconServer:
     """Continuous-batching CG-SENSE server.
 
-    backend: object with ``plan_bytes_estimate(protocol, B)``,
-    ``request_bytes(protocol)``, ``make_plan(protocol, B)``,
-    ``plan_bytes(plan)``, ``run(plan, protocol, requests, stream)``,
-    ``new_stream()``, ``release(plan)``.
+    backend: object with ``batch_capacity(protocol)`` (largest batch worth
+    forming for it), ``plan_bytes_estimate(protocol, capacity)``,
+    ``request_bytes(protocol)``, ``make_plan(protocol, capacity, stream)``,
+    ``plan_bytes(plan)``, ``run(plan, protocol, requests, stream)`` (any
+    batch up to the plan's capacity), ``new_stream()``, ``release(plan)``.
+    Plans are cached per protocol, built for its batch capacity.
     """
 
     def __init__(self, backend, hbm_budget: int, max_batch: int = 4, workers: int = 3):
@@ -69,7 +71,7 @@ class ReconServer:
         self.protocols: Dict[str, Any] = {}
         self._queues: Dict[str, collections.deque] = collections.OrderedDict()
         self._cv = threading.Condition()
-        self._idle_plans: Dict[tuple, List[Any]] = collections.defaultdict(list)
+        self._idle_plans: Dict[str, List[Any]] = collections.defaultdict(list)
         self._plan_bytes: Dict[int, int] = {}
         self._plan_last_use: Dict[int, float] = {}
         self._busy_bytes = 0          # plans in use + request buffers in flight
@@ -153,17 +155,23 @@ class ReconServer:
         return True
 
     # -- dispatcher -------------------------------------------------------
+    def capacity(self, key: str) -> int:
+        return max(1, min(self.max_batch, int(self.backend.batch_capacity(self.protocols[key]))))
+
     def _next_batch(self):
         """Oldest waiting request first; its protocol's waiting requests
-        (up to max_batch) form the batch, shrunk until it fits the budget."""
+        (up to the protocol's batch capacity) form the batch.  It starts
+        when a plan of the protocol is idle or a new one fits the budget
+        (idle plans of other protocols evicted oldest-first)."""
         heads = [(q[0].t_submit, k) for k, q in self._queues.items() if q]
         if not heads:
             return None
         _, key = min(heads)
         q = self._queues[key]
         proto = self.protocols[key]
-        alone = self.backend.plan_bytes_estimate(proto, 1) + self.backend.request_bytes(proto)
-        if alone > self.budget and not self._idle_plans[(key, 1)]:
+        cap = self.capacity(key)
+        alone = self.backend.plan_bytes_estimate(proto, cap) + self.backend.request_bytes(proto)
+        if alone > self.budget and not self._idle_plans[key]:
             # can never be admitted: fail it instead of blocking the queue
             r = q.popleft()
             r.error = MemoryError(f"protocol {key!r} needs {alone} bytes, budget {self.budget}")
@@ -171,18 +179,12 @@ class ReconServer:
             self.completed.append(r)
             r.done.set()
             return None
-        B = min(self.max_batch, len(q))
-        while B >= 1:
-            plan = self._idle_plans[(key, B)][-1] if self._idle_plans[(key, B)] else None
-            plan_need = 0 if plan is not None else self.backend.plan_bytes_estimate(proto, B)
-            need = plan_need + B * self.backend.request_bytes(proto)
-            if plan is not None:
-                # the idle plan moves to busy: its bytes are already resident
-                if self._evict_for(need, keep=plan):
-                    return key, B
-            elif self._evict_for(need):
-                return key, B
-            B -= 1
+        B = min(cap, len(q))
+        plan = self._idle_plans[key][-1] if self._idle_plans[key] else None
+        need = (0 if plan is not None else self.backend.plan_bytes_estimate(proto, cap)) + \
+            B * self.backend.request_bytes(proto)
+        if self._evict_for(need, keep=plan):
+            return key, B
         return None
 
     def _dispatch(self):
@@ -195,11 +197,11 @@ class ReconServer:
                         key, B = nb
                         proto = self.protocols[key]
                         reqs = [self._queues[key].popleft() for _ in range(B)]
-                        plan = self._idle_plans[(key, B)].pop() if self._idle_plans[(key, B)] else None
+                        plan = self._idle_plans[key].pop() if self._idle_plans[key] else None
                         if plan is not None:
                             pb = self._plan_bytes[id(plan)]
                         else:
-                            pb = self.backend.plan_bytes_estimate(proto, B)
+                            pb = self.backend.plan_bytes_estimate(proto, self.capacity(key))
                         rb = B * self.backend.request_bytes(proto)
                         self._busy_bytes += pb + rb
                         self.peak_bytes = max(self.peak_bytes, self.resident_bytes())
@@ -229,7 +231,7 @@ class ReconServer:
             err = None
             try:
                 if plan is None:
-                    plan = self.backend.make_plan(proto, B, stream)
+                    plan = self.backend.make_plan(proto, self.capacity(key), stream)
                 self.backend.run(plan, proto, reqs, stream)
             except BaseException as e:  # reported on the requests
                 err = e
@@ -240,7 +242,7 @@ class ReconServer:
                     real = self.backend.plan_bytes(plan)
                     self._plan_bytes[id(plan)] = real
                     self._plan_last_use[id(plan)] = t1
-                    self._idle_plans[(key, B)].append(plan)
+                    self._idle_plans[key].append(plan)
                     if self.resident_bytes() > self.budget:
                         self._evict_for(0)
                 elif plan is not None:
@@ -285,7 +287,20 @@ def make_protocol(t, ve, C, N_u, n_u, dK_u, kernel="kaiser") -> Protocol:
 
 
 class GpuBackend:
-    """libmona execution of the batches (Plan.bmSensa_batch)."""
+    """libmona execution of the batches (Plan.bmSensa_batch).
+
+    work_per_batch: oversampled-grid elements x coils a batch should reach;
+    a protocol whose single request already exceeds it is not batched (its
+    gridding and FFT launches fill the GPU alone, and batching would only
+    lengthen the jobs others wait behind)."""
+
+    def __init__(self, work_per_batch: float = 1.5e8, max_capacity: int = 8):
+        self.work_per_batch = work_per_batch
+        self.max_capacity = max_capacity
+
+    def batch_capacity(self, proto: "Protocol") -> int:
+        work = float(np.prod(proto.N_u)) * proto.nCh
+        return int(max(1, min(self.max_capacity, self.work_per_batch // work)))
 
     def new_stream(self):
         import torch

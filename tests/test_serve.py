@@ -21,6 +21,9 @@ class FakeBackend:
         self.concurrent = 0
         self.max_concurrent = 0
 
+    def batch_capacity(self, proto):
+        return proto.get("cap", 99)
+
     def plan_bytes_estimate(self, proto, B):
         return proto["plan"] * B
 
@@ -75,6 +78,17 @@ def test_every_request_completes_once_with_its_own_result():
         assert r.done.is_set() and r.error is None
         np.testing.assert_array_equal(r.x, np.full(3, 2 * i))
         assert r.t_submit <= r.t_start <= r.t_done
+
+
+def test_batch_capacity_per_protocol():
+    s, be = _server(budget=100_000, max_batch=4, workers=1)
+    s.register("solo", {"plan": 100, "req": 1, "t": 0.005, "cap": 1})
+    reqs = [s.submit("solo", np.zeros(1)) for _ in range(3)]
+    s.start()
+    assert s.drain(timeout=30)
+    s.stop()
+    assert [b for _, b in s.batches] == [1, 1, 1]
+    assert s.capacity("solo") == 1 and s.capacity("small") == 4
 
 
 def test_requests_of_one_protocol_are_batched_and_batches_overlap():
