@@ -406,6 +406,19 @@ def bench_serving(args, cfg, quiet=False):
     return out
 
 
+def measured_traffic(config):
+    """HBM bytes per launch of the adjoint gridding op (n2u) from the
+    committed rocprofv3 counter summary of this workload: FETCH_SIZE (scaled
+    by the factor a plain copy of known size gives) + WRITE_SIZE."""
+    path = os.path.join(ROOT, "profiles", f"{config}_n2u_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {"bytes": d["fetch_bytes"] + d["write_bytes"],
+            "source": f"profiles/{config}_n2u_traffic.json (rocprofv3 FETCH_SIZE x {d['fetch_factor']} + WRITE_SIZE)"}
+
+
 def gridding_bytes(prob):
     """Algorithmic HBM bytes per launch of the two gridding products
     (jump-CSR form; complex64 = 8 B, int32 col + float val = 8 B per entry,
@@ -491,6 +504,9 @@ def main():
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--pairs", type=int, default=10, help="NUFFT pairs timed after the recons")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-latency leg of the default line")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="start with a 1 GiB device copy (a known-size kernel for counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -500,6 +516,11 @@ def main():
     import torch
     from monalisa_amd import nufft
     cfg = CONFIGS[args.config]
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.complex64, device="cuda")
+        b = a.clone()
+        torch.cuda.synchronize()
+        del a, b
     if "nBin" in cfg:
         bench_binned(args, cfg)
         return
@@ -590,6 +611,10 @@ def main():
                      "touched_cells_U": U},
         "cpu_baseline": None,
     }
+    tr = measured_traffic(args.config)
+    if tr is not None:
+        out["roofline"]["traffic"] = tr["bytes"]
+        out["roofline"]["traffic_source"] = tr["source"]
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         try:
@@ -598,6 +623,16 @@ def main():
             out["speedup_vs_cpu"] = out["value"] / cb["value"]
         except Exception as e:  # reported, never silently replaced
             out["cpu_baseline"] = {"error": repr(e)}
+    if not args.no_serve and args.config == "c1":
+        # served latency over the 2D / 3D protocol mix (a shorter run of
+        # the recon-serving benchmark)
+        del prob, P, y, ve, x, yy
+        torch.cuda.synchronize()
+        sv = bench_serving(args, dict(CONFIGS["c4"], n_sat=48, n_poisson=120), quiet=True)
+        out["served"] = {"recons_per_s": sv["value"], "plain_per_request_recons_per_s":
+                         sv["plain_per_request_recons_per_s"], "latency_ms": sv["latency_ms"],
+                         "offered_load_req_per_s": sv["offered_load_req_per_s"],
+                         "workload": sv["config"]["workload"] + ", 10-step CG-SENSE each"}
     print(json.dumps(out), flush=True)
 
 

@@ -148,6 +148,7 @@ struct SparseDev {
     int r_size = 0, l_size = 0, l_nJump = 0;
     int64_t nnz = 0;
     bool squeezed = false;
+    bool covers_all = false;  // every output row is a stored row (the product writes all of w)
     int max_row = 0;          // longest stored row
     DevBuf col;               // int32[nnz]   absolute 0-based column
     DevBuf val;               // float[nnz]

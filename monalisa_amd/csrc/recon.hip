@@ -506,19 +506,47 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
             tcol[k] = inv[c];
         }
     }
-    std::vector<int> rows;
-    const std::vector<int>* rp = nullptr;
-    if (Gut->squeezed) {
-        rows.resize(Gut->l_nJump);
+    // Gut with one stored row per grid cell (empty cells included), cells
+    // in 8^3-tile order (raster order inside a tile): neighbouring rows
+    // gather the same points, and the product writes every grid value, so
+    // no separate zero fill is needed.  Each row keeps its entries in order.
+    const int64_t G = Gut->l_size;
+    MONA_CHECK(G == Nu, MONA_ERR_ARG, "Gut rows must be the grid cells");
+    std::vector<int64_t> cell_row(G, -1), trp(Gut->l_nJump + 1, 0);
+    {
         int64_t r = 0;
         for (int i = 0; i < Gut->l_nJump; i++) {
-            r += Gut->h_l_jump[i];
-            rows[i] = (int)r;
+            r = Gut->squeezed ? r + Gut->h_l_jump[i] : i;
+            MONA_CHECK(r >= 0 && r < G, MONA_ERR_ARG, "Gut row outside the grid");
+            cell_row[r] = i;
+            trp[i + 1] = trp[i] + Gut->h_r_nJump[i];
         }
-        rp = &rows;
+    }
+    std::vector<int> dcol(Gut->nnz), dnj(G), drow(G);
+    std::vector<float> dval(Gut->nnz);
+    {
+        int64_t o = 0, q = 0;
+        for (int z0 = 0; z0 < N[2]; z0 += 8)
+            for (int y0 = 0; y0 < N[1]; y0 += 8)
+                for (int x0 = 0; x0 < N[0]; x0 += 8)
+                    for (int z = z0; z < std::min(z0 + 8, N[2]); z++)
+                        for (int y = y0; y < std::min(y0 + 8, N[1]); y++)
+                            for (int x = x0; x < std::min(x0 + 8, N[0]); x++) {
+                                const int64_t cell = ((int64_t)z * N[1] + y) * N[0] + x;
+                                const int64_t r = cell_row[cell];
+                                drow[q] = (int)cell;
+                                dnj[q] = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
+                                if (r >= 0)
+                                    for (int64_t k = trp[r]; k < trp[r + 1]; k++, o++) {
+                                        dcol[o] = tcol[k];
+                                        dval[o] = Gut->h_m_val[k];
+                                    }
+                                q++;
+                            }
     }
     auto Gut_s = std::make_shared<SparseDev>();
-    sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, tcol, Gut->h_m_val, Gut->h_r_nJump, rp, s);
+    sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
+    Gut_s->covers_all = true;
     auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
     MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipStreamSynchronize(s));
