@@ -54,6 +54,10 @@ struct PassArgs {
     int64_t kfc_period;  // KFC index taken modulo this many pixels (frames share the maps)
     float2* img_out;     // epilogue (combine): [outer][out_cnt]
     int xs;              // register kernel: LDS stride between the N1 exchange planes
+    const uint32_t* in_mask;   // register kernel, mode 0: cell masks (see AxisPass)
+    const uint32_t* out_mask;
+    int mask_nch;
+    int64_t mask_cpl;          // mask columns per line block: S / mask_nch
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -458,13 +462,55 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const bool live = o < a.outer && ws < a.S;
     const float sg = a.inverse ? 1.0f : -1.0f;
 
+    // masked passes: the bits of this work-item's column, in registers
+    constexpr int MW = (N + 31) / 32;
+    uint32_t mw[MW];
+    const uint32_t* mask = MODE == 0 ? (a.in_mask ? a.in_mask : a.out_mask) : nullptr;
+    if (mask && live) {
+        // a column's words are MW * 4 bytes at a multiple of that: vector loads
+        const uint32_t* mp = mask + (o * a.mask_cpl + ws / a.mask_nch) * MW;
+        if constexpr (MW % 4 == 0) {
+#pragma unroll
+            for (int i = 0; i < MW / 4; i++) {
+                const uint4 q = reinterpret_cast<const uint4*>(mp)[i];
+                mw[4 * i] = q.x;
+                mw[4 * i + 1] = q.y;
+                mw[4 * i + 2] = q.z;
+                mw[4 * i + 3] = q.w;
+            }
+        } else if constexpr (MW % 2 == 0) {
+#pragma unroll
+            for (int i = 0; i < MW / 2; i++) {
+                const uint2 q = reinterpret_cast<const uint2*>(mp)[i];
+                mw[2 * i] = q.x;
+                mw[2 * i + 1] = q.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MW; i++) mw[i] = mp[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MW; i++) mw[i] = ~0u;
+    }
+    auto bit = [&](int row) {  // row < N; word picked by selects (no indexed registers)
+        uint32_t wv = mw[0];
+#pragma unroll
+        for (int i = 1; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
+        return (wv >> (row & 31)) & 1u;
+    };
     float2 v[N1];
     if (j < N2) {
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) {
             const int r = j + N2 * n1 - a.in_off;
             float2 x = make_float2(0.f, 0.f);
-            if (live && r >= 0 && r < a.in_cnt) {
+            const bool take = live && r >= 0 && r < a.in_cnt;
+            if (MODE == 0 && a.in_mask) {
+                // masked-off rows (not written by the producer) are zero and
+                // not loaded
+                if (take && bit(j + N2 * n1)) x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
+            } else if (take) {
                 if (MODE == 1) {
                     const int64_t ii = o * a.in_cnt + r;
                     x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
@@ -540,7 +586,9 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     for (int k2 = 0; k2 < N2; k2++) {
         const int k = j + N1 * k2;
         const int kl = k - a.out_off;
-        if (kl >= 0 && kl < a.out_cnt) {
+        bool put = kl >= 0 && kl < a.out_cnt;
+        if (MODE == 0 && a.out_mask) put = put && bit(k);
+        if (put) {
             float sc = a.scale;
             if (a.postsign && ((k + half) & 1)) sc = -sc;
             const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
@@ -682,6 +730,11 @@ bool fft_size_supported(int N) {
 
 bool fft_reg_supported(int N) { return reg_n2(N) != 0; }
 
+bool fft_plain_pass_is_reg(int N, int lines) {
+    const int l = lines != 0 ? lines : reg_lines_default();
+    return reg_n2(N) != 0 && l > 0;
+}
+
 bool fft_pass_fusable(int N, int64_t S) { return (int64_t)N * S <= kFusedTile || (reg_n2(N) && S <= 64); }
 
 namespace {
@@ -726,6 +779,12 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     a.kfc = p.kfc;
     a.kfc_period = p.kfc_period > 0 ? p.kfc_period : INT64_MAX;
     a.img_out = p.img_out;
+    a.in_mask = p.in_mask;
+    a.out_mask = p.out_mask;
+    a.mask_nch = p.mask_nch;
+    a.mask_cpl = p.S / std::max(p.mask_nch, 1);
+    MONA_CHECK(!(p.in_mask || p.out_mask) || (p.mode == 0 && p.mask_nch >= 1 && p.S % p.mask_nch == 0),
+               MONA_ERR_INTERNAL, "cell masks need a plain pass and S divisible by mask_nch");
     const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     const int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
@@ -776,6 +835,7 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
         fft_pass_reg(p, n2, lines > 0 ? lines : 16, s);
         return;
     }
+    MONA_CHECK(!(p.in_mask || p.out_mask), MONA_ERR_INTERNAL, "cell masks need the register FFT kernel");
     MONA_CHECK(p.mode == 0 || old_fits, MONA_ERR_INTERNAL, "fused FFT pass does not fit a tile");
     PassArgs a{};
     a.N = N;

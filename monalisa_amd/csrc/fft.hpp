@@ -9,6 +9,9 @@ bool fft_size_supported(int N);
 // True when N = 16 * N2 with N2 in {2,3,4,5,6,8,10,12,16}: passes of this
 // length run register-resident (and support bf16x2 storage).
 bool fft_reg_supported(int N);
+// True when a plain pass of length N with this lines setting (AxisPass::lines)
+// runs on the register kernel (so it can take cell masks).
+bool fft_plain_pass_is_reg(int N, int lines);
 
 // One pass along an axis of length N over an [outer][rows][S] array (S
 // contiguous).  Input rows in_off .. in_off+in_cnt-1 of the length-N line
@@ -40,6 +43,15 @@ struct AxisPass {
                              // (16 / 32 / 64), 0 = default, -1 = LDS Stockham kernel only
     float2* img_out = nullptr;
     bool bf_in = false, bf_out = false;  // in / out stored as bf16x2 (register kernel only)
+    // Cell masks (register kernel, mode 0), one bit per (line block, row):
+    // element (o, row, w) belongs to column c = o * (S / mask_nch) + w /
+    // mask_nch, whose bits are the ceil(N / 32) words at c * ceil(N / 32)
+    // (bit row).  An input element whose in_mask bit is 0 is read as 0
+    // without a load; an output element whose out_mask bit is 0 is not
+    // stored.
+    const uint32_t* in_mask = nullptr;
+    const uint32_t* out_mask = nullptr;
+    int mask_nch = 1;
 };
 
 void fft_pass(const AxisPass& p, hipStream_t s);

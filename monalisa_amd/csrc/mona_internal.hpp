@@ -204,11 +204,14 @@ struct SpView {
     int n_long = 0;
     int64_t n_chunk = 0;
 };
+// zero_rest = false leaves rows of w that S does not store untouched (for a
+// consumer that never reads them).
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s, const SpView* view = nullptr);
+                      const float* yscale, hipStream_t s, const SpView* view = nullptr, bool zero_rest = true);
 // The same product with v and / or w stored as bf16x2 (sums in float).
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
-                         const float* yscale, hipStream_t s, const SpView* view = nullptr);
+                         const float* yscale, hipStream_t s, const SpView* view = nullptr,
+                         bool zero_rest = true);
 // Block-diagonal stacking on the device: part f's columns are shifted by
 // col_off[f] and its output rows by row_off[f] (out has r_size columns and
 // l_size output rows; its stored rows are the parts' in order).  views[f]

@@ -9,6 +9,7 @@
 // never waits on the host.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -506,10 +507,14 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
             tcol[k] = inv[c];
         }
     }
-    // Gut with one stored row per grid cell (empty cells included), cells
-    // in 8^3-tile order (raster order inside a tile): neighbouring rows
-    // gather the same points, and the product writes every grid value, so
-    // no separate zero fill is needed.  Each row keeps its entries in order.
+    // Gut with its rows in 8^3-tile order (raster order inside a tile):
+    // neighbouring rows gather the same points; each row keeps its entries
+    // in order.  A frame that touches a large part of the grid stores one
+    // row per cell (empty ones too): the product then writes the whole grid
+    // and needs no zero fill.  A sparse frame stores only its touched cells
+    // and the FFT passes along the last axis skip the others through cell
+    // masks (skipping cells of a densely touched grid leaves partly
+    // written cache lines all over it, which costs more than it saves).
     const int64_t G = Gut->l_size;
     MONA_CHECK(G == Nu, MONA_ERR_ARG, "Gut rows must be the grid cells");
     std::vector<int64_t> cell_row(G, -1), trp(Gut->l_nJump + 1, 0);
@@ -522,6 +527,9 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
             trp[i + 1] = trp[i] + Gut->h_r_nJump[i];
         }
     }
+    int64_t touched = 0;
+    for (int64_t c = 0; c < G; c++) touched += cell_row[c] >= 0 && trp[cell_row[c] + 1] > trp[cell_row[c]];
+    const bool sparse_frame = touched < (G * 3) / 10;
     std::vector<int> dcol(Gut->nnz), dnj(G), drow(G);
     std::vector<float> dval(Gut->nnz);
     {
@@ -534,19 +542,44 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
                             for (int x = x0; x < std::min(x0 + 8, N[0]); x++) {
                                 const int64_t cell = ((int64_t)z * N[1] + y) * N[0] + x;
                                 const int64_t r = cell_row[cell];
+                                const int n = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
+                                if (n == 0 && sparse_frame) continue;
                                 drow[q] = (int)cell;
-                                dnj[q] = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
-                                if (r >= 0)
-                                    for (int64_t k = trp[r]; k < trp[r + 1]; k++, o++) {
-                                        dcol[o] = tcol[k];
-                                        dval[o] = Gut->h_m_val[k];
-                                    }
+                                dnj[q] = n;
+                                for (int64_t k = n ? trp[r] : 0; k < (n ? trp[r + 1] : 0); k++, o++) {
+                                    dcol[o] = tcol[k];
+                                    dval[o] = Gut->h_m_val[k];
+                                }
                                 q++;
                             }
+        dnj.resize(q);
+        drow.resize(q);
     }
     auto Gut_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
-    Gut_s->covers_all = true;
+    Gut_s->covers_all = !sparse_frame;
+    // cell masks: cells the forward gridding reads (Gu columns) and cells the
+    // adjoint gridding writes (Gut rows with entries); equal for a
+    // transposed pair.  Layout of the pass along the last axis: for each
+    // column (all other coordinates, x fastest) ceil(N_last / 32) words,
+    // bit z of the column's cell at last-axis coordinate z.
+    const int64_t NL = N[imDim - 1], cols = G / NL, mwords = (NL + 31) / 32;
+    const int64_t nw = cols * mwords;
+    std::vector<uint32_t> fmask(nw, 0u), amask(nw, 0u);
+    auto set_bit = [&](std::vector<uint32_t>& m, int64_t cell) {
+        const int64_t z = cell / cols, c = cell - z * cols;
+        m[c * mwords + (z >> 5)] |= 1u << (z & 31);
+    };
+    for (int64_t k = 0; k < Gu->nnz; k++) set_bit(fmask, col[k]);
+    for (int64_t c = 0; c < G; c++)
+        if (cell_row[c] >= 0 && trp[cell_row[c] + 1] > trp[cell_row[c]]) set_bit(amask, c);
+    auto dfm = std::make_shared<DevBuf>((size_t)nw * sizeof(uint32_t));
+    MONA_HIP(hipMemcpyAsync(dfm->p, fmask.data(), (size_t)nw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    std::shared_ptr<DevBuf> dam = dfm;
+    if (amask != fmask) {
+        dam = std::make_shared<DevBuf>((size_t)nw * sizeof(uint32_t));
+        MONA_HIP(hipMemcpyAsync(dam->p, amask.data(), (size_t)nw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    }
     auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
     MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipStreamSynchronize(s));
@@ -558,9 +591,27 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     fr.Gut_s = Gut_s;
     fr.perm = dperm;
     fr.P = P;
+    fr.fmask = dfm;
+    fr.amask = dam;
+    fr.masked = sparse_frame;
     Gu_b.reset();  // the batched matrices are rebuilt on next use
     Gut_b.reset();
     koff_b.clear();
+}
+
+// The pruned operators confine the grid to the frame's cells: the forward
+// pass along the last axis stores only the cells the gridding reads, the
+// adjoint pass loads only the cells the adjoint gridding writes (the others
+// are read as zero), so the untouched cells are never written, filled or
+// read.  Needs the register kernel on the last axis.
+bool Plan::masked_grid(int f) const {
+    const bool frames_sparse = f >= 0 ? frame(f).masked : [&] {
+        for (const Frame& fr : frames)
+            if (!fr.masked) return false;
+        return !frames.empty();
+    }();
+    return frames_sparse && imDim >= 2 && fft_pass_fusable(N[0], nCh) &&
+           fft_plain_pass_is_reg(N[imDim - 1], fft_lines);
 }
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
@@ -572,7 +623,8 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s);
+            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s, false,
+                               masked_grid(f) ? fr.fmask->as<uint32_t>() : nullptr);
         } else {
             Geom g = geom();
             prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
@@ -608,7 +660,7 @@ void Plan::ensure_fft_scratch() {
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
 void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s,
-                              bool bf) {
+                              bool bf, const uint32_t* out_mask) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     int64_t S = nCh;
@@ -633,6 +685,10 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
         p.inverse = false;
         p.bf_in = bf && a > 0;
         p.bf_out = bf;
+        if (a == imDim - 1 && out_mask) {
+            p.out_mask = out_mask;
+            p.mask_nch = nCh;
+        }
         if (a == 0) {
             p.mode = 1;
             p.img = x;
@@ -648,7 +704,7 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
 // each pass keeping only the central n_a rows; the pass along x ends with
 // .* KFC_conj (and the coil sum when combine).
 void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, int nb, DevBuf& ba,
-                              DevBuf& bb, hipStream_t s, bool bf) {
+                              DevBuf& bb, hipStream_t s, bool bf, const uint32_t* in_mask) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     for (int a = imDim - 1; a >= 0; a--) {
@@ -673,6 +729,10 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         p.scale = 1.0f / (float)N[a];
         p.bf_in = bf;
         p.bf_out = bf && a > 0;
+        if (a == imDim - 1 && in_mask) {
+            p.in_mask = in_mask;
+            p.mask_nch = nCh;
+        }
         if (a == 0) {
             p.mode = combine ? 2 : 3;
             p.kfc = KFCc.as<float2>();
@@ -694,15 +754,17 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     ensure_grid();
     Geom g = geom();
     float2* gr = grid.as<float2>();
+    const bool masked = masked_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s);
+        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s, nullptr, !masked);
     }
     {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s);
+            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s, false,
+                               masked ? fr.amask->as<uint32_t>() : nullptr);
         } else {
             cfft_nd(gr, imDim, N, nCh, true, s);
             epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
@@ -762,6 +824,21 @@ void Plan::build_batch(hipStream_t s) {
         poff += fr.P;
         koff_b[f + 1] = koff_b[f] + fr.P * nCh;
     }
+    // stacked cell masks of the frames (frame f's columns after those of
+    // frames 0..f-1)
+    mask_fb.release();
+    mask_ab.release();
+    {
+        const size_t wb = frame(0).fmask->bytes;
+        mask_fb.alloc(wb * nFr);
+        mask_ab.alloc(wb * nFr);
+        for (int f = 0; f < nFr; f++) {
+            MONA_HIP(hipMemcpyAsync(static_cast<char*>(mask_fb.p) + f * wb, frame(f).fmask->p, wb,
+                                    hipMemcpyDeviceToDevice, s));
+            MONA_HIP(hipMemcpyAsync(static_cast<char*>(mask_ab.p) + f * wb, frame(f).amask->p, wb,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+    }
     auto gu = std::make_shared<SparseDev>();
     sparse_block_diag(*gu, us, ucol, urow, (int)(nFr * Nu), (int)poff, &view_u, s);
     auto gut = std::make_shared<SparseDev>();
@@ -781,7 +858,8 @@ void Plan::forward_batch(const float2* x, float2* y, hipStream_t s, int nA) {
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s);
+        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s, false,
+                           masked_grid(-1) && mask_fb.p ? mask_fb.as<uint32_t>() : nullptr);
     }
     {
         TimedRegion t(0, s);
@@ -798,13 +876,15 @@ void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scal
     grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
     fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
+    const bool masked = masked_grid(-1) && mask_ab.p;
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA]);
+        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA], !masked);
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s);
+        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s, false,
+                           masked ? mask_ab.as<uint32_t>() : nullptr);
     }
 }
 
@@ -938,7 +1018,7 @@ size_t sparse_bytes(const SparseDev& S) {
 
 size_t Plan::device_bytes() const {
     size_t b = K.bytes + KFC.bytes + KFCc.bytes + grid.bytes + fft_a.bytes + fft_b.bytes + grid_b.bytes +
-               fft_ab.bytes + fft_bb.bytes;
+               fft_ab.bytes + fft_bb.bytes + mask_fb.bytes + mask_ab.bytes;
     const Work& w = work;
     for (const DevBuf* d : {&w.hy, &w.k1, &w.k2, &w.i1, &w.i2, &w.i3, &w.i4, &w.sc, &w.partial, &w.hist, &w.ys,
                             &w.hy0, &w.kpub, &w.z1, &w.z2, &w.z3, &w.z4, &w.z5})
@@ -953,6 +1033,8 @@ size_t Plan::device_bytes() const {
         if (once(fr.Gu_s.get())) b += sparse_bytes(*fr.Gu_s);
         if (once(fr.Gut_s.get())) b += sparse_bytes(*fr.Gut_s);
         if (once(fr.perm.get())) b += fr.perm->bytes;
+        if (once(fr.fmask.get())) b += fr.fmask->bytes;
+        if (once(fr.amask.get())) b += fr.amask->bytes;
     }
     if (Gu_b) b += sparse_bytes(*Gu_b);
     if (Gut_b) b += sparse_bytes(*Gut_b);
@@ -1069,7 +1151,8 @@ void Plan::forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s) {
     ensure_fft_scratch();
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true);
+        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true,
+                           masked_grid(f) ? fr.fmask->as<uint32_t>() : nullptr);
     }
     {
         TimedRegion t(0, s);
@@ -1083,13 +1166,15 @@ void Plan::adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, floa
     const Frame& fr = frame(f);
     ensure_grid();
     ensure_fft_scratch();
+    const bool masked = masked_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s);
+        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s, nullptr, !masked);
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true);
+        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true,
+                           masked ? fr.amask->as<uint32_t>() : nullptr);
     }
 }
 

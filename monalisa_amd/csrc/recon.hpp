@@ -32,6 +32,10 @@ struct Frame {
     const SparseDev* Gut = nullptr;
     std::shared_ptr<SparseDev> Gu_s, Gut_s;
     std::shared_ptr<DevBuf> perm;    // int32[P]
+    // bit c: grid cell c is read by the forward gridding (fmask) / written
+    // by the adjoint gridding (amask)
+    std::shared_ptr<DevBuf> fmask, amask;
+    bool masked = false;             // sparse frame: Gut_s stores touched cells only, passes masked
     int64_t P = 0;
 };
 
@@ -80,9 +84,11 @@ struct Plan {
     // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
     // (bf: grid-side arrays stored as bf16x2)
     void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s,
-                            bool bf = false);
+                            bool bf = false, const uint32_t* out_mask = nullptr);
     void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, int nb, DevBuf& a, DevBuf& b,
-                            hipStream_t s, bool bf = false);
+                            hipStream_t s, bool bf = false, const uint32_t* in_mask = nullptr);
+    // frame f (f < 0: every frame of the plan) runs with cell masks
+    bool masked_grid(int f) const;
     int64_t fft_scratch_elems() const;
     void ensure_fft_scratch();
 
@@ -91,6 +97,7 @@ struct Plan {
     std::shared_ptr<SparseDev> Gu_b, Gut_b;
     std::vector<int64_t> koff_b;       // k-space offset (float2) of each frame
     DevBuf grid_b, fft_ab, fft_bb;
+    DevBuf mask_fb, mask_ab;           // stacked cell masks of the frames
     bool use_batch = false;            // teva_chain uses the batched operators
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)

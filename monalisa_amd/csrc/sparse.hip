@@ -600,7 +600,7 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
 
 template <class TV, class TW>
 void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s,
-                   const SpView* view) {
+                   const SpView* view, bool zero_rest) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
     if (S.cv.p && S.nnz > 0) {
         SpView V;
@@ -614,7 +614,7 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
         }
         const bool al = ((uintptr_t)v % 16 == 0) && ((uintptr_t)w % 16 == 0);
         const int cpt = (al && n_vec % 4 == 0) ? 4 : (al && n_vec % 2 == 0) ? 2 : 1;
-        if ((S.squeezed || S.l_nJump != S.l_size) && !S.covers_all)
+        if ((S.squeezed || S.l_nJump != S.l_size) && !S.covers_all && zero_rest)
             fill_f32(reinterpret_cast<float*>(w), 0.0f, V.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
         if (V.rows == 0) return;
         // measured on 3D grids at 16 and 20 coils: the gridding gather
@@ -664,20 +664,20 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
 }  // namespace
 
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s, const SpView* view) {
-    spmv_c2_typed(S, v, w, n_vec, yscale, s, view);
+                      const float* yscale, hipStream_t s, const SpView* view, bool zero_rest) {
+    spmv_c2_typed(S, v, w, n_vec, yscale, s, view, zero_rest);
 }
 
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
-                         const float* yscale, hipStream_t s, const SpView* view) {
+                         const float* yscale, hipStream_t s, const SpView* view, bool zero_rest) {
     if (v_bf16 && w_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest);
     else if (v_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest);
     else if (w_bf16)
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest);
     else
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest);
 }
 
 void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& parts,

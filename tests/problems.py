@@ -23,6 +23,12 @@ def make_problem(kind, seed=0, nCh=3, kernel="kaiser"):
         nSeg, nShot, nSample = 10, 20, 32
         t = synth.koosh_ball(nSeg, nShot, nSample, k_max=8.0)
         ve = synth.ve_radial3(t, nSample, nSeg * nShot)
+    elif kind == "3d32s":
+        # 20 spokes: under 30% of the grid touched, the masked (sparse-frame) path
+        n_u, N_u = [16, 16, 16], [32, 32, 32]
+        nSeg, nShot, nSample = 2, 10, 32
+        t = synth.koosh_ball(nSeg, nShot, nSample, k_max=8.0)
+        ve = synth.ve_radial3(t, nSample, nSeg * nShot)
     elif kind == "2d_rect":
         n_u, N_u = [16, 24], [32, 48]
         nSpoke, nSample = 30, 32

@@ -53,7 +53,7 @@ def test_deapodisation_vs_oracle(kind, kernel):
 
 
 @pytest.mark.parametrize("kind,nCh", [("2d", 3), ("3d", 3), ("2d_rect", 3), ("3d32", 3), ("3d32", 20),
-                                      ("2d", 1)])
+                                      ("2d", 1), ("3d32s", 4)])
 def test_forward_adjoint_vs_oracle(kind, nCh):
     import torch
     p = make_problem(kind, nCh=nCh)
@@ -107,7 +107,7 @@ def test_operators_deterministic():
     assert np.array_equal(ya.view(np.int32), yb.view(np.int32))
 
 
-@pytest.mark.parametrize("kind", ["3d", "3d32"])
+@pytest.mark.parametrize("kind", ["3d", "3d32", "3d32s"])
 def test_fft_pass_variants(kind):
     """FFT pass kernels: the LDS Stockham kernel with tiles 2048 / 4096 /
     8192 runs the same butterflies in the same order for every tile (bit-

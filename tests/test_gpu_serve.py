@@ -9,10 +9,10 @@ from problems import make_problem
 pytestmark = pytest.mark.gpu
 
 
-def _protocols():
+def _protocols(kind3d="3d32"):
     from monalisa_amd.serve import make_protocol
     out = {}
-    for key, kind, nCh in (("2d", "2d_rect", 3), ("3d", "3d32", 4)):
+    for key, kind, nCh in (("2d", "2d_rect", 3), ("3d", kind3d, 4)):
         p = make_problem(kind, seed=21, nCh=nCh)
         out[key] = (p, make_protocol(p["t"], p["ve"], p["C"], p["N_u"], p["n_u"], p["dK_u"], p["kernel"]))
     return out
@@ -51,11 +51,14 @@ def _alone(protos, key, y, nCGD):
     return x.cpu().numpy()
 
 
-def test_sensa_batch_equals_sensa_alone():
+@pytest.mark.parametrize("kind", ["3d32", "3d32s"])
+def test_sensa_batch_equals_sensa_alone(kind):
+    """Dense frames (one adjoint row per cell) and sparse frames (cell-masked
+    FFT passes) alike."""
     import torch
     from monalisa_amd.nufft import Plan
-    print("[gpu] sensa_batch vs sensa", flush=True)
-    protos = _protocols()
+    print(f"[gpu] sensa_batch vs sensa {kind}", flush=True)
+    protos = _protocols(kind3d=kind)
     reqs = [r for r in _requests(protos, 12, seed=6) if r[0] == "3d"][:3]
     p, pr = protos["3d"]
     P = Plan(pr.N_u, pr.n_u, pr.dK_u, pr.nCh, pr.kernel)
