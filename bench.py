@@ -62,8 +62,15 @@ def build_problem(cfg, seed=1234):
     t0 = time.time()
     t = synth.koosh_ball(cfg["nSeg"], cfg["nShot"], cfg["nSample"], k_max=n / 2)
     ve = synth.ve_radial3(t, cfg["nSample"], nSpoke)
+    t1 = time.time()
     G = bmTraj2SparseMat(t, ve, N_u, dK_u, "kaiser")
-    log(f"[bench] trajectory + Gu/Gut ({t.shape[1]} points, nnz {G['Gu'].nnz}) in {time.time() - t0:.1f}s")
+    t_host = time.time() - t1
+    t1 = time.time()
+    bmTraj2SparseMat(t, ve, N_u, dK_u, "kaiser", device=True)
+    torch.cuda.synchronize()
+    t_dev = time.time() - t1
+    log(f"[bench] trajectory + Gu/Gut ({t.shape[1]} points, nnz {G['Gu'].nnz}) in {time.time() - t0:.1f}s "
+        f"(matrices: host {t_host:.2f}s, device {t_dev:.2f}s)")
     C = synth.coil_maps(n_u, nCh, rng)
     phantom = synth.ellipsoid_phantom(n_u, rng)
     P = Plan(N_u, n_u, dK_u, nCh, "kaiser")
@@ -82,7 +89,7 @@ def build_problem(cfg, seed=1234):
     ve_d = torch.from_numpy(ve.astype(np.float32)).cuda()
     torch.cuda.synchronize()
     return dict(P=P, G=G, y=y, ve=ve_d, ve_np=ve, C=C, n_u=n_u, N_u=N_u, dK_u=dK_u, nCh=nCh,
-                phantom=phantom, xd=xd, P_pts=t.shape[1])
+                phantom=phantom, xd=xd, P_pts=t.shape[1], build_s={"host": t_host, "device": t_dev})
 
 
 def build_binned_problem(cfg, seed=4321):
@@ -191,6 +198,7 @@ def bench_binned(args, cfg):
         "batched_over_per_bin": s_plain / s_batch,
         "batched_vs_per_bin_rel_l2": rel,
         "image_rel_err_vs_phantom": img_err,
+        "gridding_matrix_build_s": prob["build_s"],
         "fft_tile_autotuned": tile,
         "points_per_bin": [len(b) * cfg["nSample"] for b in prob["bins"]],
     }

@@ -194,6 +194,14 @@ int mona_traj2sparsemat(int imDim, int nPt, const double* t, const double* ve,
                         const int* N_u, const double* dK_u, int kernelType, int nWin,
                         const double* kernelParam, int nParam, int which,
                         mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut);
+/* The same construction on the GPU (same host arguments; the work runs on
+ * `stream` and the call returns when the matrices are ready).  The integer
+ * arrays equal mona_traj2sparsemat's exactly; m_val may differ in the last
+ * float bit (device exp / sqrt). */
+int mona_traj2sparsemat_device(int imDim, int nPt, const double* t, const double* ve,
+                               const int* N_u, const double* dK_u, int kernelType, int nWin,
+                               const double* kernelParam, int nParam, int which,
+                               mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut, hipStream_t stream);
 
 /* --- reconstruction plan -----------------------------------------------
  * Geometry + gridding kernel + coil maps; owns KFC/KFC_conj (bmKF.m:6-35,

@@ -56,6 +56,7 @@ SIGNATURES = {
     "mona_sparse_export": (I, [P, IP, IP, FP, IP]),
     "mona_sparse_apply": (I, [P, P, P, I, P]),
     "mona_traj2sparsemat": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP]),
+    "mona_traj2sparsemat_device": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP, P]),
     "mona_plan_create": (I, [I, IP, IP, DP, I, I, I, DP, I, PP]),
     "mona_plan_destroy": (I, [P]),
     "mona_plan_set_coils": (I, [P, P, P]),

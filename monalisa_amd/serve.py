@@ -276,11 +276,11 @@ class Protocol:
 
 
 def make_protocol(t, ve, C, N_u, n_u, dK_u, kernel="kaiser") -> Protocol:
-    """Builds the gridding matrices on the host and uploads coil maps and
+    """Builds the gridding matrices (on the GPU) and uploads coil maps and
     density weights; the plans of the protocol share them."""
     import torch
     from .sparse import bmTraj2SparseMat
-    G = bmTraj2SparseMat(t, ve, N_u, dK_u, kernel)
+    G = bmTraj2SparseMat(t, ve, N_u, dK_u, kernel, device=True)
     return Protocol(list(map(int, np.ravel(N_u))), list(map(int, np.ravel(n_u))), list(map(float, np.ravel(dK_u))),
                     int(C.shape[1]), kernel, G, torch.from_numpy(np.ascontiguousarray(C, np.complex64)).cuda(),
                     torch.from_numpy(np.asarray(ve, np.float32)).cuda(), nPt=int(G["Gu"].l_size))

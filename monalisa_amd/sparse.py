@@ -153,10 +153,11 @@ class BmSparseMat:
 
 
 def bmTraj2SparseMat(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=None,
-                     which=("Gu", "Gut")):
+                     which=("Gu", "Gut"), device=False):
     """Gridding matrices for trajectory t ([imDim, nPt] or [nPt, imDim] with
     imDim <= 3 rows), volume elements ve [nPt].  Returns a dict with the
-    requested matrices among 'Gn', 'Gu', 'Gut'."""
+    requested matrices among 'Gn', 'Gu', 'Gut'.  device=True builds them on
+    the GPU (same integer arrays; m_val may differ in the last bit)."""
     kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
     t = np.asarray(t, dtype=np.float64)
     if t.ndim == 1:
@@ -172,9 +173,15 @@ def bmTraj2SparseMat(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=N
     w = (1 if "Gn" in which else 0) | (2 if ("Gu" in which or "Gut" in which) else 0)
     hn, hu, hut = C.c_void_p(), C.c_void_p(), C.c_void_p()
     dp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
-    check(lib.mona_traj2sparsemat(imDim, nPt, dp(t), dp(ve_a), _ip(N_u), dp(dK_u),
-                                  KERNELS[kernelType], nWin, dp(kp), len(kp), w,
-                                  C.byref(hn), C.byref(hu), C.byref(hut)))
+    if device:
+        from ._lib import stream_of
+        check(lib.mona_traj2sparsemat_device(imDim, nPt, dp(t), dp(ve_a), _ip(N_u), dp(dK_u),
+                                             KERNELS[kernelType], nWin, dp(kp), len(kp), w,
+                                             C.byref(hn), C.byref(hu), C.byref(hut), stream_of()))
+    else:
+        check(lib.mona_traj2sparsemat(imDim, nPt, dp(t), dp(ve_a), _ip(N_u), dp(dK_u),
+                                      KERNELS[kernelType], nWin, dp(kp), len(kp), w,
+                                      C.byref(hn), C.byref(hu), C.byref(hut)))
     prov = dict(N_u=[int(v) for v in N_u], d_u=[float(np.float32(v)) for v in dK_u],
                 kernel_type=kernelType, nWin=nWin, kernelParam=kernelParam)
     out = {}
