@@ -202,6 +202,15 @@ int mona_traj2sparsemat_device(int imDim, int nPt, const double* t, const double
                                const int* N_u, const double* dK_u, int kernelType, int nWin,
                                const double* kernelParam, int nParam, int which,
                                mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut, hipStream_t stream);
+/* Interpolation matrices of a deformation field
+ * (bmImDeformField2SparseMat.m:14-147, bmImDeformField2sparseMat_ind_weight.m:6-150):
+ * v [prod(N_u)][imDim] displaces pixel p (1-based coordinates) to t_p;
+ * bump weights exp(-1/(1-d^2)) on the 2^imDim cells around t_p, wrapped
+ * mod N_u; points leaving [1, N_u] are deleted unless torus != 0.  Gu rows
+ * (pixels) are normalised, Gut = Gu', Gn = rownorm((w .* Dn)') (needs Dn
+ * [prod(N_u)]).  No matrix is squeezed.  which: 1 = Gn, 2 = Gu+Gut, 3 = all. */
+int mona_deform2sparsemat(int imDim, const int* N_u, const double* v, const double* Dn, int torus,
+                          int which, mona_sparse* Gn, mona_sparse* Gu, mona_sparse* Gut);
 
 /* --- reconstruction plan -----------------------------------------------
  * Geometry + gridding kernel + coil maps; owns KFC/KFC_conj (bmKF.m:6-35,
@@ -253,8 +262,9 @@ int mona_steva(mona_plan P, int frame, void* x, void* z, void* u, int zu_given,
                const float* rho, int nIter, int nCGD, double* hist,
                hipStream_t stream);
 
-/* Temporal-TV ADMM over the plan's nFr frames with identity deformations
- * (bmTevaMorphosia_chain.m:6-294, regul_mode 'normal', Tu = []).
+/* Temporal-TV ADMM over the plan's nFr frames (bmTevaMorphosia_chain.m:6-294,
+ * regul_mode 'normal'): F x_f = Tu_f x_f - x_{f-1} with the deformations
+ * given by mona_plan_set_deform (identity for frames without one, Tu = []).
  * x, z, u: nFr images each, contiguous [nFr][prod(n_u)] float2.
  * y: frame f at y_off[f] float2 elements into y ([nPt_f][nCh]); ve likewise
  * at ve_off[f] (floats).  hist (host, may be NULL): nIter x {objective,
@@ -264,6 +274,12 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
                     const long long* ve_off, float ve_max, const float* delta,
                     const float* rho, int nIter, int nCGD, double* hist,
                     hipStream_t stream);
+
+/* Deformation matrices of frame f for mona_teva_chain (the Tu{f}, Tut{f}
+ * arguments of bmTevaMorphosia_chain.m:6-13, applied as bmImDeform.m /
+ * bmImDeformT.m do): both prod(n_u) x prod(n_u), referenced, not owned.
+ * Tu = Tut = NULL restores the identity. */
+int mona_plan_set_deform(mona_plan P, int frame, mona_sparse Tu, mona_sparse Tut);
 
 /* Gridded reconstruction (bmNasha.m:6-91): n2u with Gn, centred ifft times
  * prod(N_u)*prod(dK_u), times K, crop to n_u; with C != NULL the coil

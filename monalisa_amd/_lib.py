@@ -57,6 +57,7 @@ SIGNATURES = {
     "mona_sparse_apply": (I, [P, P, P, I, P]),
     "mona_traj2sparsemat": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP]),
     "mona_traj2sparsemat_device": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP, P]),
+    "mona_deform2sparsemat": (I, [I, IP, DP, DP, I, I, PP, PP, PP]),
     "mona_plan_create": (I, [I, IP, IP, DP, I, I, I, DP, I, PP]),
     "mona_plan_destroy": (I, [P]),
     "mona_plan_set_coils": (I, [P, P, P]),
@@ -68,6 +69,7 @@ SIGNATURES = {
     "mona_cfft": (I, [I, IP, I, P, I, P]),
     "mona_sensa": (I, [P, I, P, P, P, F, I, I, DP, P]),
     "mona_steva": (I, [P, I, P, P, P, I, P, P, F, FP, FP, I, I, DP, P]),
+    "mona_plan_set_deform": (I, [P, I, P, P]),
     "mona_teva_chain": (I, [P, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_nasha": (I, [P, P, P, P, P, P]),
     "mona_sensa_batch": (I, [P, I, P, P, LLP, FP, LLP, FP, I, I, DP, P]),

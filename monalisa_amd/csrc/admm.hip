@@ -1,7 +1,8 @@
 // ADMM total-variation solvers and the gridded reconstruction.
 //
 //   bmSteva               spatial TV, one frame   (bmSteva.m:6-138)
-//   bmTevaMorphosia_chain temporal TV over frames (bmTevaMorphosia_chain.m:6-294, Tu = [])
+//   bmTevaMorphosia_chain temporal TV over frames, optionally motion-compensated by
+//                         deformation matrices Tu/Tut (bmTevaMorphosia_chain.m:6-294)
 //   bmNasha               n2u with Gn + centred ifft + deapodisation (bmNasha.m:6-91)
 //
 // The finite differences, their transposes and the soft-threshold prox are
@@ -103,17 +104,17 @@ __global__ void k_prox_update(const float2* __restrict__ Fx, float2* __restrict_
     u[i] = make_float2(w.x - zz.x, w.y - zz.y);
 }
 
-// Temporal difference over cyclic frames: out_f = x_f - x_{f-1}
-// (bmTevaMorphosia_chain.m:266-272 with identity deformation).  With z, u:
-// out = (z - u) - (x_f - x_{f-1}).
-__global__ void k_tdiff(const float2* __restrict__ x, float2* __restrict__ out, const float2* __restrict__ z,
-                        const float2* __restrict__ u, int nFr, int64_t nu) {
+// Temporal difference over cyclic frames: out_f = xa_f - x_{f-1} with
+// xa_f = Tu_f x_f (private_F, bmTevaMorphosia_chain.m:266-272; xa = x for
+// identity deformations).  With z, u: out = (z - u) - (xa_f - x_{f-1}).
+__global__ void k_tdiff(const float2* __restrict__ xa, const float2* __restrict__ x, float2* __restrict__ out,
+                        const float2* __restrict__ z, const float2* __restrict__ u, int nFr, int64_t nu) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nFr * nu) return;
     int f = (int)(t / nu);
     int64_t i = t - (int64_t)f * nu;
     int fm = (f + nFr - 1) % nFr;
-    float2 a = x[t], b = x[(int64_t)fm * nu + i];
+    float2 a = xa[t], b = x[(int64_t)fm * nu + i];
     float2 g = make_float2(a.x - b.x, a.y - b.y);
     if (z) {
         float2 zz = z[t], uu = u[t];
@@ -123,15 +124,16 @@ __global__ void k_tdiff(const float2* __restrict__ x, float2* __restrict__ out, 
     }
 }
 
-// out_f += coef * (z_f - z_{f+1})   (private_dagF, bmTevaMorphosia_chain.m:285-291)
-__global__ void k_tdiffT_acc(const float2* __restrict__ z, float2* __restrict__ out, int nFr, int64_t nu,
-                             float coef) {
+// out_f += coef * (za_f - z_{f+1}), za_f = Tut_f z_f   (private_dagF,
+// bmTevaMorphosia_chain.m:285-291)
+__global__ void k_tdiffT_acc(const float2* __restrict__ za, const float2* __restrict__ z, float2* __restrict__ out,
+                             int nFr, int64_t nu, float coef) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nFr * nu) return;
     int f = (int)(t / nu);
     int64_t i = t - (int64_t)f * nu;
     int fp = (f + 1) % nFr;
-    float2 a = z[t], b = z[(int64_t)fp * nu + i];
+    float2 a = za[t], b = z[(int64_t)fp * nu + i];
     float2 o = out[t];
     out[t] = make_float2(o.x + coef * (a.x - b.x), o.y + coef * (a.y - b.y));
 }
@@ -371,6 +373,33 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
     }
     const float2* ysorted = w.ys.as<float2>();
     auto yF = [&](int f) { return ysorted + koff[f]; };
+    // deformations: Tu_f x_f and Tut_f z_f go to a scratch image set, frames
+    // without a deformation are copied
+    bool deform = false;
+    for (int f = 0; f < nFr && f < (int)P.deform.size(); f++) deform = deform || P.deform[f].first;
+    if (deform) w.z6.ensure((size_t)nx * sizeof(float2));
+    float2* tx = deform ? w.z6.as<float2>() : nullptr;
+    auto warp = [&](const float2* v, bool transposed, hipStream_t s) -> const float2* {
+        if (!deform) return v;
+        for (int f = 0; f < nFr; f++) {
+            const SparseDev* T = f < (int)P.deform.size()
+                                     ? (transposed ? P.deform[f].second : P.deform[f].first) : nullptr;
+            if (T) spmv_c2_rowmajor(*T, v + (int64_t)f * nu, tx + (int64_t)f * nu, 1, nullptr, s);
+            else vcopy(v + (int64_t)f * nu, tx + (int64_t)f * nu, nu, s);
+        }
+        return tx;
+    };
+    // F v (with z, u: (z - u) - F v) and out += coef * F^T-part of dagF
+    auto Fop = [&](const float2* v, float2* out, const float2* zz, const float2* uu, hipStream_t s) {
+        const float2* va = warp(v, false, s);
+        k_tdiff<<<nb(nx), 256, 0, s>>>(va, v, out, zz, uu, nFr, nu);
+        MONA_HIP(hipGetLastError());
+    };
+    auto FtAcc = [&](const float2* zz, float2* out, float coef, hipStream_t s) {
+        const float2* za = warp(zz, true, s);
+        k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(za, zz, out, nFr, nu, coef);
+        MONA_HIP(hipGetLastError());
+    };
     // operators: one launch per gridding op over all frames (batched form), or
     // frame by frame
     if (P.use_batch && (!P.Gu_b || (int)P.koff_b.size() != nFr + 1)) P.build_batch(s);
@@ -385,8 +414,7 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
             P.adjoint(f, r + koff[f], out + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, cs);
     };
     if (!zu_given) {
-        k_tdiff<<<nb(nx), 256, 0, s>>>(x, z, nullptr, nullptr, nFr, nu);
-        MONA_HIP(hipGetLastError());
+        Fop(x, z, nullptr, nullptr, s);
         fill_f32(reinterpret_cast<float*>(u), 0.0f, nx * 2, s);
     }
     // monitoring values go to a fixed slot and are committed to hist[3*iter]
@@ -401,17 +429,14 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
         cg_reset(sc, s);
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
-        k_tdiff<<<nb(nx), 256, 0, s>>>(x, res_z, z, u, nFr, nu);
-        MONA_HIP(hipGetLastError());
+        Fop(x, res_z, z, u, s);
         Mh(res_y, dag);
-        k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(res_z, dag, nFr, nu, rc * invHX * HZ);
-        MONA_HIP(hipGetLastError());
+        FtAcc(res_z, dag, rc * invHX * HZ, s);
         vcopy(dag, p, nx, s);
         sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn, false, part, s);
         for (int j = 1; j <= nCGD; j++) {
             M(p, Mp);
-            k_tdiff<<<nb(nx), 256, 0, s>>>(p, Fp, nullptr, nullptr, nFr, nu);
-            MONA_HIP(hipGetLastError());
+            Fop(p, Fp, nullptr, nullptr, s);
             sqnorm(Mp, ny, hy, P.nCh, 1.0, &sc->den, false, part, s);
             sqnorm(Fp, nx, nullptr, 1, (double)rc * HZ, &sc->den, true, part, s);
             cg_alpha(sc, s);
@@ -420,22 +445,19 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
             vaxpy(res_y, Mp, &sc->a, -1.0f, ny, s);
             vaxpy(res_z, Fp, &sc->a, -1.0f, nx, s);
             Mh(res_y, dag);
-            k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(res_z, dag, nFr, nu, rc * invHX * HZ);
-            MONA_HIP(hipGetLastError());
+            FtAcc(res_z, dag, rc * invHX * HZ, s);
             sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn_new, false, part, s);
             cg_beta(sc, s);
             vxpby(p, dag, &sc->b, nx, s);
         }
-        k_tdiff<<<nb(nx), 256, 0, s>>>(x, Fp, nullptr, nullptr, nFr, nu);
-        MONA_HIP(hipGetLastError());
+        Fop(x, Fp, nullptr, nullptr, s);
         k_prox_update<<<nb(nx), 256, 0, s>>>(Fp, z, u, nx, dl / rc);
         MONA_HIP(hipGetLastError());
         // monitoring: dafi, regul (private_dafi_regul), objective
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
         sqnorm(res_y, ny, hy, P.nCh, 1.0, hslot + 1, false, part, s);
-        k_tdiff<<<nb(nx), 256, 0, s>>>(x, Fp, nullptr, nullptr, nFr, nu);
-        MONA_HIP(hipGetLastError());
+        Fop(x, Fp, nullptr, nullptr, s);
         abs1(Fp, nx, (double)HZ, hslot + 2, part, s);
         k_objective<<<1, 1, 0, s>>>(hslot, (double)dl);
         MONA_HIP(hipGetLastError());
@@ -543,6 +565,29 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given, const 
         MONA_CHECK(P && x && z && u && y && y_off && ve && ve_off && delta && rho, MONA_ERR_ARG, "NULL argument");
         teva_chain(P->P, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u), zu_given != 0,
                    static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD, hist, stream);
+    });
+}
+
+int mona_plan_set_deform(mona_plan P, int frame, mona_sparse Tu, mona_sparse Tut) {
+    return guarded([&] {
+        MONA_CHECK(P && frame >= 0, MONA_ERR_ARG, "bad argument");
+        MONA_CHECK((Tu == nullptr) == (Tut == nullptr), MONA_ERR_ARG, "Tu and Tut are given together");
+        Plan& pl = P->P;
+        if (Tu) {
+            MONA_CHECK(Tu->S.r_size == pl.nu && Tu->S.l_size == pl.nu, MONA_ERR_ARG,
+                       "Tu must be prod(n_u) x prod(n_u)");
+            MONA_CHECK(Tut->S.r_size == pl.nu && Tut->S.l_size == pl.nu, MONA_ERR_ARG,
+                       "Tut must be prod(n_u) x prod(n_u)");
+            HostCallCtx& hc = host_ctx();
+            {
+                std::lock_guard<std::mutex> lk(hc.mu);
+                hc.init();
+            }
+            sparse_upload(Tu->S, hc.stream);
+            sparse_upload(Tut->S, hc.stream);
+        }
+        if ((int)pl.deform.size() <= frame) pl.deform.resize(frame + 1, {nullptr, nullptr});
+        pl.deform[frame] = {Tu ? &Tu->S : nullptr, Tut ? &Tut->S : nullptr};
     });
 }
 

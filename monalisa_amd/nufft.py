@@ -47,6 +47,7 @@ class Plan:
                                    len(kernelParam), C.byref(h)))
         self._h = h
         self._frames = {}
+        self._deform = {}
         self.nu = int(np.prod(self.n_u))
         self.Nu = int(np.prod(self.N_u))
 
@@ -63,6 +64,15 @@ class Plan:
     def set_frame(self, f, Gu: BmSparseMat, Gut: BmSparseMat):
         check(lib.mona_plan_set_frame(self._h, int(f), Gu.handle(), Gut.handle()))
         self._frames[f] = (Gu, Gut)   # keep the handles alive
+
+    def set_deform(self, f, Tu: BmSparseMat = None, Tut: BmSparseMat = None):
+        """Deformation matrices of frame f for the temporal-TV solver (the
+        Tu{f}, Tut{f} of bmTevaMorphosia_chain.m:6-13); None = identity."""
+        if (Tu is None) != (Tut is None):
+            raise ValueError("Tu and Tut are given together")
+        check(lib.mona_plan_set_deform(self._h, int(f), Tu.handle() if Tu is not None else None,
+                                       Tut.handle() if Tut is not None else None))
+        self._deform[f] = (Tu, Tut)   # keep the handles alive
 
     def nPt(self, f=0):
         return self._frames[f][0].l_size
@@ -221,12 +231,19 @@ class Plan:
         return z, u, h
 
     def bmTevaMorphosia_chain(self, x, z, u, y_list, ve_list, delta, rho, nCGD, ve_max, nIter=1,
-                              stream=None):
+                              stream=None, Tu=None, Tut=None):
         """Temporal-TV ADMM over all frames (bmTevaMorphosia_chain.m:6-294),
-        Tu = [] (identity deformations), regul_mode 'normal'.  x, z, u are
-        [nFr, nu] complex64 (z, u None: z = F x, u = 0)."""
+        regul_mode 'normal'.  x, z, u are [nFr, nu] complex64 (z, u None:
+        z = F x, u = 0).  Tu, Tut: lists of deformation matrices per frame
+        (entries may be None), F x_f = Tu_f x_f - x_{f-1}; when not given,
+        the ones set by set_deform (none: identity, Tu = [])."""
         import torch
         nFr = x.shape[0]
+        if Tu is not None or Tut is not None:
+            Tu = list(Tu) if Tu is not None else [None] * nFr
+            Tut = list(Tut) if Tut is not None else [None] * nFr
+            for f in range(nFr):
+                self.set_deform(f, Tu[f], Tut[f])
         given = z is not None
         if z is None:
             z = torch.zeros_like(x)

@@ -193,6 +193,74 @@ def bmTraj2SparseMat(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=N
     return out
 
 
+def bmImDeformField2SparseMat(v, N_u, Dn=None, torus_flag=False, which=("Gu", "Gut")):
+    """Interpolation matrices of deformation field v (bmImDeformField2SparseMat.m:14-147):
+    v reshapes to [imDim, prod(N_u)] (or is [prod(N_u), imDim]); pixel p
+    moves to its 1-based grid position plus v[:, p].  A list of fields gives
+    a list of dicts, as the reference's cell input does.  'Gn' needs Dn."""
+    if isinstance(v, (list, tuple)):
+        Dns = Dn if isinstance(Dn, (list, tuple)) else [Dn] * len(v)
+        return [bmImDeformField2SparseMat(vi, N_u, di, torus_flag, which) for vi, di in zip(v, Dns)]
+    N_u = np.ascontiguousarray(np.ravel(N_u), dtype=np.int32)
+    imDim = N_u.size
+    nPt = int(np.prod(N_u.astype(np.int64)))
+    v = np.asarray(v, dtype=np.float64)
+    if v.size != imDim * nPt:
+        raise ValueError("the deformation field must have as many vectors as the number of pixels in the image")
+    if v.ndim == 2 and v.shape == (nPt, imDim) and imDim != nPt:
+        va = np.ascontiguousarray(v)
+    else:
+        va = np.ascontiguousarray(v.reshape(-1, order="F").reshape(nPt, imDim))  # [imDim, nPt] column-major
+    w = (1 if "Gn" in which else 0) | (2 if ("Gu" in which or "Gut" in which) else 0)
+    if (w & 1) and Dn is None:
+        raise ValueError("Gn needs Dn")
+    dn = None if Dn is None else np.ascontiguousarray(np.ravel(Dn), dtype=np.float64)
+    if dn is not None and dn.size != nPt:
+        raise ValueError("Dn must have prod(N_u) entries")
+    dp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+    hn, hu, hut = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    check(lib.mona_deform2sparsemat(imDim, _ip(N_u), dp(va), dp(dn), int(bool(torus_flag)), w,
+                                    C.byref(hn), C.byref(hu), C.byref(hut)))
+    prov = dict(N_u=[int(x) for x in N_u], d_u=[1.0, 1.0, 1.0], kernel_type="bump", nWin=2, kernelParam=None)
+    out = {}
+    if w & 1:
+        out["Gn"] = BmSparseMat._from_handle(hn, prov)
+    if w & 2:
+        out["Gu"] = BmSparseMat._from_handle(hu, prov)
+        out["Gut"] = BmSparseMat._from_handle(hut, prov)
+    return out
+
+
+def bmSparseMat_apply(s: BmSparseMat, v, stream=None):
+    """w = S v on the GPU: v a complex64 cuda tensor [r_size] or
+    [r_size, n_vec] (vector index fastest); returns [l_size(, n_vec)]."""
+    import torch
+    from ._lib import ptr, stream_of
+    if not (v.is_cuda and v.dtype == torch.complex64):
+        raise TypeError("v must be a complex64 cuda tensor")
+    if v.shape[0] != s.r_size:
+        raise ValueError(f"v has {v.shape[0]} rows, S has r_size {s.r_size}")
+    v = v.contiguous()
+    n_vec = 1 if v.dim() == 1 else int(np.prod(v.shape[1:]))
+    w = torch.empty((s.l_size,) + tuple(v.shape[1:]), dtype=torch.complex64, device=v.device)
+    check(lib.mona_sparse_apply(s.handle(), ptr(v), ptr(w), n_vec, stream_of(stream)))
+    return w
+
+
+def bmImDeform(Tu, x, n_u=None):
+    """x_out = Tu x on the image grid (src/image123/bmImDeform.m:6-43 with
+    K = []); Tu None returns x.  x: complex64 cuda tensor of prod(n_u)
+    elements (any shape), the result has x's shape."""
+    if Tu is None:
+        return x
+    return bmSparseMat_apply(Tu, x.reshape(-1)).reshape(x.shape)
+
+
+def bmImDeformT(Tut, x, n_u=None):
+    """x_out = Tut x (src/image123/bmImDeformT.m:6-45 with K = [])."""
+    return bmImDeform(Tut, x, n_u)
+
+
 def bmSparseMat_r_nJump2index(r_nJump):
     """0-based row of each entry (bmSparseMat_r_nJump2index_mex.cpp:70-89)."""
     r = _i32(r_nJump)

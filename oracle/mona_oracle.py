@@ -25,8 +25,14 @@ Every function cites the reference file:line it follows.  What pins each one
   solution of the normal equations.
 * ``back_gradient``/``back_gradient_T``: the transpose identity on inputs whose
   first slice is zero; ``prox_one_norm``: closed-form soft threshold.
+* ``deform2sparsemat``: closed forms (zero field -> identity, integer torus
+  shift -> permutation, fractional 1-D shift -> the two normalised bump
+  weights), a brute-force loop construction (``deform_dense_bruteforce``),
+  deletion of points that leave the grid, Gut == Gu^T exactly.
 * ``steva``/``teva_chain``: parity unpinned beyond the above building blocks
-  and monotone objective checks (no reference run exists to compare with).
+  and monotone objective checks (no reference run exists to compare with);
+  with deformations, identity matrices reproduce the undeformed chain and a
+  torus shift equals the same chain on shifted frames.
 """
 from __future__ import annotations
 
@@ -201,6 +207,111 @@ def traj2sparsemat(t, ve, N_u, dK_u, kernelType=None, nWin=None, kernelParam=Non
         out['Gu'] = Gu
         out['Gut'] = Gut
     return out
+
+
+def deform_entries(v, N_u, Dn=None, torus_flag=False):
+    """bmImDeformField2sparseMat_ind_weight
+    (src/gridding123/m/bmImDeformField2sparseMat_ind_weight.m:6-150):
+    pixel p (1-based grid position, x fastest) moves to t = x_p + v[:, p];
+    its 2^imDim neighbours floor(t) + c, c in {0,1}^imDim (x fastest),
+    wrapped mod N_u, weigh exp(-1/(1-d^2)) (inf -> 0) times (d < 1) with d
+    the Euclidean distance of t - floor(t) to c.  Points that leave
+    [1, N_u] are deleted unless torus_flag.  v: [imDim, nPt] (float64)."""
+    N_u = np.asarray(N_u, dtype=np.int64).ravel()
+    imDim = N_u.size
+    nPt = int(np.prod(N_u))
+    v = np.asarray(v, dtype=np.float64).reshape(-1, order='F').reshape(imDim, nPt, order='F')
+    grids = np.meshgrid(*[np.arange(1, n + 1, dtype=np.float64) for n in N_u], indexing='ij')
+    x = np.stack([g.reshape(-1, order='F') for g in grids])          # [imDim, nPt]
+    t = x + v
+    delete = np.zeros(nPt, bool)
+    for d in range(imDim):
+        delete |= (t[d] < 1) | (t[d] > N_u[d])
+    c = np.stack([g.reshape(-1, order='F') for g in
+                  np.meshgrid(*[np.arange(2)] * imDim, indexing='ij')])  # [imDim, nNb]
+    nNb = c.shape[1]
+    tf = np.floor(t)
+    tr = t - tf
+    dd = tr[:, None, :] - c[:, :, None]                               # [imDim, nNb, nPt]
+    dist = np.sqrt((dd ** 2).sum(axis=0))
+    with np.errstate(divide='ignore', over='ignore'):
+        w = np.exp(-1.0 / (1.0 - dist ** 2))
+    w[np.isinf(w)] = 0.0
+    w = w * (np.abs(dist) < 1)
+    n = tf[:, None, :] + c[:, :, None]
+    lin = np.zeros((nNb, nPt), np.int64)
+    stride = 1
+    for d in range(imDim):
+        lin += (np.mod(n[d] - 1, N_u[d]).astype(np.int64)) * stride
+        stride *= int(N_u[d])
+    keep = np.ones(nPt, bool) if torus_flag else ~delete
+    Dn = None if Dn is None else np.asarray(Dn, dtype=np.float64).ravel()
+    return dict(ind_1=lin[:, keep], weight=w[:, keep], pt=np.nonzero(keep)[0],
+                Dn=None if Dn is None else Dn[keep], nNb=nNb, nPt=nPt, Nu_tot=nPt, N_u=N_u)
+
+
+def deform2sparsemat(v, N_u, Dn=None, torus_flag=False, which=('Gu', 'Gut')):
+    """bmImDeformField2SparseMat (src/gridding123/m/bmImDeformField2SparseMat.m:14-147):
+    Gu = rownorm(sparse(pt, cell, w)), Gut = Gu', Gn = rownorm(sparse(pt,
+    cell, w.*Dn)'), all cpp_prepared without l_squeeze ('bump' kernel,
+    d_u = 1, nWin = 2)."""
+    e = deform_entries(v, N_u, Dn, torus_flag)
+    nNb, nPt, Nu = e['nNb'], e['nPt'], e['Nu_tot']
+    rows = np.repeat(e['pt'], nNb)
+    cols = e['ind_1'].T.reshape(-1)
+    w = e['weight'].T.reshape(-1)
+    prov = dict(N_u=e['N_u'], d_u=np.ones(3), kernel_type='bump', nWin=2, kernelParam=None)
+    out = {}
+    if 'Gn' in which:
+        if e['Dn'] is None:
+            raise ValueError('Gn needs Dn')
+        r, c, val = _assemble(cols, rows, w * np.repeat(e['Dn'], nNb), (Nu, nPt))
+        val = _row_normalise(r, c, val, Nu)
+        G = to_bmsparse(r, c, val, (Nu, nPt), prov)
+        cpp_prepare(G)
+        out['Gn'] = G
+    if 'Gu' in which or 'Gut' in which:
+        r, c, val = _assemble(rows, cols, w, (nPt, Nu))
+        val = _row_normalise(r, c, val, nPt)
+        Gu = to_bmsparse(r, c, val, (nPt, Nu), prov)
+        Gut = to_bmsparse(c, r, val, (Nu, nPt), prov)
+        cpp_prepare(Gu)
+        cpp_prepare(Gut)
+        out['Gu'] = Gu
+        out['Gut'] = Gut
+    return out
+
+
+def deform_dense_bruteforce(v, N_u, torus_flag=False, normalise=True):
+    """Independent restatement used to pin deform2sparsemat: plain loops over
+    pixels and neighbour corners, math.exp, dense accumulation and (unless
+    normalise is False) row normalisation (float64)."""
+    N_u = [int(n) for n in np.ravel(N_u)]
+    imDim = len(N_u)
+    nPt = int(np.prod(N_u))
+    v = np.asarray(v, dtype=np.float64).reshape(-1, order='F')
+    A = np.zeros((nPt, nPt))
+    for p in range(nPt):
+        q, pos = p, []
+        for d in range(imDim):
+            pos.append(q % N_u[d] + 1 + v[p * imDim + d])
+            q //= N_u[d]
+        if not torus_flag and any(pos[d] < 1 or pos[d] > N_u[d] for d in range(imDim)):
+            continue
+        for corner in range(2 ** imDim):
+            sq, cell, stride = 0.0, 0, 1
+            for d in range(imDim):
+                cd = (corner >> d) & 1
+                base = math.floor(pos[d])
+                sq += (pos[d] - base - cd) ** 2
+                cell += ((base + cd - 1) % N_u[d]) * stride
+                stride *= N_u[d]
+            if sq < 1.0:
+                A[p, cell] += math.exp(-1.0 / (1.0 - sq))
+        s = A[p].sum()
+        if s and normalise:
+            A[p] /= s
+    return A
 
 
 def to_bmsparse(r, c, v, shape, prov=None):
@@ -651,12 +762,14 @@ def steva(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
 
 
 def teva_chain(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
-               ve_max, nIter=1, kernel=(None, None, None)):
-    """Temporal-TV ADMM over nFr cyclic frames with Tu = [] (identity
-    deformation), regul_mode 'normal'
+               ve_max, nIter=1, kernel=(None, None, None), Tu=None, Tut=None):
+    """Temporal-TV ADMM over nFr cyclic frames, regul_mode 'normal'
     (src/fourier123/solver_function/imDim_morphosia/nonCartesian/
     bmTevaMorphosia_chain.m:6-294), float64.
-    x, y, ve, Gu, Gut, z, u: lists over frames."""
+    x, y, ve, Gu, Gut, z, u: lists over frames.  Tu, Tut: lists of
+    deformation matrices (entries None = identity, bmImDeform.m:6-10):
+    F x_i = Tu_i x_i - x_{i-1} (private_F, :266-272) and
+    dagF z_i = (1/HX)(Tut_i (HZ z_i) - HZ z_{i+1}) (private_dagF, :285-291)."""
     eps = 10 * np.finfo(np.float32).eps
     nFr = len(y)
     KFC = kf(C, N_u, n_u, dK_u, *kernel)
@@ -673,8 +786,11 @@ def teva_chain(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
     M = lambda v: [shanna(v[i], Gu[i], KFC, N_u, n_u) for i in range(nFr)]
     Mh = lambda r: [(1.0 / HX) * nakatsha(HY[i] * r[i], Gut[i], KFCc, True, N_u, n_u)
                     for i in range(nFr)]
-    Fop = lambda v: [v[i] - v[(i - 1) % nFr] for i in range(nFr)]
-    FT = lambda w: [(1.0 / HX) * (HZ * w[i] - HZ * w[(i + 1) % nFr]) for i in range(nFr)]
+    Tu = [None] * nFr if Tu is None else [None if T is None else to_scipy(T) for T in Tu]
+    Tut = [None] * nFr if Tut is None else [None if T is None else to_scipy(T) for T in Tut]
+    deform = lambda T, a: a if T is None else (T @ a.reshape(-1, order='F')).reshape(a.shape, order='F')
+    Fop = lambda v: [deform(Tu[i], v[i]) - v[(i - 1) % nFr] for i in range(nFr)]
+    FT = lambda w: [(1.0 / HX) * (deform(Tut[i], HZ * w[i]) - HZ * w[(i + 1) % nFr]) for i in range(nFr)]
     sq = lambda a, H: sum(float(np.real(np.vdot(ai, (H[i] if isinstance(H, list) else H) * ai)))
                           for i, ai in enumerate(a))
     z = Fop(x) if z is None else [np.array(v, dtype=np.complex128) for v in z]
