@@ -275,11 +275,37 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given,
                     const float* rho, int nIter, int nCGD, double* hist,
                     hipStream_t stream);
 
-/* Deformation matrices of frame f for mona_teva_chain (the Tu{f}, Tut{f}
- * arguments of bmTevaMorphosia_chain.m:6-13, applied as bmImDeform.m /
- * bmImDeformT.m do): both prod(n_u) x prod(n_u), referenced, not owned.
- * Tu = Tut = NULL restores the identity. */
-int mona_plan_set_deform(mona_plan P, int frame, mona_sparse Tu, mona_sparse Tut);
+/* Deformation matrices of frame f for the Morphosia solvers, applied as
+ * bmImDeform.m / bmImDeformT.m do (K = []): set 0 holds the Tu{f}, Tut{f}
+ * (or Tu1, Tu1t) arguments of bmTevaMorphosia_chain.m:6-13,
+ * bmTevaDuoMorphosia_chain.m:6-16, bmTevaMorphosia_sheet.m:16-26 and
+ * bmSensitivaMorphosia_chain.m / _sheet.m; set 1 holds Tu2, Tu2t.  Both
+ * prod(n_u) x prod(n_u), referenced, not owned; Tu = Tut = NULL restores
+ * the identity. */
+int mona_plan_set_deform(mona_plan P, int set, int frame, mona_sparse Tu, mona_sparse Tut);
+
+/* The Morphosia solvers over the plan's frames, regul_mode 'normal'.
+ * Frames f = i + nFr1*j of an nFr1 x nFr2 cell array (y(:) order; chains
+ * use nFr1 = nFr).  x [nFr][prod(n_u)] float2; z, u: one block of nFr
+ * images per difference term (z1 then z2), NULL for the Sensitiva solvers;
+ * rho is unused (may be NULL) for them.  Other arguments as mona_teva_chain.
+ *   MONA_MORPH_TEVA_CHAIN      bmTevaMorphosia_chain.m:6-294
+ *   MONA_MORPH_TEVA_DUO_CHAIN  bmTevaDuoMorphosia_chain.m:6-359 (0.5 (Tu1 x_i - x_{i-1}),
+ *                              0.5 (Tu2 x_i - x_{i+1}))
+ *   MONA_MORPH_TEVA_SHEET      bmTevaMorphosia_sheet.m:16-379 (differences along i and j)
+ *   MONA_MORPH_SENSITIVA_CHAIN bmSensitivaMorphosia_chain.m:6-280 (delta HZ ||Tu x_i - x_{i-1}||^2)
+ *   MONA_MORPH_SENSITIVA_SHEET bmSensitivaMorphosia_sheet.m:6-336 (both axes; the first
+ *                              gradient of each iteration carries the penalty without
+ *                              delta, as bmSensitivaMorphosia_sheet.m:118-119 does) */
+#define MONA_MORPH_TEVA_CHAIN 0
+#define MONA_MORPH_TEVA_DUO_CHAIN 1
+#define MONA_MORPH_TEVA_SHEET 2
+#define MONA_MORPH_SENSITIVA_CHAIN 3
+#define MONA_MORPH_SENSITIVA_SHEET 4
+int mona_morphosia(mona_plan P, int solver, int nFr1, void* x, void* z, void* u, int zu_given,
+                   const void* y, const long long* y_off, const float* ve,
+                   const long long* ve_off, float ve_max, const float* delta,
+                   const float* rho, int nIter, int nCGD, double* hist, hipStream_t stream);
 
 /* Gridded reconstruction (bmNasha.m:6-91): n2u with Gn, centred ifft times
  * prod(N_u)*prod(dK_u), times K, crop to n_u; with C != NULL the coil

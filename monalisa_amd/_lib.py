@@ -69,7 +69,8 @@ SIGNATURES = {
     "mona_cfft": (I, [I, IP, I, P, I, P]),
     "mona_sensa": (I, [P, I, P, P, P, F, I, I, DP, P]),
     "mona_steva": (I, [P, I, P, P, P, I, P, P, F, FP, FP, I, I, DP, P]),
-    "mona_plan_set_deform": (I, [P, I, P, P]),
+    "mona_plan_set_deform": (I, [P, I, I, P, P]),
+    "mona_morphosia": (I, [P, I, I, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_teva_chain": (I, [P, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_nasha": (I, [P, P, P, P, P, P]),
     "mona_sensa_batch": (I, [P, I, P, P, LLP, FP, LLP, FP, I, I, DP, P]),

@@ -1,8 +1,9 @@
 // ADMM total-variation solvers and the gridded reconstruction.
 //
 //   bmSteva               spatial TV, one frame   (bmSteva.m:6-138)
-//   bmTevaMorphosia_chain temporal TV over frames, optionally motion-compensated by
-//                         deformation matrices Tu/Tut (bmTevaMorphosia_chain.m:6-294)
+//   Morphosia solvers     temporal TV (Teva) or quadratic (Sensitiva) penalties over
+//                         frame chains and sheets, motion-compensated by deformation
+//                         matrices Tu/Tut (imDim_morphosia/nonCartesian/*.m)
 //   bmNasha               n2u with Gn + centred ifft + deapodisation (bmNasha.m:6-91)
 //
 // The finite differences, their transposes and the soft-threshold prox are
@@ -104,36 +105,56 @@ __global__ void k_prox_update(const float2* __restrict__ Fx, float2* __restrict_
     u[i] = make_float2(w.x - zz.x, w.y - zz.y);
 }
 
-// Temporal difference over cyclic frames: out_f = xa_f - x_{f-1} with
-// xa_f = Tu_f x_f (private_F, bmTevaMorphosia_chain.m:266-272; xa = x for
-// identity deformations).  With z, u: out = (z - u) - (xa_f - x_{f-1}).
+// Frame f = i + n1*j of an n1 x n2 cell array (MATLAB order) moved by d
+// along axis 0 (i) or 1 (j), cyclically.
+__device__ __forceinline__ int frame_shift(int f, int n1, int n2, int axis, int d) {
+    int i = f % n1, j = f / n1;
+    if (axis == 0) i = (i + d + n1) % n1;
+    else j = (j + d + n2) % n2;
+    return i + n1 * j;
+}
+
+struct FrameGrid {
+    int nFr, n1, n2;
+    int64_t nu;
+};
+
+// Difference term of the Morphosia solvers: g = scale * (xa_f - x_shift(f))
+// with xa_f = Tu_f x_f (private_F / private_F1 / private_F2 of
+// bmTevaMorphosia_chain.m:266-272, bmTevaDuoMorphosia_chain.m:313-329,
+// bmTevaMorphosia_sheet.m:322-344; xa = x for identity deformations).
+// mode 0: out = g; 1: out = (z - u) - g (ADMM residual); 2: out = 0 - g.
 __global__ void k_tdiff(const float2* __restrict__ xa, const float2* __restrict__ x, float2* __restrict__ out,
-                        const float2* __restrict__ z, const float2* __restrict__ u, int nFr, int64_t nu) {
+                        const float2* __restrict__ z, const float2* __restrict__ u, FrameGrid G, int axis, int dir,
+                        float scale, int mode) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nFr * nu) return;
-    int f = (int)(t / nu);
-    int64_t i = t - (int64_t)f * nu;
-    int fm = (f + nFr - 1) % nFr;
-    float2 a = xa[t], b = x[(int64_t)fm * nu + i];
+    if (t >= (int64_t)G.nFr * G.nu) return;
+    int f = (int)(t / G.nu);
+    int64_t i = t - (int64_t)f * G.nu;
+    int fm = frame_shift(f, G.n1, G.n2, axis, dir);
+    float2 a = xa[t], b = x[(int64_t)fm * G.nu + i];
     float2 g = make_float2(a.x - b.x, a.y - b.y);
-    if (z) {
+    if (scale != 1.0f) g = make_float2(scale * g.x, scale * g.y);
+    if (mode == 1) {
         float2 zz = z[t], uu = u[t];
         out[t] = make_float2((zz.x - uu.x) - g.x, (zz.y - uu.y) - g.y);
+    } else if (mode == 2) {
+        out[t] = make_float2(0.0f - g.x, 0.0f - g.y);
     } else {
         out[t] = g;
     }
 }
 
-// out_f += coef * (za_f - z_{f+1}), za_f = Tut_f z_f   (private_dagF,
-// bmTevaMorphosia_chain.m:285-291)
+// out_f += coef * (za_f - z_shift(f, -dir)), za_f = Tut_f z_f (private_dagF
+// of the same files: bmTevaMorphosia_chain.m:285-291)
 __global__ void k_tdiffT_acc(const float2* __restrict__ za, const float2* __restrict__ z, float2* __restrict__ out,
-                             int nFr, int64_t nu, float coef) {
+                             FrameGrid G, int axis, int dir, float coef) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nFr * nu) return;
-    int f = (int)(t / nu);
-    int64_t i = t - (int64_t)f * nu;
-    int fp = (f + 1) % nFr;
-    float2 a = za[t], b = z[(int64_t)fp * nu + i];
+    if (t >= (int64_t)G.nFr * G.nu) return;
+    int f = (int)(t / G.nu);
+    int64_t i = t - (int64_t)f * G.nu;
+    int fp = frame_shift(f, G.n1, G.n2, axis, -dir);
+    float2 a = za[t], b = z[(int64_t)fp * G.nu + i];
     float2 o = out[t];
     out[t] = make_float2(o.x + coef * (a.x - b.x), o.y + coef * (a.y - b.y));
 }
@@ -179,8 +200,13 @@ void abs1(const float2* a, int64_t n, double scale, double* out, double* partial
     MONA_HIP(hipGetLastError());
 }
 
-// objective = 0.5 * dafi + 0.5 * delta * regul
-__global__ void k_objective(double* h, double delta) { h[0] = 0.5 * h[1] + 0.5 * delta * h[2]; }
+// regul = sum of the per-term values h[3..3+nT); objective = 0.5 * dafi + 0.5 * delta * regul
+__global__ void k_objective(double* h, double delta, int nT) {
+    double r = h[3];
+    for (int t = 1; t < nT; t++) r += h[3 + t];
+    h[2] = r;
+    h[0] = 0.5 * h[1] + 0.5 * delta * h[2];
+}
 
 // hist[n*ctr .. n*ctr+n-1] = slot[0..n-1]; ctr += 1
 __global__ void k_hist_commit(double* hist, const double* slot, int* ctr, int n) {
@@ -329,12 +355,74 @@ void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const
 }
 
 // ================================================================ bmTevaMorphosia_chain
-void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const float2* y,
-                const long long* y_off, const float* ve, const long long* ve_off, float ve_max,
-                const float* delta_in, const float* rho_in, int nIter, int nCGD, double* hist_out, hipStream_t s) {
+// ================================================================ Morphosia solvers
+// One difference term: F_t x_f = scale (Tu_{set,f} x_f - x_shift(f, axis, dir)).
+struct MorphTerm {
+    int axis, dir;
+    float scale;
+    int set;
+};
+
+struct MorphSpec {
+    int nT;
+    MorphTerm t[2];
+    bool l1;                  // ADMM with soft threshold (Teva) or quadratic penalty (Sensitiva)
+    bool first_dagF_scaled;   // bmSensitivaMorphosia_sheet.m:118-119 leaves delta off
+};
+
+MorphSpec morph_spec(int solver) {
+    MorphSpec m{};
+    m.l1 = true;
+    m.first_dagF_scaled = true;
+    switch (solver) {
+        case MONA_MORPH_TEVA_CHAIN:  // bmTevaMorphosia_chain.m:266-291
+            m.nT = 1;
+            m.t[0] = {0, -1, 1.0f, 0};
+            break;
+        case MONA_MORPH_TEVA_DUO_CHAIN:  // bmTevaDuoMorphosia_chain.m:313-359
+            m.nT = 2;
+            m.t[0] = {0, -1, 0.5f, 0};
+            m.t[1] = {0, +1, 0.5f, 1};
+            break;
+        case MONA_MORPH_TEVA_SHEET:  // bmTevaMorphosia_sheet.m:322-379
+            m.nT = 2;
+            m.t[0] = {0, -1, 1.0f, 0};
+            m.t[1] = {1, -1, 1.0f, 1};
+            break;
+        case MONA_MORPH_SENSITIVA_CHAIN:  // bmSensitivaMorphosia_chain.m:251-277
+            m.nT = 1;
+            m.t[0] = {0, -1, 1.0f, 0};
+            m.l1 = false;
+            break;
+        case MONA_MORPH_SENSITIVA_SHEET:  // bmSensitivaMorphosia_sheet.m
+            m.nT = 2;
+            m.t[0] = {0, -1, 1.0f, 0};
+            m.t[1] = {1, -1, 1.0f, 1};
+            m.l1 = false;
+            m.first_dagF_scaled = false;
+            break;
+        default:
+            MONA_CHECK(false, MONA_ERR_ARG, "unknown Morphosia solver");
+    }
+    return m;
+}
+
+// The Morphosia solvers over the plan's frames, regul_mode 'normal'
+// (src/fourier123/solver_function/imDim_morphosia/nonCartesian/: Teva = ADMM
+// with z_t = F_t x split off and soft-thresholded, Sensitiva = quadratic
+// penalty delta*HZ*||F_t x||^2 minimised by nCGD CG steps per iteration).
+// Frames f = i + nFr1*j of an nFr1 x nFr2 cell array; z, u hold nT blocks of
+// nFr images.  hist: nIter x {objective, dafi, regul}.
+void morphosia(Plan& P, int solver, int nFr1, float2* x, float2* z, float2* u, bool zu_given, const float2* y,
+               const long long* y_off, const float* ve, const long long* ve_off, float ve_max,
+               const float* delta_in, const float* rho_in, int nIter, int nCGD, double* hist_out, hipStream_t s) {
+    const MorphSpec spec = morph_spec(solver);
     int nFr = (int)P.frames.size();
     MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
     MONA_CHECK(ve_max > 0, MONA_ERR_ARG, "ve_max must be > 0");
+    MONA_CHECK(nFr1 >= 1 && nFr % nFr1 == 0, MONA_ERR_ARG, "nFr1 must divide the number of frames");
+    MONA_CHECK(!spec.l1 || (z && u && rho_in), MONA_ERR_ARG, "the ADMM solvers need z, u and rho");
+    const FrameGrid FG{nFr, nFr1, nFr / nFr1, P.nu};
     int64_t nu = P.nu, nx = (int64_t)nFr * nu;
     std::vector<int64_t> Pf(nFr), koff(nFr + 1, 0);
     for (int f = 0; f < nFr; f++) {
@@ -342,13 +430,14 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
         koff[f + 1] = koff[f] + Pf[f] * P.nCh;
     }
     int64_t ny = koff[nFr];
-    std::vector<float> delta = schedule(delta_in, nIter), rho = schedule(rho_in, nIter);
+    std::vector<float> delta = schedule(delta_in, nIter);
+    std::vector<float> rho = spec.l1 ? schedule(rho_in, nIter) : delta;  // weight of the penalty terms
     Work& w = P.work;
     w.hy.ensure((size_t)(ny / P.nCh) * sizeof(float));
     w.k1.ensure((size_t)ny * sizeof(float2));
     w.k2.ensure((size_t)ny * sizeof(float2));
-    w.z1.ensure((size_t)nx * sizeof(float2));
-    w.z2.ensure((size_t)nx * sizeof(float2));
+    w.z1.ensure((size_t)nx * spec.nT * sizeof(float2));
+    w.z2.ensure((size_t)nx * spec.nT * sizeof(float2));
     w.z3.ensure((size_t)nx * sizeof(float2));
     w.z4.ensure((size_t)nx * sizeof(float2));
     w.sc.ensure(sizeof(CgScalars));
@@ -356,7 +445,11 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
     w.hist.ensure((size_t)std::max(nIter, 1) * 3 * sizeof(double));
     float* hy = w.hy.as<float>();
     float2 *res_y = w.k1.as<float2>(), *Mp = w.k2.as<float2>();
-    float2 *res_z = w.z1.as<float2>(), *Fp = w.z2.as<float2>(), *dag = w.z3.as<float2>(), *p = w.z4.as<float2>();
+    float2 *dag = w.z3.as<float2>(), *p = w.z4.as<float2>();
+    float2* res_z[2] = {w.z1.as<float2>(), w.z1.as<float2>() + nx};  // per term
+    float2* Fp[2] = {w.z2.as<float2>(), w.z2.as<float2>() + nx};
+    float2* zt[2] = {z, z ? z + nx : nullptr};
+    float2* ut[2] = {u, u ? u + nx : nullptr};
     CgScalars* sc = w.sc.as<CgScalars>();
     double* part = w.partial.as<double>();
     double* hist = w.hist.as<double>();
@@ -375,29 +468,37 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
     auto yF = [&](int f) { return ysorted + koff[f]; };
     // deformations: Tu_f x_f and Tut_f z_f go to a scratch image set, frames
     // without a deformation are copied
-    bool deform = false;
-    for (int f = 0; f < nFr && f < (int)P.deform.size(); f++) deform = deform || P.deform[f].first;
-    if (deform) w.z6.ensure((size_t)nx * sizeof(float2));
-    float2* tx = deform ? w.z6.as<float2>() : nullptr;
-    auto warp = [&](const float2* v, bool transposed, hipStream_t s) -> const float2* {
-        if (!deform) return v;
+    auto T_of = [&](int set, int f, bool transposed) -> const SparseDev* {
+        if (f >= (int)P.deform[set].size()) return nullptr;
+        return transposed ? P.deform[set][f].second : P.deform[set][f].first;
+    };
+    bool any_deform = false;
+    for (int t = 0; t < spec.nT; t++)
+        for (int f = 0; f < nFr; f++) any_deform = any_deform || T_of(spec.t[t].set, f, false);
+    if (any_deform) w.z6.ensure((size_t)nx * sizeof(float2));
+    float2* tx = any_deform ? w.z6.as<float2>() : nullptr;
+    auto warp = [&](int set, const float2* v, bool transposed, hipStream_t s) -> const float2* {
+        bool mine = false;
+        for (int f = 0; f < nFr; f++) mine = mine || T_of(set, f, false);
+        if (!mine) return v;
         for (int f = 0; f < nFr; f++) {
-            const SparseDev* T = f < (int)P.deform.size()
-                                     ? (transposed ? P.deform[f].second : P.deform[f].first) : nullptr;
+            const SparseDev* T = T_of(set, f, transposed);
             if (T) spmv_c2_rowmajor(*T, v + (int64_t)f * nu, tx + (int64_t)f * nu, 1, nullptr, s);
             else vcopy(v + (int64_t)f * nu, tx + (int64_t)f * nu, nu, s);
         }
         return tx;
     };
-    // F v (with z, u: (z - u) - F v) and out += coef * F^T-part of dagF
-    auto Fop = [&](const float2* v, float2* out, const float2* zz, const float2* uu, hipStream_t s) {
-        const float2* va = warp(v, false, s);
-        k_tdiff<<<nb(nx), 256, 0, s>>>(va, v, out, zz, uu, nFr, nu);
+    auto Fop = [&](int t, const float2* v, float2* out, int mode, float scale, hipStream_t s) {
+        const MorphTerm& m = spec.t[t];
+        const float2* va = warp(m.set, v, false, s);
+        k_tdiff<<<nb(nx), 256, 0, s>>>(va, v, out, mode == 1 ? zt[t] : nullptr, mode == 1 ? ut[t] : nullptr, FG,
+                                       m.axis, m.dir, scale, mode);
         MONA_HIP(hipGetLastError());
     };
-    auto FtAcc = [&](const float2* zz, float2* out, float coef, hipStream_t s) {
-        const float2* za = warp(zz, true, s);
-        k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(za, zz, out, nFr, nu, coef);
+    auto FtAcc = [&](int t, const float2* zz, float2* out, float coef, hipStream_t s) {
+        const MorphTerm& m = spec.t[t];
+        const float2* za = warp(m.set, zz, true, s);
+        k_tdiffT_acc<<<nb(nx), 256, 0, s>>>(za, zz, out, FG, m.axis, m.dir, coef * m.scale);
         MONA_HIP(hipGetLastError());
     };
     // operators: one launch per gridding op over all frames (batched form), or
@@ -413,53 +514,60 @@ void teva_chain(Plan& P, float2* x, float2* z, float2* u, bool zu_given, const f
         for (int f = 0; f < nFr; f++)
             P.adjoint(f, r + koff[f], out + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, cs);
     };
-    if (!zu_given) {
-        Fop(x, z, nullptr, nullptr, s);
-        fill_f32(reinterpret_cast<float*>(u), 0.0f, nx * 2, s);
+    if (spec.l1 && !zu_given) {
+        for (int t = 0; t < spec.nT; t++) Fop(t, x, zt[t], 0, spec.t[t].scale, s);
+        fill_f32(reinterpret_cast<float*>(u), 0.0f, nx * 2 * spec.nT, s);
     }
     // monitoring values go to a fixed slot and are committed to hist[3*iter]
     // by a kernel that reads and advances a device counter, so the body has
     // no per-iteration pointers and can be replayed from a graph
-    w.z5.ensure(4 * sizeof(double) + sizeof(int));
+    w.z5.ensure(6 * sizeof(double) + sizeof(int));
     double* hslot = w.z5.as<double>();
-    int* hctr = reinterpret_cast<int*>(hslot + 4);
+    int* hctr = reinterpret_cast<int*>(hslot + 6);
     fill_i32(hctr, 0, 1, s);
     auto body = [&](float dl, float rc) {
         hipStream_t s = cs;
+        const float wz = rc * invHX * HZ;   // penalty weight times 1/HX times HZ
         cg_reset(sc, s);
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
-        Fop(x, res_z, z, u, s);
+        for (int t = 0; t < spec.nT; t++) Fop(t, x, res_z[t], spec.l1 ? 1 : 2, spec.t[t].scale, s);
         Mh(res_y, dag);
-        FtAcc(res_z, dag, rc * invHX * HZ, s);
+        for (int t = 0; t < spec.nT; t++) FtAcc(t, res_z[t], dag, spec.first_dagF_scaled ? wz : invHX * HZ, s);
         vcopy(dag, p, nx, s);
         sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn, false, part, s);
         for (int j = 1; j <= nCGD; j++) {
             M(p, Mp);
-            Fop(p, Fp, nullptr, nullptr, s);
+            for (int t = 0; t < spec.nT; t++) Fop(t, p, Fp[t], 0, spec.t[t].scale, s);
             sqnorm(Mp, ny, hy, P.nCh, 1.0, &sc->den, false, part, s);
-            sqnorm(Fp, nx, nullptr, 1, (double)rc * HZ, &sc->den, true, part, s);
+            for (int t = 0; t < spec.nT; t++) sqnorm(Fp[t], nx, nullptr, 1, (double)rc * HZ, &sc->den, true, part, s);
             cg_alpha(sc, s);
             vaxpy(x, p, &sc->a, 1.0f, nx, s);
             if (j == nCGD) break;
             vaxpy(res_y, Mp, &sc->a, -1.0f, ny, s);
-            vaxpy(res_z, Fp, &sc->a, -1.0f, nx, s);
+            for (int t = 0; t < spec.nT; t++) vaxpy(res_z[t], Fp[t], &sc->a, -1.0f, nx, s);
             Mh(res_y, dag);
-            FtAcc(res_z, dag, rc * invHX * HZ, s);
+            for (int t = 0; t < spec.nT; t++) FtAcc(t, res_z[t], dag, wz, s);
             sqnorm(dag, nx, nullptr, 1, P.HX, &sc->sqn_new, false, part, s);
             cg_beta(sc, s);
             vxpby(p, dag, &sc->b, nx, s);
         }
-        Fop(x, Fp, nullptr, nullptr, s);
-        k_prox_update<<<nb(nx), 256, 0, s>>>(Fp, z, u, nx, dl / rc);
-        MONA_HIP(hipGetLastError());
-        // monitoring: dafi, regul (private_dafi_regul), objective
+        if (spec.l1)
+            for (int t = 0; t < spec.nT; t++) {
+                Fop(t, x, Fp[t], 0, spec.t[t].scale, s);
+                k_prox_update<<<nb(nx), 256, 0, s>>>(Fp[t], zt[t], ut[t], nx, dl / rc);
+                MONA_HIP(hipGetLastError());
+            }
+        // monitoring: dafi, regul of the unscaled differences (private_dafi_regul), objective
         M(x, Mp);
         for (int f = 0; f < nFr; f++) vsub(yF(f), Mp + koff[f], res_y + koff[f], koff[f + 1] - koff[f], s);
         sqnorm(res_y, ny, hy, P.nCh, 1.0, hslot + 1, false, part, s);
-        Fop(x, Fp, nullptr, nullptr, s);
-        abs1(Fp, nx, (double)HZ, hslot + 2, part, s);
-        k_objective<<<1, 1, 0, s>>>(hslot, (double)dl);
+        for (int t = 0; t < spec.nT; t++) {
+            Fop(t, x, Fp[0], 0, 1.0f, s);
+            if (spec.l1) abs1(Fp[0], nx, (double)HZ, hslot + 3 + t, part, s);
+            else sqnorm(Fp[0], nx, nullptr, 1, (double)HZ, hslot + 3 + t, false, part, s);
+        }
+        k_objective<<<1, 1, 0, s>>>(hslot, (double)dl, spec.nT);
         MONA_HIP(hipGetLastError());
         k_hist_commit<<<1, 1, 0, s>>>(hist, hslot, hctr, 3);
         MONA_HIP(hipGetLastError());
@@ -563,14 +671,26 @@ int mona_teva_chain(mona_plan P, void* x, void* z, void* u, int zu_given, const 
                     int nIter, int nCGD, double* hist, hipStream_t stream) {
     return guarded([&] {
         MONA_CHECK(P && x && z && u && y && y_off && ve && ve_off && delta && rho, MONA_ERR_ARG, "NULL argument");
-        teva_chain(P->P, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u), zu_given != 0,
-                   static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD, hist, stream);
+        morphosia(P->P, MONA_MORPH_TEVA_CHAIN, (int)P->P.frames.size(), static_cast<float2*>(x),
+                  static_cast<float2*>(z), static_cast<float2*>(u), zu_given != 0, static_cast<const float2*>(y),
+                  y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD, hist, stream);
     });
 }
 
-int mona_plan_set_deform(mona_plan P, int frame, mona_sparse Tu, mona_sparse Tut) {
+int mona_morphosia(mona_plan P, int solver, int nFr1, void* x, void* z, void* u, int zu_given, const void* y,
+                   const long long* y_off, const float* ve, const long long* ve_off, float ve_max,
+                   const float* delta, const float* rho, int nIter, int nCGD, double* hist, hipStream_t stream) {
     return guarded([&] {
-        MONA_CHECK(P && frame >= 0, MONA_ERR_ARG, "bad argument");
+        MONA_CHECK(P && x && y && y_off && ve && ve_off && delta, MONA_ERR_ARG, "NULL argument");
+        morphosia(P->P, solver, nFr1, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u),
+                  zu_given != 0, static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD,
+                  hist, stream);
+    });
+}
+
+int mona_plan_set_deform(mona_plan P, int set, int frame, mona_sparse Tu, mona_sparse Tut) {
+    return guarded([&] {
+        MONA_CHECK(P && frame >= 0 && (set == 0 || set == 1), MONA_ERR_ARG, "bad argument");
         MONA_CHECK((Tu == nullptr) == (Tut == nullptr), MONA_ERR_ARG, "Tu and Tut are given together");
         Plan& pl = P->P;
         if (Tu) {
@@ -586,8 +706,9 @@ int mona_plan_set_deform(mona_plan P, int frame, mona_sparse Tu, mona_sparse Tut
             sparse_upload(Tu->S, hc.stream);
             sparse_upload(Tut->S, hc.stream);
         }
-        if ((int)pl.deform.size() <= frame) pl.deform.resize(frame + 1, {nullptr, nullptr});
-        pl.deform[frame] = {Tu ? &Tu->S : nullptr, Tut ? &Tut->S : nullptr};
+        auto& d = pl.deform[set];
+        if ((int)d.size() <= frame) d.resize(frame + 1, {nullptr, nullptr});
+        d[frame] = {Tu ? &Tu->S : nullptr, Tut ? &Tut->S : nullptr};
     });
 }
 

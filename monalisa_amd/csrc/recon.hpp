@@ -55,7 +55,7 @@ struct Work {
     DevBuf hy, k1, k2, i1, i2, i3, i4, sc, partial, hist;
     DevBuf ys, hy0;             // k-space data and weights in tile order
     DevBuf kpub;                // staging of the public operator calls
-    DevBuf z1, z2, z3, z4, z5, z6;  // ADMM image-space buffers
+    DevBuf z1, z2, z3, z4, z5, z6;  // ADMM image-space buffers (z1, z2: one block per term)
 };
 
 struct Plan {
@@ -69,9 +69,9 @@ struct Plan {
     DevBuf K, KFC, KFCc, grid;
     DevBuf fft_a, fft_b;  // pruned-pass intermediates
     std::vector<Frame> frames;
-    // per-frame deformation Tu (image of frame f -> its neighbour's grid) and
-    // Tut; frames without one use the identity
-    std::vector<std::pair<const SparseDev*, const SparseDev*>> deform;
+    // per-frame deformations (Tu, Tut) of the solvers' two difference terms;
+    // frames without one use the identity
+    std::vector<std::pair<const SparseDev*, const SparseDev*>> deform[2];
     Work work;
 
     void init(int imDim, const int* N, const int* n, const double* dK, int nCh, int ktype, int nWin,

@@ -65,14 +65,24 @@ class Plan:
         check(lib.mona_plan_set_frame(self._h, int(f), Gu.handle(), Gut.handle()))
         self._frames[f] = (Gu, Gut)   # keep the handles alive
 
-    def set_deform(self, f, Tu: BmSparseMat = None, Tut: BmSparseMat = None):
-        """Deformation matrices of frame f for the temporal-TV solver (the
-        Tu{f}, Tut{f} of bmTevaMorphosia_chain.m:6-13); None = identity."""
+    def set_deform(self, f, Tu: BmSparseMat = None, Tut: BmSparseMat = None, set=0):
+        """Deformation matrices of frame f for the Morphosia solvers: set 0
+        holds Tu{f}, Tut{f} (Tu1, Tu1t), set 1 holds Tu2, Tu2t
+        (bmTevaMorphosia_chain.m:6-13, bmTevaDuoMorphosia_chain.m:6-16);
+        None = identity."""
         if (Tu is None) != (Tut is None):
             raise ValueError("Tu and Tut are given together")
-        check(lib.mona_plan_set_deform(self._h, int(f), Tu.handle() if Tu is not None else None,
+        check(lib.mona_plan_set_deform(self._h, int(set), int(f), Tu.handle() if Tu is not None else None,
                                        Tut.handle() if Tut is not None else None))
-        self._deform[f] = (Tu, Tut)   # keep the handles alive
+        self._deform[(set, f)] = (Tu, Tut)   # keep the handles alive
+
+    def _set_deforms(self, nFr, set, Tu, Tut):
+        if Tu is None and Tut is None:
+            return
+        Tu = list(Tu) if Tu is not None else [None] * nFr
+        Tut = list(Tut) if Tut is not None else [None] * nFr
+        for f in range(nFr):
+            self.set_deform(f, Tu[f], Tut[f], set=set)
 
     def nPt(self, f=0):
         return self._frames[f][0].l_size
@@ -239,11 +249,7 @@ class Plan:
         the ones set by set_deform (none: identity, Tu = [])."""
         import torch
         nFr = x.shape[0]
-        if Tu is not None or Tut is not None:
-            Tu = list(Tu) if Tu is not None else [None] * nFr
-            Tut = list(Tut) if Tut is not None else [None] * nFr
-            for f in range(nFr):
-                self.set_deform(f, Tu[f], Tut[f])
+        self._set_deforms(nFr, 0, Tu, Tut)
         given = z is not None
         if z is None:
             z = torch.zeros_like(x)
@@ -262,6 +268,86 @@ class Plan:
                                   int(nIter), int(nCGD), hist, stream_of(stream)))
         h = np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
         return z, u, h
+
+    # -- the other Morphosia solvers (mona_morphosia) ------------------------
+    MORPH_TEVA_CHAIN, MORPH_TEVA_DUO_CHAIN, MORPH_TEVA_SHEET = 0, 1, 2
+    MORPH_SENSITIVA_CHAIN, MORPH_SENSITIVA_SHEET = 3, 4
+
+    def _morph(self, solver, nFr1, x, zs, us, y_list, ve_list, delta, rho, nCGD, ve_max, nIter, stream, nT):
+        import torch
+        nFr = x.shape[0]
+        if nFr1 is None:
+            nFr1 = nFr
+        l1 = rho is not None
+        z = u = None
+        given = False
+        if l1:
+            given = zs is not None and all(v is not None for v in zs)
+            z = torch.stack(list(zs)).contiguous() if given else torch.zeros((nT,) + tuple(x.shape), dtype=x.dtype,
+                                                                            device=x.device)
+            u = torch.stack(list(us)).contiguous() if given else torch.zeros_like(z)
+        ycat = torch.cat([yy.reshape(-1) for yy in y_list])
+        vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
+        yo = np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]
+        vo = np.cumsum([0] + [vv.numel() for vv in ve_list])[:-1]
+        yoff = (C.c_longlong * nFr)(*[int(v) for v in yo])
+        voff = (C.c_longlong * nFr)(*[int(v) for v in vo])
+        d = (C.c_float * 2)(*_sched(delta))
+        r = (C.c_float * 2)(*_sched(rho)) if l1 else None
+        hist = (C.c_double * (3 * max(nIter, 1)))()
+        check(lib.mona_morphosia(self._h, int(solver), int(nFr1), ptr(x), ptr(z), ptr(u), 1 if given else 0,
+                                 ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), d, r,
+                                 int(nIter), int(nCGD), hist, stream_of(stream)))
+        h = np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
+        if not l1:
+            return h
+        return [z[k] for k in range(nT)], [u[k] for k in range(nT)], h
+
+    def bmTevaDuoMorphosia_chain(self, x, z1, z2, u1, u2, y_list, ve_list, delta, rho, nCGD, ve_max, nIter=1,
+                                 Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+        """bmTevaDuoMorphosia_chain.m:6-242: two half-weighted temporal
+        differences, 0.5 (Tu1_i x_i - x_{i-1}) and 0.5 (Tu2_i x_i - x_{i+1}).
+        x [nFr, nu] complex64 is updated in place; z1..u2 None start from
+        z = F x, u = 0.  Returns (z1, z2, u1, u2, hist)."""
+        nFr = x.shape[0]
+        self._set_deforms(nFr, 0, Tu1, Tu1t)
+        self._set_deforms(nFr, 1, Tu2, Tu2t)
+        zs, us, h = self._morph(self.MORPH_TEVA_DUO_CHAIN, None, x, None if z1 is None else (z1, z2),
+                                None if u1 is None else (u1, u2), y_list, ve_list, delta, rho, nCGD, ve_max, nIter,
+                                stream, 2)
+        return zs[0], zs[1], us[0], us[1], h
+
+    def bmTevaMorphosia_sheet(self, x, z1, z2, u1, u2, y_list, ve_list, nFr_1, delta, rho, nCGD, ve_max, nIter=1,
+                              Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+        """bmTevaMorphosia_sheet.m:16-244: frames on an nFr_1 x nFr_2 cell
+        array, frame k = i + nFr_1*j (the y(:) order of x, y_list, ve_list
+        and the deformation lists); differences along i (Tu1) and j (Tu2).
+        Returns (z1, z2, u1, u2, hist)."""
+        nFr = x.shape[0]
+        self._set_deforms(nFr, 0, Tu1, Tu1t)
+        self._set_deforms(nFr, 1, Tu2, Tu2t)
+        zs, us, h = self._morph(self.MORPH_TEVA_SHEET, nFr_1, x, None if z1 is None else (z1, z2),
+                                None if u1 is None else (u1, u2), y_list, ve_list, delta, rho, nCGD, ve_max, nIter,
+                                stream, 2)
+        return zs[0], zs[1], us[0], us[1], h
+
+    def bmSensitivaMorphosia_chain(self, x, y_list, ve_list, delta, nCGD, ve_max, nIter=1, Tu=None, Tut=None,
+                                   stream=None):
+        """bmSensitivaMorphosia_chain.m:6-184: quadratic temporal penalty
+        delta*HZ*||Tu_i x_i - x_{i-1}||^2; x updated in place; returns hist."""
+        self._set_deforms(x.shape[0], 0, Tu, Tut)
+        return self._morph(self.MORPH_SENSITIVA_CHAIN, None, x, None, None, y_list, ve_list, delta, None, nCGD,
+                           ve_max, nIter, stream, 1)
+
+    def bmSensitivaMorphosia_sheet(self, x, y_list, ve_list, nFr_1, delta, nCGD, ve_max, nIter=1, Tu1=None,
+                                   Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+        """bmSensitivaMorphosia_sheet.m:6-187: the quadratic penalty along
+        both cell-array axes (frame k = i + nFr_1*j); returns hist."""
+        nFr = x.shape[0]
+        self._set_deforms(nFr, 0, Tu1, Tu1t)
+        self._set_deforms(nFr, 1, Tu2, Tu2t)
+        return self._morph(self.MORPH_SENSITIVA_SHEET, nFr_1, x, None, None, y_list, ve_list, delta, None, nCGD,
+                           ve_max, nIter, stream, 2)
 
     def bmNasha(self, y, Gn: BmSparseMat, Cmaps=None, stream=None):
         """Gridded recon (bmNasha.m:6-91)."""

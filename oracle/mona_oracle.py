@@ -761,73 +761,201 @@ def steva(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
     return x, z, u, hist
 
 
-def teva_chain(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
-               ve_max, nIter=1, kernel=(None, None, None), Tu=None, Tut=None):
-    """Temporal-TV ADMM over nFr cyclic frames, regul_mode 'normal'
-    (src/fourier123/solver_function/imDim_morphosia/nonCartesian/
-    bmTevaMorphosia_chain.m:6-294), float64.
-    x, y, ve, Gu, Gut, z, u: lists over frames.  Tu, Tut: lists of
-    deformation matrices (entries None = identity, bmImDeform.m:6-10):
-    F x_i = Tu_i x_i - x_{i-1} (private_F, :266-272) and
-    dagF z_i = (1/HX)(Tut_i (HZ z_i) - HZ z_{i+1}) (private_dagF, :285-291)."""
+def _frame_shift(f, n1, n2, axis, d):
+    """Frame f = i + n1*j of an n1 x n2 cell array (MATLAB order) moved by d
+    along axis 0 (i) or 1 (j), cyclically."""
+    i, j = f % n1, f // n1
+    if axis == 0:
+        i = (i + d) % n1
+    else:
+        j = (j + d) % n2
+    return i + n1 * j
+
+
+def _schedule(v, nIter):
+    """private_init_regul_param (bmTevaMorphosia_chain.m:226-235)."""
+    return np.broadcast_to(np.atleast_1d(np.abs(v)), (nIter,)) if np.size(v) == 1 \
+        else np.linspace(abs(v[0]), abs(v[1]), nIter)
+
+
+def morph_terms(terms, nFr, n1, HX, HZ):
+    """Per term: the unscaled difference Tu_f x_f - x_shift(f, dir), F (its
+    scaled form) and dagF (private_F / private_dagF of the Morphosia
+    solvers).  Images are n_u arrays, flattened column-major for Tu."""
+    n2 = nFr // n1
+    deform = lambda G, a: a if G is None else (G @ a.reshape(-1, order='F')).reshape(a.shape, order='F')
+    out = []
+    for t in terms:
+        Tu = [None] * nFr if t.get('Tu') is None else [None if G is None else to_scipy(G) for G in t['Tu']]
+        Tut = [None] * nFr if t.get('Tut') is None else [None if G is None else to_scipy(G) for G in t['Tut']]
+        fw = [_frame_shift(f, n1, n2, t['axis'], t['dir']) for f in range(nFr)]
+        bw = [_frame_shift(f, n1, n2, t['axis'], -t['dir']) for f in range(nFr)]
+        sc = float(t.get('scale', 1.0))
+        d = (lambda Tu, fw: lambda v: [deform(Tu[i], v[i]) - v[fw[i]] for i in range(nFr)])(Tu, fw)
+        F = (lambda d, sc: lambda v: [sc * e for e in d(v)])(d, sc)
+        FT = (lambda Tut, bw, sc: lambda w: [(sc / HX) * (deform(Tut[i], HZ * w[i]) - HZ * w[bw[i]])
+                                             for i in range(nFr)])(Tut, bw, sc)
+        out.append(dict(diff=d, F=F, FT=FT))
+    return out
+
+
+def morphosia(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nIter, kernel,
+              terms, l1=True, nFr1=None, first_dagF_scaled=True):
+    """Shared body of the Morphosia solvers (src/fourier123/solver_function/
+    imDim_morphosia/nonCartesian/bmTevaMorphosia_chain.m, bmTevaDuoMorphosia_chain.m,
+    bmTevaMorphosia_sheet.m, bmSensitivaMorphosia_chain.m, bmSensitivaMorphosia_sheet.m),
+    regul_mode 'normal', float64.
+
+    Frames f = i + nFr1*j of an nFr1 x nFr2 cell array.  Term t has
+    F_t x_f = s_t (Tu_t,f x_f - x_shift(f, axis_t, dir_t)) and
+    dagF_t z_f = (s_t/HX)(Tut_t,f (HZ z_f) - HZ z_shift(f, axis_t, -dir_t)).
+    l1: ADMM with z_t, u_t and the soft threshold (Teva); else the quadratic
+    penalty delta*||F x||^2 solved by restarted CG (Sensitiva: z = u = 0,
+    delta in place of rho).  regul sums |.| (l1) or |.|^2 of the unscaled
+    differences Tu x_f - x_shift.  first_dagF_scaled False leaves the
+    penalty weight off the first gradient of each outer iteration
+    (bmSensitivaMorphosia_sheet.m:118-119 against :164-165)."""
     eps = 10 * np.finfo(np.float32).eps
     nFr = len(y)
+    n1 = nFr if nFr1 is None else nFr1
     KFC = kf(C, N_u, n_u, dK_u, *kernel)
     KFCc = kf_conj(np.conj(C), N_u, n_u, dK_u, *kernel)
     HX = hx_of(N_u, dK_u)
     HZ = HX
     HY = [np.minimum(np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1, 1), yy.shape),
                      ve_max) for v, yy in zip(ve, y)]
-    delta = np.broadcast_to(np.atleast_1d(np.abs(delta)), (nIter,)) if np.size(delta) == 1 \
-        else np.linspace(abs(delta[0]), abs(delta[1]), nIter)
-    rho = np.broadcast_to(np.atleast_1d(np.abs(rho)), (nIter,)) if np.size(rho) == 1 \
-        else np.linspace(abs(rho[0]), abs(rho[1]), nIter)
+    delta = _schedule(delta, nIter)
+    rho = _schedule(rho, nIter) if l1 else None
     x = [np.array(v, dtype=np.complex128) for v in x]
     M = lambda v: [shanna(v[i], Gu[i], KFC, N_u, n_u) for i in range(nFr)]
     Mh = lambda r: [(1.0 / HX) * nakatsha(HY[i] * r[i], Gut[i], KFCc, True, N_u, n_u)
                     for i in range(nFr)]
-    Tu = [None] * nFr if Tu is None else [None if T is None else to_scipy(T) for T in Tu]
-    Tut = [None] * nFr if Tut is None else [None if T is None else to_scipy(T) for T in Tut]
-    deform = lambda T, a: a if T is None else (T @ a.reshape(-1, order='F')).reshape(a.shape, order='F')
-    Fop = lambda v: [deform(Tu[i], v[i]) - v[(i - 1) % nFr] for i in range(nFr)]
-    FT = lambda w: [(1.0 / HX) * (deform(Tut[i], HZ * w[i]) - HZ * w[(i + 1) % nFr]) for i in range(nFr)]
+    T = morph_terms(terms, nFr, n1, HX, HZ)
+    diff = lambda t, v: t['diff'](v)
+    Fop = lambda t, v: t['F'](v)
+    FT = lambda t, w: t['FT'](w)
     sq = lambda a, H: sum(float(np.real(np.vdot(ai, (H[i] if isinstance(H, list) else H) * ai)))
                           for i, ai in enumerate(a))
-    z = Fop(x) if z is None else [np.array(v, dtype=np.complex128) for v in z]
-    u = [np.zeros_like(v) for v in x] if u is None else [np.array(v, dtype=np.complex128) for v in u]
+    add = lambda a, b: [p + q for p, q in zip(a, b)]
+    if l1:
+        z = [Fop(t, x) for t in T] if z is None else [[np.array(v, dtype=np.complex128) for v in zt] for zt in z]
+        u = [[np.zeros_like(v) for v in x] for _ in T] if u is None else \
+            [[np.array(v, dtype=np.complex128) for v in ut] for ut in u]
     hist = []
     for c in range(nIter):
-        dl, rc = delta[c], rho[c]
+        dl = delta[c]
+        w = rho[c] if l1 else dl                   # weight of the penalty terms
         ry = [yy - m for yy, m in zip(y, M(x))]
-        Fx = Fop(x)
-        rz = [(z[i] - u[i]) - Fx[i] for i in range(nFr)]
-        dag = [a + rc * b for a, b in zip(Mh(ry), FT(rz))]
+        if l1:
+            rz = [[(z[k][i] - u[k][i]) - f for i, f in enumerate(Fop(t, x))] for k, t in enumerate(T)]
+        else:
+            rz = [[0 - f for f in Fop(t, x)] for t in T]
+        w0 = w if first_dagF_scaled else 1.0
+        dag = Mh(ry)
+        pen = None
+        for k, t in enumerate(T):
+            g = [w0 * v for v in FT(t, rz[k])]
+            pen = g if pen is None else add(pen, g)
+        dag = add(dag, pen)
         p = dag
         sqn = sq(dag, HX)
         for j in range(1, nCGD + 1):
             if sqn < eps:
                 break
             Mp = M(p)
-            Fp = Fop(p)
-            den = sq(Mp, HY) + sq(Fp, rc * HZ)
+            Fp = [Fop(t, p) for t in T]
+            den = sq(Mp, HY) + sum(sq(Fp[k], w * HZ) for k in range(len(T)))
             a = sqn / den
             x = [xi + a * pi for xi, pi in zip(x, p)]
             if j == nCGD:
                 break
             ry = [r - a * m for r, m in zip(ry, Mp)]
-            rz = [r - a * f for r, f in zip(rz, Fp)]
-            dag = [a_ + rc * b_ for a_, b_ in zip(Mh(ry), FT(rz))]
+            rz = [[r - a * f for r, f in zip(rz[k], Fp[k])] for k in range(len(T))]
+            dag = Mh(ry)
+            pen = None
+            for k, t in enumerate(T):
+                g = [w * v for v in FT(t, rz[k])]
+                pen = g if pen is None else add(pen, g)
+            dag = add(dag, pen)
             sqn_new = sq(dag, HX)
             b = sqn_new / sqn
             sqn = sqn_new
             p = [dg + b * pi for dg, pi in zip(dag, p)]
-        Fx_u = [ui + fi for ui, fi in zip(u, Fop(x))]
-        z = [prox_one_norm(v, dl / rc) for v in Fx_u]
-        u = [a_ - b_ for a_, b_ in zip(Fx_u, z)]
+        if l1:
+            for k, t in enumerate(T):
+                Fx_u = [ui + fi for ui, fi in zip(u[k], Fop(t, x))]
+                z[k] = [prox_one_norm(v, dl / w) for v in Fx_u]
+                u[k] = [a_ - b_ for a_, b_ in zip(Fx_u, z[k])]
         dafi = sq([yy - m for yy, m in zip(y, M(x))], HY)
-        regul = sum(HZ * (np.abs(f.real).sum() + np.abs(f.imag).sum()) for f in Fop(x))
+        if l1:
+            regul = sum(HZ * (np.abs(d.real).sum() + np.abs(d.imag).sum()) for t in T for d in diff(t, x))
+        else:
+            regul = sum(sq(diff(t, x), HZ) for t in T)
         hist.append((0.5 * dafi + 0.5 * dl * regul, dafi, regul))
-    return x, z, u, hist
+    return x, (z if l1 else None), (u if l1 else None), hist
+
+
+def teva_chain(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD,
+               ve_max, nIter=1, kernel=(None, None, None), Tu=None, Tut=None):
+    """bmTevaMorphosia_chain (src/fourier123/solver_function/imDim_morphosia/
+    nonCartesian/bmTevaMorphosia_chain.m:6-294): one term,
+    F x_i = Tu_i x_i - x_{i-1} (private_F, :266-272),
+    dagF z_i = (1/HX)(Tut_i (HZ z_i) - HZ z_{i+1}) (private_dagF, :285-291);
+    Tu entries None = identity (bmImDeform.m:6-10).  Lists over frames."""
+    xo, zo, uo, h = morphosia(x, None if z is None else [z], None if u is None else [u], y, ve, C, Gu, Gut,
+                              N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nIter, kernel,
+                              [dict(axis=0, dir=-1, scale=1.0, Tu=Tu, Tut=Tut)])
+    return xo, zo[0], uo[0], h
+
+
+def teva_duo_chain(x, z1, z2, u1, u2, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nIter=1,
+                   kernel=(None, None, None), Tu1=None, Tu1t=None, Tu2=None, Tu2t=None):
+    """bmTevaDuoMorphosia_chain (.../bmTevaDuoMorphosia_chain.m:6-242): two
+    terms, F1 x_i = 0.5 (Tu1_i x_i - x_{i-1}) (:313-320),
+    F2 x_i = 0.5 (Tu2_i x_i - x_{i+1}) (:322-329), adjoints with 0.5/HX
+    (:340-359); regul of the unscaled differences (:254-271)."""
+    z = None if z1 is None else [z1, z2]
+    u = None if u1 is None else [u1, u2]
+    xo, zo, uo, h = morphosia(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nIter, kernel,
+                              [dict(axis=0, dir=-1, scale=0.5, Tu=Tu1, Tut=Tu1t),
+                               dict(axis=0, dir=+1, scale=0.5, Tu=Tu2, Tut=Tu2t)])
+    return xo, zo, uo, h
+
+
+def teva_sheet(x, z1, z2, u1, u2, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nFr1, nIter=1,
+               kernel=(None, None, None), Tu1=None, Tu1t=None, Tu2=None, Tu2t=None):
+    """bmTevaMorphosia_sheet (.../bmTevaMorphosia_sheet.m:16-244): frames on
+    an nFr1 x nFr2 cell array (lists in y(:) order), F1 along i (:322-332),
+    F2 along j (:334-344), both x_{.} - previous, scale 1."""
+    z = None if z1 is None else [z1, z2]
+    u = None if u1 is None else [u1, u2]
+    xo, zo, uo, h = morphosia(x, z, u, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, rho, nCGD, ve_max, nIter, kernel,
+                              [dict(axis=0, dir=-1, scale=1.0, Tu=Tu1, Tut=Tu1t),
+                               dict(axis=1, dir=-1, scale=1.0, Tu=Tu2, Tut=Tu2t)], nFr1=nFr1)
+    return xo, zo, uo, h
+
+
+def sensitiva_chain(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, nCGD, ve_max, nIter=1,
+                    kernel=(None, None, None), Tu=None, Tut=None):
+    """bmSensitivaMorphosia_chain (.../bmSensitivaMorphosia_chain.m:6-184):
+    quadratic temporal penalty delta*HZ*||Tu_i x_i - x_{i-1}||^2, nCGD CG
+    steps per outer iteration restarted from the current x (:91-171)."""
+    xo, _, _, h = morphosia(x, None, None, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, None, nCGD, ve_max, nIter,
+                            kernel, [dict(axis=0, dir=-1, scale=1.0, Tu=Tu, Tut=Tut)], l1=False)
+    return xo, h
+
+
+def sensitiva_sheet(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, nCGD, ve_max, nFr1, nIter=1,
+                    kernel=(None, None, None), Tu1=None, Tu1t=None, Tu2=None, Tu2t=None):
+    """bmSensitivaMorphosia_sheet (.../bmSensitivaMorphosia_sheet.m:6-187):
+    the quadratic penalty along both cell-array axes; the first gradient of
+    each outer iteration carries the penalty without delta (:118-119), the
+    later ones with it (:164-165)."""
+    xo, _, _, h = morphosia(x, None, None, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, None, nCGD, ve_max, nIter,
+                            kernel, [dict(axis=0, dir=-1, scale=1.0, Tu=Tu1, Tut=Tu1t),
+                                     dict(axis=1, dir=-1, scale=1.0, Tu=Tu2, Tut=Tu2t)],
+                            l1=False, nFr1=nFr1, first_dagF_scaled=False)
+    return xo, h
 
 
 def nasha(y, Gn, N_u, n_u, dK_u, C=None, kernel=(None, None, None)):
