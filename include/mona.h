@@ -296,16 +296,28 @@ int mona_plan_set_deform(mona_plan P, int set, int frame, mona_sparse Tu, mona_s
  *   MONA_MORPH_SENSITIVA_CHAIN bmSensitivaMorphosia_chain.m:6-280 (delta HZ ||Tu x_i - x_{i-1}||^2)
  *   MONA_MORPH_SENSITIVA_SHEET bmSensitivaMorphosia_sheet.m:6-336 (both axes; the first
  *                              gradient of each iteration carries the penalty without
- *                              delta, as bmSensitivaMorphosia_sheet.m:118-119 does) */
+ *                              delta, as bmSensitivaMorphosia_sheet.m:118-119 does)
+ *   MONA_MORPH_SENSITIVA_DUO_CHAIN bmSensitivaDuoMorphosia_chain.m:6-313 (the duo
+ *                              differences under the quadratic penalty) */
 #define MONA_MORPH_TEVA_CHAIN 0
 #define MONA_MORPH_TEVA_DUO_CHAIN 1
 #define MONA_MORPH_TEVA_SHEET 2
 #define MONA_MORPH_SENSITIVA_CHAIN 3
 #define MONA_MORPH_SENSITIVA_SHEET 4
+#define MONA_MORPH_SENSITIVA_DUO_CHAIN 5
 int mona_morphosia(mona_plan P, int solver, int nFr1, void* x, void* z, void* u, int zu_given,
                    const void* y, const long long* y_off, const float* ve,
                    const long long* ve_off, float ve_max, const float* delta,
                    const float* rho, int nIter, int nCGD, double* hist, hipStream_t stream);
+
+/* Motion-compensated SENSE (bmSensaMorphosia_chain.m:6-104): one image x
+ * [prod(n_u)] float2 observed by every frame through its deformation
+ * (set 0; identity where none): minimises sum_f ||M_f Tu_f x - y_f||^2_HY
+ * by nCGD CG steps per iteration, restarted nIter times.  y, ve as
+ * mona_teva_chain; hist (host, may be NULL): nIter residuals R. */
+int mona_sensa_morphosia(mona_plan P, void* x, const void* y, const long long* y_off,
+                         const float* ve, const long long* ve_off, float ve_max, int nIter,
+                         int nCGD, double* hist, hipStream_t stream);
 
 /* Gridded reconstruction (bmNasha.m:6-91): n2u with Gn, centred ifft times
  * prod(N_u)*prod(dK_u), times K, crop to n_u; with C != NULL the coil

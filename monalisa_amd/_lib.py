@@ -71,6 +71,7 @@ SIGNATURES = {
     "mona_steva": (I, [P, I, P, P, P, I, P, P, F, FP, FP, I, I, DP, P]),
     "mona_plan_set_deform": (I, [P, I, I, P, P]),
     "mona_morphosia": (I, [P, I, I, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
+    "mona_sensa_morphosia": (I, [P, P, P, LLP, P, LLP, F, I, I, DP, P]),
     "mona_teva_chain": (I, [P, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_nasha": (I, [P, P, P, P, P, P]),
     "mona_sensa_batch": (I, [P, I, P, P, LLP, FP, LLP, FP, I, I, DP, P]),

@@ -159,6 +159,19 @@ __global__ void k_tdiffT_acc(const float2* __restrict__ za, const float2* __rest
     out[t] = make_float2(o.x + coef * (a.x - b.x), o.y + coef * (a.y - b.y));
 }
 
+// out[i] = sum_f a[f][i], frames added in order (bmSensaMorphosia_chain.m:63-70)
+__global__ void k_frame_sum(const float2* __restrict__ a, float2* __restrict__ out, int nFr, int64_t nu) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    float2 s = a[i];
+    for (int f = 1; f < nFr; f++) {
+        float2 v = a[(int64_t)f * nu + i];
+        s.x += v.x;
+        s.y += v.y;
+    }
+    out[i] = s;
+}
+
 constexpr int kRT = 256;
 constexpr int kRB = 1024;
 
@@ -392,6 +405,12 @@ MorphSpec morph_spec(int solver) {
         case MONA_MORPH_SENSITIVA_CHAIN:  // bmSensitivaMorphosia_chain.m:251-277
             m.nT = 1;
             m.t[0] = {0, -1, 1.0f, 0};
+            m.l1 = false;
+            break;
+        case MONA_MORPH_SENSITIVA_DUO_CHAIN:  // bmSensitivaDuoMorphosia_chain.m:150-183
+            m.nT = 2;
+            m.t[0] = {0, -1, 0.5f, 0};
+            m.t[1] = {0, +1, 0.5f, 1};
             m.l1 = false;
             break;
         case MONA_MORPH_SENSITIVA_SHEET:  // bmSensitivaMorphosia_sheet.m
@@ -628,6 +647,116 @@ void morphosia(Plan& P, int solver, int nFr1, float2* x, float2* z, float2* u, b
     }
 }
 
+// ================================================================ bmSensaMorphosia_chain
+// One image x seen through every frame's deformation: minimises
+// sum_f ||M_f Tu_f x - y_f||^2_HY by nCGD CG steps per iteration
+// (bmSensaMorphosia_chain.m:6-104; deformation set 0).  hist[c] = R, the
+// HY-weighted residual after iteration c.
+void sensa_morphosia(Plan& P, float2* x, const float2* y, const long long* y_off, const float* ve,
+                     const long long* ve_off, float ve_max, int nIter, int nCGD, double* hist_out, hipStream_t s) {
+    int nFr = (int)P.frames.size();
+    MONA_CHECK(nFr >= 1, MONA_ERR_STATE, "the plan has no frames");
+    MONA_CHECK(ve_max > 0, MONA_ERR_ARG, "ve_max must be > 0");
+    int64_t nu = P.nu, nx = (int64_t)nFr * nu;
+    std::vector<int64_t> Pf(nFr), koff(nFr + 1, 0);
+    for (int f = 0; f < nFr; f++) {
+        Pf[f] = P.frame(f).Gu->l_size;
+        koff[f + 1] = koff[f] + Pf[f] * P.nCh;
+    }
+    int64_t ny = koff[nFr];
+    Work& w = P.work;
+    w.hy.ensure((size_t)(ny / P.nCh) * sizeof(float));
+    w.hy0.ensure((size_t)(ny / P.nCh) * sizeof(float));
+    w.ys.ensure((size_t)ny * sizeof(float2));
+    w.k1.ensure((size_t)ny * sizeof(float2));
+    w.k2.ensure((size_t)ny * sizeof(float2));
+    w.z1.ensure((size_t)nx * sizeof(float2));
+    w.z2.ensure((size_t)nx * sizeof(float2));
+    w.z3.ensure((size_t)nu * sizeof(float2));
+    w.z4.ensure((size_t)nu * sizeof(float2));
+    w.z6.ensure((size_t)nx * sizeof(float2));
+    w.sc.ensure(sizeof(CgScalars));
+    w.partial.ensure(kRB * sizeof(double));
+    w.hist.ensure((size_t)std::max(nIter, 1) * sizeof(double));
+    float* hy = w.hy.as<float>();
+    float2 *res = w.k1.as<float2>(), *Ap = w.k2.as<float2>();
+    float2 *tx = w.z1.as<float2>(), *img = w.z2.as<float2>(), *tmp = w.z6.as<float2>();
+    float2 *dag = w.z3.as<float2>(), *p = w.z4.as<float2>();
+    CgScalars* sc = w.sc.as<CgScalars>();
+    double* part = w.partial.as<double>();
+    double* hist = w.hist.as<double>();
+    const float invHX = (float)(1.0 / P.HX);
+    for (int f = 0; f < nFr; f++) {
+        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], s);
+        P.weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / P.nCh, hy + koff[f] / P.nCh, s);
+        P.to_tile_order(f, y + y_off[f], w.ys.as<float2>() + koff[f], s);
+    }
+    const float2* ys = w.ys.as<float2>();
+    if (P.use_batch && (!P.Gu_b || (int)P.koff_b.size() != nFr + 1)) P.build_batch(s);
+    auto T_of = [&](int f, bool transposed) -> const SparseDev* {
+        if (f >= (int)P.deform[0].size()) return nullptr;
+        return transposed ? P.deform[0][f].second : P.deform[0][f].first;
+    };
+    // A v: Tu_f v for every frame, then the forward operator of each frame
+    auto A = [&](const float2* v, float2* out) {
+        for (int f = 0; f < nFr; f++) {
+            const SparseDev* T = T_of(f, false);
+            if (T) spmv_c2_rowmajor(*T, v, tx + (int64_t)f * nu, 1, nullptr, s);
+            else vcopy(v, tx + (int64_t)f * nu, nu, s);
+        }
+        if (P.use_batch) P.forward_batch(tx, out, s);
+        else
+            for (int f = 0; f < nFr; f++) P.forward(f, tx + (int64_t)f * nu, out + koff[f], s);
+    };
+    // dag = sum_f (1/HX) Tut_f Nakatsha(HY r_f)
+    auto Ah = [&](const float2* r, float2* out) {
+        if (P.use_batch) P.adjoint_batch(r, img, hy, invHX, s);
+        else
+            for (int f = 0; f < nFr; f++)
+                P.adjoint(f, r + koff[f], img + (int64_t)f * nu, true, hy + koff[f] / P.nCh, invHX, s);
+        const float2* src = img;
+        bool any = false;
+        for (int f = 0; f < nFr; f++) any = any || T_of(f, true);
+        if (any) {
+            for (int f = 0; f < nFr; f++) {
+                const SparseDev* T = T_of(f, true);
+                if (T) spmv_c2_rowmajor(*T, img + (int64_t)f * nu, tmp + (int64_t)f * nu, 1, nullptr, s);
+                else vcopy(img + (int64_t)f * nu, tmp + (int64_t)f * nu, nu, s);
+            }
+            src = tmp;
+        }
+        k_frame_sum<<<nb(nu), 256, 0, s>>>(src, out, nFr, nu);
+        MONA_HIP(hipGetLastError());
+    };
+    for (int c = 0; c < nIter; c++) {
+        cg_reset(sc, s);
+        A(x, Ap);
+        vsub(ys, Ap, res, ny, s);
+        Ah(res, dag);
+        sqnorm(dag, nu, nullptr, 1, P.HX, &sc->sqn, false, part, s);
+        vcopy(dag, p, nu, s);
+        for (int j = 1; j <= nCGD; j++) {
+            A(p, Ap);
+            sqnorm(Ap, ny, hy, P.nCh, 1.0, &sc->den, false, part, s);
+            cg_alpha(sc, s);
+            vaxpy(x, p, &sc->a, 1.0f, nu, s);
+            if (j == nCGD) break;
+            vaxpy(res, Ap, &sc->a, -1.0f, ny, s);
+            Ah(res, dag);
+            sqnorm(dag, nu, nullptr, 1, P.HX, &sc->sqn_new, false, part, s);
+            cg_beta(sc, s);
+            vxpby(p, dag, &sc->b, nu, s);
+        }
+        A(x, Ap);
+        vsub(ys, Ap, res, ny, s);
+        sqnorm(res, ny, hy, P.nCh, 1.0, hist + c, false, part, s);
+    }
+    if (hist_out && nIter > 0) {
+        MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * sizeof(double), hipMemcpyDeviceToHost, s));
+        MONA_HIP(hipStreamSynchronize(s));
+    }
+}
+
 // ================================================================ bmNasha
 void nasha(Plan& P, const SparseDev& Gn, const float2* y, const float2* C, float2* x, hipStream_t s) {
     MONA_CHECK(P.coils_set, MONA_ERR_STATE, "coil maps not set (the deapodisation is computed there)");
@@ -685,6 +814,16 @@ int mona_morphosia(mona_plan P, int solver, int nFr1, void* x, void* z, void* u,
         morphosia(P->P, solver, nFr1, static_cast<float2*>(x), static_cast<float2*>(z), static_cast<float2*>(u),
                   zu_given != 0, static_cast<const float2*>(y), y_off, ve, ve_off, ve_max, delta, rho, nIter, nCGD,
                   hist, stream);
+    });
+}
+
+int mona_sensa_morphosia(mona_plan P, void* x, const void* y, const long long* y_off, const float* ve,
+                         const long long* ve_off, float ve_max, int nIter, int nCGD, double* hist,
+                         hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && x && y && y_off && ve && ve_off, MONA_ERR_ARG, "NULL argument");
+        sensa_morphosia(P->P, static_cast<float2*>(x), static_cast<const float2*>(y), y_off, ve, ve_off, ve_max,
+                        nIter, nCGD, hist, stream);
     });
 }
 

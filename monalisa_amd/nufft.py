@@ -271,7 +271,7 @@ class Plan:
 
     # -- the other Morphosia solvers (mona_morphosia) ------------------------
     MORPH_TEVA_CHAIN, MORPH_TEVA_DUO_CHAIN, MORPH_TEVA_SHEET = 0, 1, 2
-    MORPH_SENSITIVA_CHAIN, MORPH_SENSITIVA_SHEET = 3, 4
+    MORPH_SENSITIVA_CHAIN, MORPH_SENSITIVA_SHEET, MORPH_SENSITIVA_DUO_CHAIN = 3, 4, 5
 
     def _morph(self, solver, nFr1, x, zs, us, y_list, ve_list, delta, rho, nCGD, ve_max, nIter, stream, nT):
         import torch
@@ -348,6 +348,32 @@ class Plan:
         self._set_deforms(nFr, 1, Tu2, Tu2t)
         return self._morph(self.MORPH_SENSITIVA_SHEET, nFr_1, x, None, None, y_list, ve_list, delta, None, nCGD,
                            ve_max, nIter, stream, 2)
+
+    def bmSensitivaDuoMorphosia_chain(self, x, y_list, ve_list, delta, nCGD, ve_max, nIter=1, Tu1=None, Tu1t=None,
+                                      Tu2=None, Tu2t=None, stream=None):
+        """bmSensitivaDuoMorphosia_chain.m:6-108: the quadratic penalty on
+        0.5 (Tu1_i x_i - x_{i-1}) and 0.5 (Tu2_i x_i - x_{i+1}); returns hist."""
+        nFr = x.shape[0]
+        self._set_deforms(nFr, 0, Tu1, Tu1t)
+        self._set_deforms(nFr, 1, Tu2, Tu2t)
+        return self._morph(self.MORPH_SENSITIVA_DUO_CHAIN, None, x, None, None, y_list, ve_list, delta, None, nCGD,
+                           ve_max, nIter, stream, 2)
+
+    def bmSensaMorphosia_chain(self, x, y_list, ve_list, nCGD, ve_max, nIter=1, Tu=None, Tut=None, stream=None):
+        """bmSensaMorphosia_chain.m:6-104: one image x [nu] (complex64,
+        updated in place) seen by every frame through Tu_f; returns the
+        residual R after each iteration."""
+        import torch
+        nFr = len(y_list)
+        self._set_deforms(nFr, 0, Tu, Tut)
+        ycat = torch.cat([yy.reshape(-1) for yy in y_list])
+        vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
+        yoff = (C.c_longlong * nFr)(*[int(v) for v in np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]])
+        voff = (C.c_longlong * nFr)(*[int(v) for v in np.cumsum([0] + [vv.numel() for vv in ve_list])[:-1]])
+        hist = (C.c_double * max(nIter, 1))()
+        check(lib.mona_sensa_morphosia(self._h, ptr(x), ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)),
+                                       voff, float(ve_max), int(nIter), int(nCGD), hist, stream_of(stream)))
+        return np.array(list(hist)[:nIter])
 
     def bmNasha(self, y, Gn: BmSparseMat, Cmaps=None, stream=None):
         """Gridded recon (bmNasha.m:6-91)."""

@@ -31,8 +31,13 @@ Every function cites the reference file:line it follows.  What pins each one
   deletion of points that leave the grid, Gut == Gu^T exactly.
 * ``steva``/``teva_chain``: parity unpinned beyond the above building blocks
   and monotone objective checks (no reference run exists to compare with);
-  with deformations, identity matrices reproduce the undeformed chain and a
-  torus shift equals the same chain on shifted frames.
+  with deformations, identity matrices reproduce the undeformed chain.
+* ``morphosia`` and its solvers (tests/test_oracle_morphosia.py): every
+  difference term is the adjoint of its dagF; the duo chain with identities
+  is the chain at rho/2; a one-column sheet is the chain; the sheet is
+  symmetric in its axes; the quadratic (Sensitiva) solvers and
+  ``sensa_morphosia`` converge to the dense least-squares solution, and
+  ``sensa_morphosia`` with one identity frame is ``sensa``.
 """
 from __future__ import annotations
 
@@ -956,6 +961,63 @@ def sensitiva_sheet(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, nCGD, ve_max, n
                                      dict(axis=1, dir=-1, scale=1.0, Tu=Tu2, Tut=Tu2t)],
                             l1=False, nFr1=nFr1, first_dagF_scaled=False)
     return xo, h
+
+
+def sensitiva_duo_chain(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, nCGD, ve_max, nIter=1,
+                        kernel=(None, None, None), Tu1=None, Tu1t=None, Tu2=None, Tu2t=None):
+    """bmSensitivaDuoMorphosia_chain (.../bmSensitivaDuoMorphosia_chain.m:6-108):
+    the duo chain's half-weighted differences (:150-162) and adjoints
+    (:170-183) under the quadratic penalty; regul of the unscaled
+    differences (:117-130)."""
+    xo, _, _, h = morphosia(x, None, None, y, ve, C, Gu, Gut, N_u, n_u, dK_u, delta, None, nCGD, ve_max, nIter,
+                            kernel, [dict(axis=0, dir=-1, scale=0.5, Tu=Tu1, Tut=Tu1t),
+                                     dict(axis=0, dir=+1, scale=0.5, Tu=Tu2, Tut=Tu2t)], l1=False)
+    return xo, h
+
+
+def sensa_morphosia(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, nCGD, ve_max, nIter=1, kernel=(None, None, None),
+                    Tu=None, Tut=None):
+    """bmSensaMorphosia_chain (.../bmSensaMorphosia_chain.m:6-104): one image
+    x, frame f observes M_f Tu_f x; CG on sum_f ||M_f Tu_f x - y_f||^2_HY,
+    nCGD steps per iteration restarted nIter times.  Returns x and R per
+    iteration (:93-99)."""
+    eps = 10 * np.finfo(np.float32).eps
+    nFr = len(y)
+    KFC = kf(C, N_u, n_u, dK_u, *kernel)
+    KFCc = kf_conj(np.conj(C), N_u, n_u, dK_u, *kernel)
+    HX = hx_of(N_u, dK_u)
+    HY = [np.minimum(np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1, 1), yy.shape), ve_max)
+          for v, yy in zip(ve, y)]
+    Tu = [None] * nFr if Tu is None else [None if G is None else to_scipy(G) for G in Tu]
+    Tut = [None] * nFr if Tut is None else [None if G is None else to_scipy(G) for G in Tut]
+    deform = lambda G, a: a if G is None else (G @ a.reshape(-1, order='F')).reshape(a.shape, order='F')
+    A = lambda v: [shanna(deform(Tu[i], v), Gu[i], KFC, N_u, n_u) for i in range(nFr)]
+    Ah = lambda r: sum((1.0 / HX) * deform(Tut[i], nakatsha(HY[i] * r[i], Gut[i], KFCc, True, N_u, n_u))
+                       for i in range(nFr))
+    sq = lambda a: sum(float(np.real(np.vdot(ai, HY[i] * ai))) for i, ai in enumerate(a))
+    x = np.array(x, dtype=np.complex128)
+    hist = []
+    for c in range(nIter):
+        r = [yy - m for yy, m in zip(y, A(x))]
+        dag = Ah(r)
+        sqn = HX * float(np.real(np.vdot(dag, dag)))
+        p = dag
+        for j in range(1, nCGD + 1):
+            if sqn < eps:
+                break
+            Ap = A(p)
+            a = sqn / sq(Ap)
+            x = x + a * p
+            if j == nCGD:
+                break
+            r = [ri - a * api for ri, api in zip(r, Ap)]
+            dag = Ah(r)
+            sqn_new = HX * float(np.real(np.vdot(dag, dag)))
+            b = sqn_new / sqn
+            sqn = sqn_new
+            p = dag + b * p
+        hist.append(sq([yy - m for yy, m in zip(y, A(x))]))
+    return x, hist
 
 
 def nasha(y, Gn, N_u, n_u, dK_u, C=None, kernel=(None, None, None)):

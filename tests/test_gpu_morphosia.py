@@ -1,5 +1,6 @@
 """The Morphosia solvers on the GPU (bmTevaDuoMorphosia_chain,
-bmTevaMorphosia_sheet, bmSensitivaMorphosia_chain / _sheet) with deformation
+bmTevaMorphosia_sheet, bmSensitivaMorphosia_chain / _sheet,
+bmSensitivaDuoMorphosia_chain, bmSensaMorphosia_chain) with deformation
 matrices, against the float64 oracle."""
 import numpy as np
 import pytest
@@ -88,11 +89,11 @@ def test_teva_two_term_solvers_vs_oracle(solver):
     assert np.allclose(hist, np.array(hr), rtol=2e-3)
 
 
-@pytest.mark.parametrize("solver", ["chain", "sheet"])
+@pytest.mark.parametrize("solver", ["chain", "sheet", "duo"])
 def test_sensitiva_solvers_vs_oracle(solver):
     import torch
     print(f"[gpu] sensitiva {solver}", flush=True)
-    nFr, n1 = (3, None) if solver == "chain" else (6, 3)
+    nFr, n1 = (6, 3) if solver == "sheet" else (3, None)
     q, T = _setup(nFr, 50)
     P = q["P"]
     ys, ves = _gpu_lists(q)
@@ -105,6 +106,11 @@ def test_sensitiva_solvers_vs_oracle(solver):
         hist = P.bmSensitivaMorphosia_chain(x, ys, ves, delta, nCGD, ve_max, nIter, Tu1, Tu1t)
         xr, hr = O.sensitiva_chain([np.zeros(q["n_u"], complex)] * nFr, *_oracle_common(q), delta, nCGD, ve_max,
                                    nIter, kernel=("kaiser", None, None), Tu=rTu1, Tut=rTu1t)
+    elif solver == "duo":
+        hist = P.bmSensitivaDuoMorphosia_chain(x, ys, ves, delta, nCGD, ve_max, nIter, Tu1, Tu1t, Tu2, Tu2t)
+        xr, hr = O.sensitiva_duo_chain([np.zeros(q["n_u"], complex)] * nFr, *_oracle_common(q), delta, nCGD,
+                                       ve_max, nIter, kernel=("kaiser", None, None), Tu1=rTu1, Tu1t=rTu1t, Tu2=rTu2,
+                                       Tu2t=rTu2t)
     else:
         hist = P.bmSensitivaMorphosia_sheet(x, ys, ves, n1, delta, nCGD, ve_max, nIter, Tu1, Tu1t, Tu2, Tu2t)
         xr, hr = O.sensitiva_sheet([np.zeros(q["n_u"], complex)] * nFr, *_oracle_common(q), delta, nCGD, ve_max,
@@ -159,3 +165,28 @@ def test_chain_through_general_entry_equals_teva_chain():
     np.testing.assert_array_equal(x1.cpu().numpy(), x2.cpu().numpy())
     np.testing.assert_array_equal(z.cpu().numpy(), zs[0].cpu().numpy())
     np.testing.assert_array_equal(h1, h2)
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_sensa_morphosia_vs_oracle(batch):
+    """One image through every frame's deformation (bmSensaMorphosia_chain)."""
+    import torch
+    print(f"[gpu] sensa morphosia batch={batch}", flush=True)
+    q, T = _setup(3, 110)
+    P = q["P"]
+    if batch:
+        P.set_options(batch_frames=True)
+    ys, ves = _gpu_lists(q)
+    ve_max = float(max(v.max() for v in q["ves"]))
+    Tu, rTu = _mats(T[0], "Gu")
+    Tut, rTut = _mats(T[0], "Gut")
+    Tu[2] = Tut[2] = rTu[2] = rTut[2] = None
+    nCGD, nIter = 4, 3
+    x = torch.zeros(P.nu, dtype=torch.complex64, device="cuda")
+    hist = P.bmSensaMorphosia_chain(x, ys, ves, nCGD, ve_max, nIter, Tu, Tut)
+    torch.cuda.synchronize()
+    common = _oracle_common(q)
+    xr, hr = O.sensa_morphosia(np.zeros(q["n_u"], complex), *common, nCGD, ve_max, nIter,
+                               kernel=("kaiser", None, None), Tu=rTu, Tut=rTut)
+    assert rel(x.cpu().numpy(), flat(xr, q["n_u"])) < 1e-4
+    assert np.allclose(hist, np.array(hr), rtol=2e-3)

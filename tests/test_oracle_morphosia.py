@@ -185,3 +185,54 @@ def test_sensitiva_converges_to_least_squares(sheet):
     r = M @ xs - y
     obj = 0.5 * np.real(np.vdot(r, hy * r)) + 0.5 * delta * HX * sum(np.linalg.norm(F @ xs) ** 2 for F in Fs)
     assert abs(hist[-1][0] - obj) < 1e-8 * obj
+
+
+def test_sensitiva_duo_converges_to_least_squares():
+    q = _problem(3, n_u=(6, 6), seed=4)
+    q["y"] = [1e4 * v for v in q["y"]]
+    T1, T2 = _fields(q["n_u"], 3, 11), _fields(q["n_u"], 3, 12)
+    Tu1, Tu1t = [G["Gu"] for G in T1], [G["Gut"] for G in T1]
+    Tu2, Tu2t = [G["Gu"] for G in T2], [G["Gut"] for G in T2]
+    delta = 0.4
+    x, hist = O.sensitiva_duo_chain(_x0(q), *_args(q), delta, 12, q["ve_max"], 25, kernel=q["kernel"],
+                                    Tu1=Tu1, Tu1t=Tu1t, Tu2=Tu2, Tu2t=Tu2t)
+    terms = [dict(axis=0, dir=-1, Tu=Tu1, Tut=Tu1t), dict(axis=0, dir=+1, Tu=Tu2, Tut=Tu2t)]
+    M, hy, y, Fs, HX = _dense_ops(q, terms, 3)
+    # F_t = 0.5 * difference: the penalty is delta HZ sum_t ||0.5 D_t x||^2
+    A = M.conj().T @ (hy[:, None] * M) + delta * HX * 0.25 * sum(F.conj().T @ F for F in Fs)
+    xs = np.linalg.solve(A, M.conj().T @ (hy * y))
+    got = np.concatenate([v.reshape(-1, order="F") for v in x])
+    assert np.linalg.norm(got - xs) < 1e-6 * np.linalg.norm(xs)
+    # regul: squared norms of the unscaled differences
+    r = M @ xs - y
+    regul = HX * sum(np.linalg.norm(F @ xs) ** 2 for F in Fs)
+    np.testing.assert_allclose(hist[-1][2], regul, rtol=1e-6)
+    np.testing.assert_allclose(hist[-1][1], np.real(np.vdot(r, hy * r)), rtol=1e-6)
+
+
+def test_sensa_morphosia_converges_to_least_squares():
+    """One image seen through every frame's deformation: the minimiser of
+    sum_f ||M_f Tu_f x - y_f||^2_HY."""
+    q = _problem(3, n_u=(6, 6), seed=6)
+    q["y"] = [1e4 * v for v in q["y"]]
+    T = _fields(q["n_u"], 3, 13)
+    Tu, Tut = [G["Gu"] for G in T], [G["Gut"] for G in T]
+    Tu[1] = Tut[1] = None
+    x, hist = O.sensa_morphosia(np.zeros(q["n_u"], complex), *_args(q), 12, q["ve_max"], 20, kernel=q["kernel"],
+                                Tu=Tu, Tut=Tut)
+    M, hy, y, _, HX = _dense_ops(q, [], 3)
+    nu = 36
+    D = [np.eye(nu) if G is None else O.to_dense(G) for G in Tu]
+    A = M @ np.vstack(D)                                   # block rows M_f Tu_f
+    xs = np.linalg.solve(A.conj().T @ (hy[:, None] * A), A.conj().T @ (hy * y))
+    assert np.linalg.norm(x.reshape(-1, order="F") - xs) < 1e-6 * np.linalg.norm(xs)
+    r = A @ xs - y
+    np.testing.assert_allclose(hist[-1], np.real(np.vdot(r, hy * r)), rtol=1e-6)
+
+
+def test_sensa_morphosia_of_one_identity_frame_is_sensa():
+    q = _problem(1, seed=8)
+    xm, hm = O.sensa_morphosia(np.zeros(q["n_u"], complex), *_args(q), 4, q["ve_max"], 2, kernel=q["kernel"])
+    xs, hs = O.sensa(np.zeros(q["n_u"], complex), q["y"][0], q["ve"][0], q["C"], q["Gu"][0], q["Gut"][0], q["N_u"],
+                     q["n_u"], q["dK_u"], 4, q["ve_max"], 2, kernel=q["kernel"])
+    np.testing.assert_allclose(xm, xs, rtol=0, atol=1e-12 * np.abs(xs).max())
