@@ -238,17 +238,44 @@ __global__ void k_hy(const float* ve, float vmax, float* hy, int64_t n) {
 constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = 1024;
 
+// Each thread sums the elements i0, i0 + G, i0 + 2G, ... (G = gridDim *
+// kRedThreads) in that order; the loads of four consecutive terms are issued
+// before their adds, which keeps the order.  Weight rows use 32-bit
+// division (n < 2^31 is checked on the host).
+template <bool W>
+__device__ __forceinline__ double sqn_term(const float2* a, const float* w, int64_t i, unsigned nvec) {
+    float2 v = a[i];
+    double m = (double)v.x * v.x + (double)v.y * v.y;
+    if constexpr (W) m *= (double)w[(unsigned)i / nvec];
+    return m;
+}
+
+template <bool W>
 __global__ void __launch_bounds__(kRedThreads)
 k_sqnorm_partial(const float2* a, int64_t n, const float* w, int nvec, double cw, double* partial) {
     __shared__ double sh[kRedThreads];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kRedThreads) {
-        float2 v = a[i];
-        double m = (double)v.x * v.x + (double)v.y * v.y;
-        if (w) m *= (double)w[i / nvec];
-        s += m;
+    const int64_t G = (int64_t)gridDim.x * kRedThreads;
+    int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x;
+    const unsigned nv = (unsigned)nvec;
+    for (; i + 3 * G < n; i += 4 * G) {
+        float2 v0 = a[i], v1 = a[i + G], v2 = a[i + 2 * G], v3 = a[i + 3 * G];
+        double m0 = (double)v0.x * v0.x + (double)v0.y * v0.y;
+        double m1 = (double)v1.x * v1.x + (double)v1.y * v1.y;
+        double m2 = (double)v2.x * v2.x + (double)v2.y * v2.y;
+        double m3 = (double)v3.x * v3.x + (double)v3.y * v3.y;
+        if constexpr (W) {
+            m0 *= (double)w[(unsigned)i / nv];
+            m1 *= (double)w[(unsigned)(i + G) / nv];
+            m2 *= (double)w[(unsigned)(i + 2 * G) / nv];
+            m3 *= (double)w[(unsigned)(i + 3 * G) / nv];
+        }
+        s += m0;
+        s += m1;
+        s += m2;
+        s += m3;
     }
+    for (; i < n; i += G) s += sqn_term<W>(a, w, i, nv);
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = kRedThreads / 2; o > 0; o >>= 1) {
@@ -259,13 +286,22 @@ k_sqnorm_partial(const float2* a, int64_t n, const float* w, int nvec, double cw
 }
 
 // partial[block] = cw * sum_i Re(conj(a_i) b_i), FP64 accumulation in a
-// fixed order (deterministic)
+// fixed order (deterministic; loads issued four terms ahead)
 __global__ void __launch_bounds__(kRedThreads)
 k_dotre_partial(const float2* a, const float2* b, int64_t n, double cw, double* partial) {
     __shared__ double sh[kRedThreads];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kRedThreads) {
+    const int64_t G = (int64_t)gridDim.x * kRedThreads;
+    int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x;
+    for (; i + 3 * G < n; i += 4 * G) {
+        float2 u0 = a[i], u1 = a[i + G], u2 = a[i + 2 * G], u3 = a[i + 3 * G];
+        float2 v0 = b[i], v1 = b[i + G], v2 = b[i + 2 * G], v3 = b[i + 3 * G];
+        s += (double)u0.x * v0.x + (double)u0.y * v0.y;
+        s += (double)u1.x * v1.x + (double)u1.y * v1.y;
+        s += (double)u2.x * v2.x + (double)u2.y * v2.y;
+        s += (double)u3.x * v3.x + (double)u3.y * v3.y;
+    }
+    for (; i < n; i += G) {
         float2 u = a[i], v = b[i];
         s += (double)u.x * v.x + (double)u.y * v.y;
     }
@@ -908,7 +944,9 @@ void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s
 void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
             double* partial, hipStream_t s) {
     int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
-    k_sqnorm_partial<<<nb, kRedThreads, 0, s>>>(a, n, w, nvec, cw, partial);
+    MONA_CHECK(!w || n < 2147483647LL, MONA_ERR_ARG, "weighted norm over 2^31 or more elements");
+    if (w) k_sqnorm_partial<true><<<nb, kRedThreads, 0, s>>>(a, n, w, nvec, cw, partial);
+    else k_sqnorm_partial<false><<<nb, kRedThreads, 0, s>>>(a, n, w, nvec, cw, partial);
     MONA_HIP(hipGetLastError());
     k_reduce_final<<<1, kRedThreads, 0, s>>>(partial, nb, out, acc ? 1 : 0);
     MONA_HIP(hipGetLastError());

@@ -328,7 +328,20 @@ spmv_c2_combine(const int* __restrict__ long_row, const int* __restrict__ long_f
     if (t >= (int64_t)n_long * n_vec) return;
     int r = (int)(t / n_vec), k = (int)(t - (int64_t)r * n_vec);
     float ar = 0.0f, ai = 0.0f;
-    for (int c = long_first[r], e = long_first[r + 1]; c < e; c++) {
+    int c = long_first[r];
+    const int e = long_first[r + 1];
+    // eight loads in flight, added in chunk order
+    for (; c + 8 <= e; c += 8) {
+        float2 p[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) p[q] = partial[(int64_t)(c + q) * n_vec + k];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            ar = ar + p[q].x;
+            ai = ai + p[q].y;
+        }
+    }
+    for (; c < e; c++) {
         float2 p = partial[(int64_t)c * n_vec + k];
         ar = ar + p.x;
         ai = ai + p.y;
