@@ -58,6 +58,8 @@ struct PassArgs {
     const uint32_t* out_mask;
     int mask_nch;
     int64_t mask_cpl;          // mask columns per line block: S / mask_nch
+    const int2* in_ival;       // register kernel, mode 0: row intervals per column (see AxisPass)
+    const int2* out_ival;
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -443,8 +445,12 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
 // loads and stores cover runs of consecutive lines (inner positions) in
 // consecutive rows.  MODE as for the LDS kernel; the coil sum of MODE 2 goes
 // through a second LDS pass and adds the coils in order.
-template <int N2, int MODE, class TI, class TO>
+// SPLIT (plain passes only): the exchange goes through LDS in two halves of
+// eight planes (k1 < 8, then k1 >= 8), so a tile needs half the exchange
+// space and more tiles fit on a CU.
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
 __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
+    static_assert(!SPLIT || MODE == 0, "split exchange is for plain passes");
     const TI* __restrict__ in = reinterpret_cast<const TI*>(a.in);
     TO* __restrict__ out = reinterpret_cast<TO*>(a.out);
     constexpr int N1 = 16;
@@ -499,6 +505,16 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         for (int i = 1; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
         return (wv >> (row & 31)) & 1u;
     };
+    // interval passes: this work-item's column keeps rows [ilo, ihi)
+    int ilo = 0, ihi = N;
+    {
+        const int2* iv = MODE == 0 ? (a.in_ival ? a.in_ival : a.out_ival) : nullptr;
+        if (iv && live) {
+            const int2 q = iv[o * a.mask_cpl + ws / a.mask_nch];
+            ilo = q.x;
+            ihi = q.y;
+        }
+    }
     float2 v[N1];
     if (j < N2) {
 #pragma unroll
@@ -506,7 +522,11 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             const int r = j + N2 * n1 - a.in_off;
             float2 x = make_float2(0.f, 0.f);
             const bool take = live && r >= 0 && r < a.in_cnt;
-            if (MODE == 0 && a.in_mask) {
+            if (MODE == 0 && a.in_ival) {
+                // rows outside the column's interval are zero and not loaded
+                const int row = j + N2 * n1;
+                if (take && row >= ilo && row < ihi) x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
+            } else if (MODE == 0 && a.in_mask) {
                 // masked-off rows (not written by the producer) are zero and
                 // not loaded
                 if (take && bit(j + N2 * n1)) x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
@@ -536,13 +556,37 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             wv.y *= -sg;  // table holds exp(-2 pi i t / N); conj for the inverse
             v[k1] = cmul(v[k1], wv);
         }
+        if constexpr (!SPLIT) {
 #pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) ex[k1 * XS + j * L + line] = v[k1];
+            for (int k1 = 0; k1 < N1; k1++) ex[k1 * XS + j * L + line] = v[k1];
+        } else {
+#pragma unroll
+            for (int k1 = 0; k1 < N1 / 2; k1++) ex[k1 * XS + j * L + line] = v[k1];
+        }
     }
     __syncthreads();
     float2 u[N2];
+    if constexpr (!SPLIT) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[j * XS + n2 * L + line];
+        for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[j * XS + n2 * L + line];
+    } else {
+        // work-item j reads plane k1 = j: the first half now, the second
+        // after the planes k1 >= 8 replace them
+        if (j < N1 / 2) {
+#pragma unroll
+            for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[j * XS + n2 * L + line];
+        }
+        __syncthreads();
+        if (j < N2) {
+#pragma unroll
+            for (int k1 = N1 / 2; k1 < N1; k1++) ex[(k1 - N1 / 2) * XS + j * L + line] = v[k1];
+        }
+        __syncthreads();
+        if (j >= N1 / 2) {
+#pragma unroll
+            for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[(j - N1 / 2) * XS + n2 * L + line];
+        }
+    }
     Dft<N2>::run(u, sg);
     const int half = N / 2;
     if (MODE == 2) {
@@ -588,6 +632,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         const int kl = k - a.out_off;
         bool put = kl >= 0 && kl < a.out_cnt;
         if (MODE == 0 && a.out_mask) put = put && bit(k);
+        if (MODE == 0 && a.out_ival) put = put && k >= ilo && k < ihi;
         if (put) {
             float sc = a.scale;
             if (a.postsign && ((k + half) & 1)) sc = -sc;
@@ -653,6 +698,16 @@ void launch(const PassArgs& a, int mode, unsigned nblocks, size_t lds, const flo
     }
 }
 
+// Two-half exchange of the plain register passes (MONA_FFT_SPLIT=0 turns
+// it off).
+bool split_exchange() {
+    static const bool on = [] {
+        const char* e = std::getenv("MONA_FFT_SPLIT");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
 // Register kernel: lines per tile on strided axes (16, 32 or 64;
 // MONA_FFT_LINES, default 32); -1 disables the register kernel.
 int reg_lines_default() {
@@ -674,9 +729,9 @@ int reg_n2(int N) {
     }
 }
 
-template <int N2, int MODE, class TI, class TO>
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
 void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
-    auto kern = fft_reg_kernel<N2, MODE, TI, TO>;
+    auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT>;
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
@@ -691,7 +746,13 @@ void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, co
 // (MODE 1 writes, MODE 0 reads and writes, MODE 2 reads bf16x2).
 template <int N2>
 void launch_reg(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsigned nblocks, int nt, size_t lds,
-                const float2* tw, hipStream_t s) {
+                const float2* tw, hipStream_t s, bool split) {
+    if (split) {
+        MONA_CHECK(mode == 0 && bf_in == bf_out, MONA_ERR_INTERNAL, "split exchange is for plain passes");
+        if (bf_in) launch_reg_mode<N2, 0, bf16x2, bf16x2, true>(a, nblocks, nt, lds, tw, s);
+        else launch_reg_mode<N2, 0, float2, float2, true>(a, nblocks, nt, lds, tw, s);
+        return;
+    }
     if (bf_in || bf_out) {
         switch (mode) {
             case 1:
@@ -783,27 +844,36 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     a.out_mask = p.out_mask;
     a.mask_nch = p.mask_nch;
     a.mask_cpl = p.S / std::max(p.mask_nch, 1);
-    MONA_CHECK(!(p.in_mask || p.out_mask) || (p.mode == 0 && p.mask_nch >= 1 && p.S % p.mask_nch == 0),
-               MONA_ERR_INTERNAL, "cell masks need a plain pass and S divisible by mask_nch");
+    a.in_ival = p.in_ival;
+    a.out_ival = p.out_ival;
+    const bool limited = p.in_mask || p.out_mask || p.in_ival || p.out_ival;
+    MONA_CHECK(!limited || (p.mode == 0 && p.mask_nch >= 1 && p.S % p.mask_nch == 0), MONA_ERR_INTERNAL,
+               "cell masks / intervals need a plain pass and S divisible by mask_nch");
+    MONA_CHECK(!((p.in_mask || p.out_mask) && (p.in_ival || p.out_ival)), MONA_ERR_INTERNAL,
+               "a pass takes cell masks or intervals, not both");
     const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     const int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
     const int nt = 16 * L;
     MONA_CHECK(nt <= 1024 && (p.mode != 2 || a.W == p.S), MONA_ERR_INTERNAL, "bad register FFT tile");
     size_t ex = std::max<size_t>((size_t)16 * xs, (size_t)a.Bo * p.out_cnt * (a.W | 1));
+    // plain passes whose exchange would hold a CU to fewer than three tiles
+    // exchange in two halves
+    const bool split = p.mode == 0 && split_exchange() && ((size_t)N + ex) * sizeof(float2) > 53 * 1024;
+    if (split) ex = (size_t)8 * xs;
     size_t lds = ((size_t)N + ex) * sizeof(float2);
     MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
     const float2* tw = twiddles().get(N);
     switch (n2) {
-        case 2: launch_reg<2>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 3: launch_reg<3>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 4: launch_reg<4>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 5: launch_reg<5>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 6: launch_reg<6>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 8: launch_reg<8>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 10: launch_reg<10>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        case 12: launch_reg<12>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
-        default: launch_reg<16>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s); break;
+        case 2: launch_reg<2>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 3: launch_reg<3>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 4: launch_reg<4>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 5: launch_reg<5>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 6: launch_reg<6>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 8: launch_reg<8>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 10: launch_reg<10>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        case 12: launch_reg<12>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
+        default: launch_reg<16>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
     }
     MONA_HIP(hipGetLastError());
 }
@@ -835,7 +905,8 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
         fft_pass_reg(p, n2, lines > 0 ? lines : 16, s);
         return;
     }
-    MONA_CHECK(!(p.in_mask || p.out_mask), MONA_ERR_INTERNAL, "cell masks need the register FFT kernel");
+    MONA_CHECK(!(p.in_mask || p.out_mask || p.in_ival || p.out_ival), MONA_ERR_INTERNAL,
+               "cell masks and intervals need the register FFT kernel");
     MONA_CHECK(p.mode == 0 || old_fits, MONA_ERR_INTERNAL, "fused FFT pass does not fit a tile");
     PassArgs a{};
     a.N = N;

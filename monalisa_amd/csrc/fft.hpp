@@ -52,6 +52,11 @@ struct AxisPass {
     const uint32_t* in_mask = nullptr;
     const uint32_t* out_mask = nullptr;
     int mask_nch = 1;
+    // Row intervals (register kernel, mode 0), the contiguous form of the
+    // masks: column c (as above) holds rows [x, y) of in_ival[c] /
+    // out_ival[c]; rows outside are read as 0 without a load / not stored.
+    const int2* in_ival = nullptr;
+    const int2* out_ival = nullptr;
 };
 
 void fft_pass(const AxisPass& p, hipStream_t s);

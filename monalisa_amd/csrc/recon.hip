@@ -474,6 +474,15 @@ const Frame& Plan::frame(int f) const {
 
 // Tile-order copies of a frame's matrices (host decode of the reference
 // encoding, stable sort of the points by 8^3 cell tile, upload).
+// Row intervals for dense frames (MONA_GRID_INTERVALS=0 turns them off).
+bool grid_intervals() {
+    static const bool on = [] {
+        const char* e = std::getenv("MONA_GRID_INTERVALS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
 void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s) {
     MONA_CHECK(f >= 0, MONA_ERR_ARG, "negative frame index");
     MONA_CHECK(!Gu->squeezed && Gu->l_nJump == Gu->l_size, MONA_ERR_ARG,
@@ -566,6 +575,28 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     int64_t touched = 0;
     for (int64_t c = 0; c < G; c++) touched += cell_row[c] >= 0 && trp[cell_row[c] + 1] > trp[cell_row[c]];
     const bool sparse_frame = touched < (G * 3) / 10;
+    // row intervals of the last-axis pass columns (dense frames): rows
+    // [lo, hi) of column c span every cell of c that Gu reads or Gut writes
+    const int64_t NLa = N[imDim - 1], colsa = G / NLa;
+    const bool use_ival = !sparse_frame && imDim >= 2 && grid_intervals();
+    std::vector<int2> ival;
+    if (use_ival) {
+        ival.assign(colsa, make_int2((int)NLa, 0));
+        auto widen = [&](int64_t cell) {
+            const int64_t z = cell / colsa, c = cell - z * colsa;
+            ival[c].x = std::min(ival[c].x, (int)z);
+            ival[c].y = std::max(ival[c].y, (int)z + 1);
+        };
+        for (int64_t k = 0; k < Gu->nnz; k++) widen(col[k]);
+        for (int64_t c = 0; c < G; c++)
+            if (cell_row[c] >= 0 && trp[cell_row[c] + 1] > trp[cell_row[c]]) widen(c);
+        for (auto& q : ival)
+            if (q.y <= q.x) q = make_int2(0, 0);
+    }
+    auto inside = [&](int64_t cell) {
+        const int64_t z = cell / colsa, c = cell - z * colsa;
+        return z >= ival[c].x && z < ival[c].y;
+    };
     std::vector<int> dcol(Gut->nnz), dnj(G), drow(G);
     std::vector<float> dval(Gut->nnz);
     {
@@ -579,7 +610,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
                                 const int64_t cell = ((int64_t)z * N[1] + y) * N[0] + x;
                                 const int64_t r = cell_row[cell];
                                 const int n = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
-                                if (n == 0 && sparse_frame) continue;
+                                if (n == 0 && (sparse_frame || (use_ival && !inside(cell)))) continue;
                                 drow[q] = (int)cell;
                                 dnj[q] = n;
                                 for (int64_t k = n ? trp[r] : 0; k < (n ? trp[r + 1] : 0); k++, o++) {
@@ -593,7 +624,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     }
     auto Gut_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
-    Gut_s->covers_all = !sparse_frame;
+    Gut_s->covers_all = !sparse_frame && !use_ival;
     // cell masks: cells the forward gridding reads (Gu columns) and cells the
     // adjoint gridding writes (Gut rows with entries); equal for a
     // transposed pair.  Layout of the pass along the last axis: for each
@@ -616,6 +647,11 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         dam = std::make_shared<DevBuf>((size_t)nw * sizeof(uint32_t));
         MONA_HIP(hipMemcpyAsync(dam->p, amask.data(), (size_t)nw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     }
+    std::shared_ptr<DevBuf> div;
+    if (use_ival) {
+        div = std::make_shared<DevBuf>((size_t)colsa * sizeof(int2));
+        MONA_HIP(hipMemcpyAsync(div->p, ival.data(), (size_t)colsa * sizeof(int2), hipMemcpyHostToDevice, s));
+    }
     auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
     MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipStreamSynchronize(s));
@@ -630,6 +666,8 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     fr.fmask = dfm;
     fr.amask = dam;
     fr.masked = sparse_frame;
+    fr.ival = div;
+    fr.ivals = use_ival;
     Gu_b.reset();  // the batched matrices are rebuilt on next use
     Gut_b.reset();
     koff_b.clear();
@@ -640,14 +678,39 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
 // adjoint pass loads only the cells the adjoint gridding writes (the others
 // are read as zero), so the untouched cells are never written, filled or
 // read.  Needs the register kernel on the last axis.
+bool Plan::limits_usable() const {
+    return imDim >= 2 && fft_pass_fusable(N[0], nCh) && fft_plain_pass_is_reg(N[imDim - 1], fft_lines);
+}
+
 bool Plan::masked_grid(int f) const {
     const bool frames_sparse = f >= 0 ? frame(f).masked : [&] {
         for (const Frame& fr : frames)
             if (!fr.masked) return false;
         return !frames.empty();
     }();
-    return frames_sparse && imDim >= 2 && fft_pass_fusable(N[0], nCh) &&
-           fft_plain_pass_is_reg(N[imDim - 1], fft_lines);
+    return frames_sparse && limits_usable();
+}
+
+bool Plan::ival_grid(int f) const {
+    const bool all = f >= 0 ? frame(f).ivals : [&] {
+        for (const Frame& fr : frames)
+            if (!fr.ivals) return false;
+        return !frames.empty();
+    }();
+    return all && limits_usable();
+}
+
+const uint32_t* Plan::fwd_mask(int f) const {
+    if (f < 0) return masked_grid(-1) && mask_fb.p ? mask_fb.as<uint32_t>() : nullptr;
+    return masked_grid(f) ? frame(f).fmask->as<uint32_t>() : nullptr;
+}
+const uint32_t* Plan::adj_mask(int f) const {
+    if (f < 0) return masked_grid(-1) && mask_ab.p ? mask_ab.as<uint32_t>() : nullptr;
+    return masked_grid(f) ? frame(f).amask->as<uint32_t>() : nullptr;
+}
+const int2* Plan::ivals_of(int f) const {
+    if (f < 0) return ival_grid(-1) && ival_b.p ? ival_b.as<int2>() : nullptr;
+    return ival_grid(f) ? frame(f).ival->as<int2>() : nullptr;
 }
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
@@ -659,8 +722,7 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s, false,
-                               masked_grid(f) ? fr.fmask->as<uint32_t>() : nullptr);
+            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s, false, fwd_mask(f), ivals_of(f));
         } else {
             Geom g = geom();
             prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
@@ -696,7 +758,7 @@ void Plan::ensure_fft_scratch() {
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
 void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s,
-                              bool bf, const uint32_t* out_mask) {
+                              bool bf, const uint32_t* out_mask, const int2* out_ival) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     int64_t S = nCh;
@@ -721,8 +783,9 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
         p.inverse = false;
         p.bf_in = bf && a > 0;
         p.bf_out = bf;
-        if (a == imDim - 1 && out_mask) {
+        if (a == imDim - 1 && (out_mask || out_ival)) {
             p.out_mask = out_mask;
+            p.out_ival = out_ival;
             p.mask_nch = nCh;
         }
         if (a == 0) {
@@ -740,7 +803,7 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
 // each pass keeping only the central n_a rows; the pass along x ends with
 // .* KFC_conj (and the coil sum when combine).
 void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, int nb, DevBuf& ba,
-                              DevBuf& bb, hipStream_t s, bool bf, const uint32_t* in_mask) {
+                              DevBuf& bb, hipStream_t s, bool bf, const uint32_t* in_mask, const int2* in_ival) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     for (int a = imDim - 1; a >= 0; a--) {
@@ -765,8 +828,9 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         p.scale = 1.0f / (float)N[a];
         p.bf_in = bf;
         p.bf_out = bf && a > 0;
-        if (a == imDim - 1 && in_mask) {
+        if (a == imDim - 1 && (in_mask || in_ival)) {
             p.in_mask = in_mask;
+            p.in_ival = in_ival;
             p.mask_nch = nCh;
         }
         if (a == 0) {
@@ -790,17 +854,16 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     ensure_grid();
     Geom g = geom();
     float2* gr = grid.as<float2>();
-    const bool masked = masked_grid(f);
+    const bool limited = masked_grid(f) || ival_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s, nullptr, !masked);
+        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s, nullptr, !limited);
     }
     {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s, false,
-                               masked ? fr.amask->as<uint32_t>() : nullptr);
+            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s, false, adj_mask(f), ivals_of(f));
         } else {
             cfft_nd(gr, imDim, N, nCh, true, s);
             epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
@@ -875,6 +938,19 @@ void Plan::build_batch(hipStream_t s) {
                                     hipMemcpyDeviceToDevice, s));
         }
     }
+    // stacked row intervals (when every frame has them)
+    ival_b.release();
+    {
+        bool all = true;
+        for (int f = 0; f < nFr; f++) all = all && frame(f).ivals;
+        if (all) {
+            const size_t ib = frame(0).ival->bytes;
+            ival_b.alloc(ib * nFr);
+            for (int f = 0; f < nFr; f++)
+                MONA_HIP(hipMemcpyAsync(static_cast<char*>(ival_b.p) + f * ib, frame(f).ival->p, ib,
+                                        hipMemcpyDeviceToDevice, s));
+        }
+    }
     auto gu = std::make_shared<SparseDev>();
     sparse_block_diag(*gu, us, ucol, urow, (int)(nFr * Nu), (int)poff, &view_u, s);
     auto gut = std::make_shared<SparseDev>();
@@ -894,8 +970,7 @@ void Plan::forward_batch(const float2* x, float2* y, hipStream_t s, int nA) {
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s, false,
-                           masked_grid(-1) && mask_fb.p ? mask_fb.as<uint32_t>() : nullptr);
+        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s, false, fwd_mask(-1), ivals_of(-1));
     }
     {
         TimedRegion t(0, s);
@@ -912,15 +987,15 @@ void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scal
     grid_b.ensure((size_t)nFr * Nu * nCh * sizeof(float2));
     fft_ab.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
-    const bool masked = masked_grid(-1) && mask_ab.p;
+    const bool limited = fwd_mask(-1) || ivals_of(-1);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA], !masked);
+        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA], !limited);
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s, false,
-                           masked ? mask_ab.as<uint32_t>() : nullptr);
+        adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s, false, adj_mask(-1),
+                           ivals_of(-1));
     }
 }
 
@@ -1056,7 +1131,7 @@ size_t sparse_bytes(const SparseDev& S) {
 
 size_t Plan::device_bytes() const {
     size_t b = K.bytes + KFC.bytes + KFCc.bytes + grid.bytes + fft_a.bytes + fft_b.bytes + grid_b.bytes +
-               fft_ab.bytes + fft_bb.bytes + mask_fb.bytes + mask_ab.bytes;
+               fft_ab.bytes + fft_bb.bytes + mask_fb.bytes + mask_ab.bytes + ival_b.bytes;
     const Work& w = work;
     for (const DevBuf* d : {&w.hy, &w.k1, &w.k2, &w.i1, &w.i2, &w.i3, &w.i4, &w.sc, &w.partial, &w.hist, &w.ys,
                             &w.hy0, &w.kpub, &w.z1, &w.z2, &w.z3, &w.z4, &w.z5})
@@ -1189,8 +1264,7 @@ void Plan::forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s) {
     ensure_fft_scratch();
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true,
-                           masked_grid(f) ? fr.fmask->as<uint32_t>() : nullptr);
+        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true, fwd_mask(f), ivals_of(f));
     }
     {
         TimedRegion t(0, s);
@@ -1204,15 +1278,14 @@ void Plan::adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, floa
     const Frame& fr = frame(f);
     ensure_grid();
     ensure_fft_scratch();
-    const bool masked = masked_grid(f);
+    const bool limited = masked_grid(f) || ival_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s, nullptr, !masked);
+        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s, nullptr, !limited);
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true,
-                           masked ? fr.amask->as<uint32_t>() : nullptr);
+        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true, adj_mask(f), ivals_of(f));
     }
 }
 

@@ -36,6 +36,11 @@ struct Frame {
     // by the adjoint gridding (amask)
     std::shared_ptr<DevBuf> fmask, amask;
     bool masked = false;             // sparse frame: Gut_s stores touched cells only, passes masked
+    // dense frame with row intervals: per column of the last-axis pass the
+    // rows [lo, hi) spanning its touched cells; Gut_s stores the cells
+    // inside them only and the last-axis passes skip the rest
+    std::shared_ptr<DevBuf> ival;
+    bool ivals = false;
     int64_t P = 0;
 };
 
@@ -87,11 +92,19 @@ struct Plan {
     // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
     // (bf: grid-side arrays stored as bf16x2)
     void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s,
-                            bool bf = false, const uint32_t* out_mask = nullptr);
+                            bool bf = false, const uint32_t* out_mask = nullptr, const int2* out_ival = nullptr);
     void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, int nb, DevBuf& a, DevBuf& b,
-                            hipStream_t s, bool bf = false, const uint32_t* in_mask = nullptr);
-    // frame f (f < 0: every frame of the plan) runs with cell masks
+                            hipStream_t s, bool bf = false, const uint32_t* in_mask = nullptr,
+                            const int2* in_ival = nullptr);
+    // the last-axis pass can be confined (register kernel, fused x pass)
+    bool limits_usable() const;
+    // frame f (f < 0: every frame of the plan) runs with cell masks / row intervals
     bool masked_grid(int f) const;
+    bool ival_grid(int f) const;
+    // the grid-side confinement of frame f for the forward / adjoint passes
+    const uint32_t* fwd_mask(int f) const;
+    const uint32_t* adj_mask(int f) const;
+    const int2* ivals_of(int f) const;
     int64_t fft_scratch_elems() const;
     void ensure_fft_scratch();
 
@@ -101,6 +114,7 @@ struct Plan {
     std::vector<int64_t> koff_b;       // k-space offset (float2) of each frame
     DevBuf grid_b, fft_ab, fft_bb;
     DevBuf mask_fb, mask_ab;           // stacked cell masks of the frames
+    DevBuf ival_b;                     // stacked row intervals of the frames
     bool use_batch = false;            // teva_chain uses the batched operators
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)
