@@ -445,12 +445,11 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
 // loads and stores cover runs of consecutive lines (inner positions) in
 // consecutive rows.  MODE as for the LDS kernel; the coil sum of MODE 2 goes
 // through a second LDS pass and adds the coils in order.
-// SPLIT (plain passes only): the exchange goes through LDS in two halves of
-// eight planes (k1 < 8, then k1 >= 8), so a tile needs half the exchange
-// space and more tiles fit on a CU.
+// SPLIT: the exchange goes through LDS in two halves of eight planes
+// (k1 < 8, then k1 >= 8), so a tile needs half the exchange space and more
+// tiles fit on a CU.
 template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
 __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
-    static_assert(!SPLIT || MODE == 0, "split exchange is for plain passes");
     const TI* __restrict__ in = reinterpret_cast<const TI*>(a.in);
     TO* __restrict__ out = reinterpret_cast<TO*>(a.out);
     constexpr int N1 = 16;
@@ -744,39 +743,40 @@ void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, co
 
 // Storage variants: float2 throughout, or bf16x2 for the grid-side arrays
 // (MODE 1 writes, MODE 0 reads and writes, MODE 2 reads bf16x2).
-template <int N2>
-void launch_reg(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsigned nblocks, int nt, size_t lds,
-                const float2* tw, hipStream_t s, bool split) {
-    if (split) {
-        MONA_CHECK(mode == 0 && bf_in == bf_out, MONA_ERR_INTERNAL, "split exchange is for plain passes");
-        if (bf_in) launch_reg_mode<N2, 0, bf16x2, bf16x2, true>(a, nblocks, nt, lds, tw, s);
-        else launch_reg_mode<N2, 0, float2, float2, true>(a, nblocks, nt, lds, tw, s);
-        return;
-    }
+template <int N2, bool SPLIT>
+void launch_reg_split(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsigned nblocks, int nt, size_t lds,
+                      const float2* tw, hipStream_t s) {
     if (bf_in || bf_out) {
         switch (mode) {
             case 1:
                 MONA_CHECK(bf_out, MONA_ERR_INTERNAL, "bad bf16 pass");
-                launch_reg_mode<N2, 1, float2, bf16x2>(a, nblocks, nt, lds, tw, s);
+                launch_reg_mode<N2, 1, float2, bf16x2, SPLIT>(a, nblocks, nt, lds, tw, s);
                 break;
             case 2:
                 MONA_CHECK(bf_in, MONA_ERR_INTERNAL, "bad bf16 pass");
-                launch_reg_mode<N2, 2, bf16x2, float2>(a, nblocks, nt, lds, tw, s);
+                launch_reg_mode<N2, 2, bf16x2, float2, SPLIT>(a, nblocks, nt, lds, tw, s);
                 break;
             case 0:
                 MONA_CHECK(bf_in && bf_out, MONA_ERR_INTERNAL, "bad bf16 pass");
-                launch_reg_mode<N2, 0, bf16x2, bf16x2>(a, nblocks, nt, lds, tw, s);
+                launch_reg_mode<N2, 0, bf16x2, bf16x2, SPLIT>(a, nblocks, nt, lds, tw, s);
                 break;
             default: MONA_CHECK(false, MONA_ERR_INTERNAL, "bf16 storage has no per-coil epilogue");
         }
         return;
     }
     switch (mode) {
-        case 1: launch_reg_mode<N2, 1, float2, float2>(a, nblocks, nt, lds, tw, s); break;
-        case 2: launch_reg_mode<N2, 2, float2, float2>(a, nblocks, nt, lds, tw, s); break;
-        case 3: launch_reg_mode<N2, 3, float2, float2>(a, nblocks, nt, lds, tw, s); break;
-        default: launch_reg_mode<N2, 0, float2, float2>(a, nblocks, nt, lds, tw, s); break;
+        case 1: launch_reg_mode<N2, 1, float2, float2, SPLIT>(a, nblocks, nt, lds, tw, s); break;
+        case 2: launch_reg_mode<N2, 2, float2, float2, SPLIT>(a, nblocks, nt, lds, tw, s); break;
+        case 3: launch_reg_mode<N2, 3, float2, float2, SPLIT>(a, nblocks, nt, lds, tw, s); break;
+        default: launch_reg_mode<N2, 0, float2, float2, SPLIT>(a, nblocks, nt, lds, tw, s); break;
     }
+}
+
+template <int N2>
+void launch_reg(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsigned nblocks, int nt, size_t lds,
+                const float2* tw, hipStream_t s, bool split) {
+    if (split) launch_reg_split<N2, true>(a, mode, bf_in, bf_out, nblocks, nt, lds, tw, s);
+    else launch_reg_split<N2, false>(a, mode, bf_in, bf_out, nblocks, nt, lds, tw, s);
 }
 
 }  // namespace
@@ -856,11 +856,12 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
     const int nt = 16 * L;
     MONA_CHECK(nt <= 1024 && (p.mode != 2 || a.W == p.S), MONA_ERR_INTERNAL, "bad register FFT tile");
-    size_t ex = std::max<size_t>((size_t)16 * xs, (size_t)a.Bo * p.out_cnt * (a.W | 1));
-    // plain passes whose exchange would hold a CU to fewer than three tiles
+    const size_t coil_sum = p.mode == 2 ? (size_t)a.Bo * p.out_cnt * (a.W | 1) : 0;
+    size_t ex = std::max<size_t>((size_t)16 * xs, coil_sum);
+    // passes whose exchange would hold a CU to fewer than three tiles
     // exchange in two halves
-    const bool split = p.mode == 0 && split_exchange() && ((size_t)N + ex) * sizeof(float2) > 53 * 1024;
-    if (split) ex = (size_t)8 * xs;
+    const bool split = split_exchange() && ((size_t)N + ex) * sizeof(float2) > 53 * 1024;
+    if (split) ex = std::max<size_t>((size_t)8 * xs, coil_sum);
     size_t lds = ((size_t)N + ex) * sizeof(float2);
     MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
     const float2* tw = twiddles().get(N);
