@@ -504,14 +504,17 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         for (int i = 1; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
         return (wv >> (row & 31)) & 1u;
     };
-    // interval passes: this work-item's column keeps rows [ilo, ihi)
+    // interval passes: this work-item's column keeps rows [ilo, ihi); a dead
+    // column (-1, -1) is neither loaded nor stored
     int ilo = 0, ihi = N;
+    bool dead = false;
     {
         const int2* iv = MODE == 0 ? (a.in_ival ? a.in_ival : a.out_ival) : nullptr;
         if (iv && live) {
             const int2 q = iv[o * a.mask_cpl + ws / a.mask_nch];
             ilo = q.x;
             ihi = q.y;
+            dead = q.x < 0;
         }
     }
     float2 v[N1];
@@ -520,7 +523,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         for (int n1 = 0; n1 < N1; n1++) {
             const int r = j + N2 * n1 - a.in_off;
             float2 x = make_float2(0.f, 0.f);
-            const bool take = live && r >= 0 && r < a.in_cnt;
+            const bool take = live && !dead && r >= 0 && r < a.in_cnt;
             if (MODE == 0 && a.in_ival) {
                 // rows outside the column's interval are zero and not loaded
                 const int row = j + N2 * n1;
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         }
         return;
     }
-    if (!live) return;
+    if (!live || dead) return;
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
         const int k = j + N1 * k2;

@@ -593,6 +593,26 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         for (auto& q : ival)
             if (q.y <= q.x) q = make_int2(0, 0);
     }
+    // 3-D: the span over y of the non-empty columns of each x; columns
+    // outside it are dead (-1, -1): no pass loads or stores them
+    std::vector<int2> ivaly;
+    if (use_ival && imDim == 3) {
+        std::vector<int2> span(N[0], make_int2(N[1], 0));
+        for (int y = 0; y < N[1]; y++)
+            for (int x = 0; x < N[0]; x++)
+                if (ival[(int64_t)y * N[0] + x].y > 0) {
+                    span[x].x = std::min(span[x].x, y);
+                    span[x].y = std::max(span[x].y, y + 1);
+                }
+        for (auto& q : span)
+            if (q.y <= q.x) q = make_int2(0, 0);
+        for (int y = 0; y < N[1]; y++)
+            for (int x = 0; x < N[0]; x++)
+                if (y < span[x].x || y >= span[x].y) ival[(int64_t)y * N[0] + x] = make_int2(-1, -1);
+        ivaly.resize((size_t)n[2] * N[0]);
+        for (int z = 0; z < n[2]; z++)
+            for (int x = 0; x < N[0]; x++) ivaly[(size_t)z * N[0] + x] = span[x];
+    }
     auto inside = [&](int64_t cell) {
         const int64_t z = cell / colsa, c = cell - z * colsa;
         return z >= ival[c].x && z < ival[c].y;
@@ -647,10 +667,14 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         dam = std::make_shared<DevBuf>((size_t)nw * sizeof(uint32_t));
         MONA_HIP(hipMemcpyAsync(dam->p, amask.data(), (size_t)nw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     }
-    std::shared_ptr<DevBuf> div;
+    std::shared_ptr<DevBuf> div, divy;
     if (use_ival) {
         div = std::make_shared<DevBuf>((size_t)colsa * sizeof(int2));
         MONA_HIP(hipMemcpyAsync(div->p, ival.data(), (size_t)colsa * sizeof(int2), hipMemcpyHostToDevice, s));
+    }
+    if (!ivaly.empty()) {
+        divy = std::make_shared<DevBuf>(ivaly.size() * sizeof(int2));
+        MONA_HIP(hipMemcpyAsync(divy->p, ivaly.data(), ivaly.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     }
     auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
     MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -667,6 +691,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     fr.amask = dam;
     fr.masked = sparse_frame;
     fr.ival = div;
+    fr.ivaly = divy;
     fr.ivals = use_ival;
     Gu_b.reset();  // the batched matrices are rebuilt on next use
     Gut_b.reset();
@@ -712,6 +737,11 @@ const int2* Plan::ivals_of(int f) const {
     if (f < 0) return ival_grid(-1) && ival_b.p ? ival_b.as<int2>() : nullptr;
     return ival_grid(f) ? frame(f).ival->as<int2>() : nullptr;
 }
+// (with ivals_of: the two go together, dead columns rely on both)
+const int2* Plan::mid_ivals_of(int f) const {
+    if (f < 0) return ival_grid(-1) && ival_b.p && ivaly_b.p ? ivaly_b.as<int2>() : nullptr;
+    return ival_grid(f) && frame(f).ivaly ? frame(f).ivaly->as<int2>() : nullptr;
+}
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
     MONA_CHECK(coils_set, MONA_ERR_STATE, "coil maps not set");
@@ -722,7 +752,7 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s, false, fwd_mask(f), ivals_of(f));
+            forward_fft_pruned(x, gr, 1, fft_a, fft_b, s, false, fwd_mask(f), ivals_of(f), mid_ivals_of(f));
         } else {
             Geom g = geom();
             prologue_kernel<<<nb256(Nu * nCh), 256, 0, s>>>(x, KFC.as<float2>(), gr, g);
@@ -758,7 +788,7 @@ void Plan::ensure_fft_scratch() {
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
 void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, DevBuf& bb, hipStream_t s,
-                              bool bf, const uint32_t* out_mask, const int2* out_ival) {
+                              bool bf, const uint32_t* out_mask, const int2* out_ival, const int2* mid_ival) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     int64_t S = nCh;
@@ -788,6 +818,10 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
             p.out_ival = out_ival;
             p.mask_nch = nCh;
         }
+        if (a == 1 && imDim == 3 && mid_ival) {
+            p.out_ival = mid_ival;
+            p.mask_nch = nCh;
+        }
         if (a == 0) {
             p.mode = 1;
             p.img = x;
@@ -803,7 +837,8 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
 // each pass keeping only the central n_a rows; the pass along x ends with
 // .* KFC_conj (and the coil sum when combine).
 void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float scale, int nb, DevBuf& ba,
-                              DevBuf& bb, hipStream_t s, bool bf, const uint32_t* in_mask, const int2* in_ival) {
+                              DevBuf& bb, hipStream_t s, bool bf, const uint32_t* in_mask, const int2* in_ival,
+                              const int2* mid_ival) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
     for (int a = imDim - 1; a >= 0; a--) {
@@ -831,6 +866,10 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         if (a == imDim - 1 && (in_mask || in_ival)) {
             p.in_mask = in_mask;
             p.in_ival = in_ival;
+            p.mask_nch = nCh;
+        }
+        if (a == 1 && imDim == 3 && mid_ival) {
+            p.in_ival = mid_ival;
             p.mask_nch = nCh;
         }
         if (a == 0) {
@@ -863,7 +902,8 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
         TimedRegion t(2, s);
         if (fft_pass_fusable(N[0], nCh)) {
             ensure_fft_scratch();
-            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s, false, adj_mask(f), ivals_of(f));
+            adjoint_fft_pruned(gr, x, combine, scale, 1, fft_a, fft_b, s, false, adj_mask(f), ivals_of(f),
+                               mid_ivals_of(f));
         } else {
             cfft_nd(gr, imDim, N, nCh, true, s);
             epilogue_kernel<<<nb256(nu), 256, 0, s>>>(gr, KFCc.as<float2>(), x, g, combine ? 1 : 0, scale);
@@ -940,6 +980,7 @@ void Plan::build_batch(hipStream_t s) {
     }
     // stacked row intervals (when every frame has them)
     ival_b.release();
+    ivaly_b.release();
     {
         bool all = true;
         for (int f = 0; f < nFr; f++) all = all && frame(f).ivals;
@@ -949,6 +990,13 @@ void Plan::build_batch(hipStream_t s) {
             for (int f = 0; f < nFr; f++)
                 MONA_HIP(hipMemcpyAsync(static_cast<char*>(ival_b.p) + f * ib, frame(f).ival->p, ib,
                                         hipMemcpyDeviceToDevice, s));
+            if (frame(0).ivaly) {
+                const size_t yb = frame(0).ivaly->bytes;
+                ivaly_b.alloc(yb * nFr);
+                for (int f = 0; f < nFr; f++)
+                    MONA_HIP(hipMemcpyAsync(static_cast<char*>(ivaly_b.p) + f * yb, frame(f).ivaly->p, yb,
+                                            hipMemcpyDeviceToDevice, s));
+            }
         }
     }
     auto gu = std::make_shared<SparseDev>();
@@ -970,7 +1018,8 @@ void Plan::forward_batch(const float2* x, float2* y, hipStream_t s, int nA) {
     fft_bb.ensure((size_t)nFr * fft_scratch_elems() * sizeof(float2));
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s, false, fwd_mask(-1), ivals_of(-1));
+        forward_fft_pruned(x, grid_b.as<float2>(), nA, fft_ab, fft_bb, s, false, fwd_mask(-1), ivals_of(-1),
+                           mid_ivals_of(-1));
     }
     {
         TimedRegion t(0, s);
@@ -995,7 +1044,7 @@ void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scal
     {
         TimedRegion t(2, s);
         adjoint_fft_pruned(grid_b.as<float2>(), x, true, scale, nA, fft_ab, fft_bb, s, false, adj_mask(-1),
-                           ivals_of(-1));
+                           ivals_of(-1), mid_ivals_of(-1));
     }
 }
 
@@ -1131,7 +1180,7 @@ size_t sparse_bytes(const SparseDev& S) {
 
 size_t Plan::device_bytes() const {
     size_t b = K.bytes + KFC.bytes + KFCc.bytes + grid.bytes + fft_a.bytes + fft_b.bytes + grid_b.bytes +
-               fft_ab.bytes + fft_bb.bytes + mask_fb.bytes + mask_ab.bytes + ival_b.bytes;
+               fft_ab.bytes + fft_bb.bytes + mask_fb.bytes + mask_ab.bytes + ival_b.bytes + ivaly_b.bytes;
     const Work& w = work;
     for (const DevBuf* d : {&w.hy, &w.k1, &w.k2, &w.i1, &w.i2, &w.i3, &w.i4, &w.sc, &w.partial, &w.hist, &w.ys,
                             &w.hy0, &w.kpub, &w.z1, &w.z2, &w.z3, &w.z4, &w.z5})
@@ -1264,7 +1313,8 @@ void Plan::forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s) {
     ensure_fft_scratch();
     {
         TimedRegion t(2, s);
-        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true, fwd_mask(f), ivals_of(f));
+        forward_fft_pruned(x, grid.as<float2>(), 1, fft_a, fft_b, s, true, fwd_mask(f), ivals_of(f),
+                           mid_ivals_of(f));
     }
     {
         TimedRegion t(0, s);
@@ -1285,7 +1335,8 @@ void Plan::adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, floa
     }
     {
         TimedRegion t(2, s);
-        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true, adj_mask(f), ivals_of(f));
+        adjoint_fft_pruned(grid.as<float2>(), x, true, scale, 1, fft_a, fft_b, s, true, adj_mask(f), ivals_of(f),
+                           mid_ivals_of(f));
     }
 }
 

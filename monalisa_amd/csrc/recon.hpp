@@ -40,6 +40,10 @@ struct Frame {
     // rows [lo, hi) spanning its touched cells; Gut_s stores the cells
     // inside them only and the last-axis passes skip the rest
     std::shared_ptr<DevBuf> ival;
+    // 3-D: per x, the rows [lo, hi) of the middle-axis pass spanning the
+    // columns with cells (one copy per pruned z row); columns outside that
+    // span are marked (-1, -1) in ival and skipped by both passes
+    std::shared_ptr<DevBuf> ivaly;
     bool ivals = false;
     int64_t P = 0;
 };
@@ -92,10 +96,11 @@ struct Plan {
     // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
     // (bf: grid-side arrays stored as bf16x2)
     void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s,
-                            bool bf = false, const uint32_t* out_mask = nullptr, const int2* out_ival = nullptr);
+                            bool bf = false, const uint32_t* out_mask = nullptr, const int2* out_ival = nullptr,
+                            const int2* mid_ival = nullptr);
     void adjoint_fft_pruned(const float2* grid, float2* x, bool combine, float scale, int nb, DevBuf& a, DevBuf& b,
                             hipStream_t s, bool bf = false, const uint32_t* in_mask = nullptr,
-                            const int2* in_ival = nullptr);
+                            const int2* in_ival = nullptr, const int2* mid_ival = nullptr);
     // the last-axis pass can be confined (register kernel, fused x pass)
     bool limits_usable() const;
     // frame f (f < 0: every frame of the plan) runs with cell masks / row intervals
@@ -105,6 +110,7 @@ struct Plan {
     const uint32_t* fwd_mask(int f) const;
     const uint32_t* adj_mask(int f) const;
     const int2* ivals_of(int f) const;
+    const int2* mid_ivals_of(int f) const;
     int64_t fft_scratch_elems() const;
     void ensure_fft_scratch();
 
@@ -114,7 +120,7 @@ struct Plan {
     std::vector<int64_t> koff_b;       // k-space offset (float2) of each frame
     DevBuf grid_b, fft_ab, fft_bb;
     DevBuf mask_fb, mask_ab;           // stacked cell masks of the frames
-    DevBuf ival_b;                     // stacked row intervals of the frames
+    DevBuf ival_b, ivaly_b;            // stacked row intervals of the frames
     bool use_batch = false;            // teva_chain uses the batched operators
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)
