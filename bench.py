@@ -115,10 +115,13 @@ def build_binned_problem(cfg, seed=4321):
     moving = synth.ellipsoid_phantom(n_u, rng, n_ellipsoids=3) - synth.ellipsoid_phantom(n_u, rng, n_ellipsoids=0)
     Gs, ves, ys, xs = [], [], [], []
     gen = torch.Generator(device="cuda").manual_seed(seed)
+    t_mat = 0.0
     for f, sp in enumerate(bins):
         tb = np.ascontiguousarray(t[:, :, sp].reshape(3, -1, order="F"))
         ve = synth.ve_radial3(tb, nSample, len(sp))
-        G = bmTraj2SparseMat(tb, ve, N_u, dK_u, "kaiser")
+        tm = time.time()
+        G = bmTraj2SparseMat(tb, ve, N_u, dK_u, "kaiser", device=True)
+        t_mat += time.time() - tm
         P.set_frame(f, G["Gu"], G["Gut"])
         xf = (base + 0.3 * np.sin(2 * np.pi * f / nBin) * moving).astype(np.complex64)
         y = P.bmShanna(torch.from_numpy(xf).cuda(), f=f)
@@ -138,7 +141,7 @@ def build_binned_problem(cfg, seed=4321):
     log(f"[bench] {nBin} bins, points per bin {min(len(b) for b in bins) * nSample}.."
         f"{max(len(b) for b in bins) * nSample}, built in {time.time() - t0:.1f}s")
     return dict(P=P, G=Gs, ys=ys, ves=ves, ve_max=ve_max, x0=x0, delta=delta, rho=10 * delta, xs=xs,
-                n_u=n_u, N_u=N_u, nCh=nCh, nBin=nBin, bins=bins)
+                n_u=n_u, N_u=N_u, nCh=nCh, nBin=nBin, bins=bins, build_s={"device_all_bins": t_mat})
 
 
 def bench_binned(args, cfg):
@@ -504,7 +507,7 @@ def cpu_baseline(prob, threads):
                 nufft_pairs_per_s=1.0 / (t_fwd + t_adj))
 
 
-def main():
+def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -516,7 +519,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="start with a 1 GiB device copy (a known-size kernel for counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
