@@ -326,6 +326,15 @@ int mona_sensa_morphosia(mona_plan P, void* x, const void* y, const long long* y
 int mona_nasha(mona_plan P, mona_sparse Gn, const void* y, const void* C, void* x,
                hipStream_t stream);
 
+/* Gridded reconstruction after a direct gridder (bmMathilda.m:14-125): grid
+ * [prod(N_u)][nCh] float2 (device; the normalised n2u that
+ * bmGridder_n2u[_leight] returns, cells x fastest, coil fastest within a
+ * cell) -> centred inverse FFT times prod(N_u) prod(dK_u) (bmIDF*) -> .* K
+ * -> crop to n_u -> coil combine with C (bmCoilSense_pinv.m:6-12), or
+ * per-coil images [prod(n_u)][nCh] when C is NULL.  Needs the plan's coil
+ * maps (the deapodisation K is computed there). */
+int mona_mathilda(mona_plan P, const void* grid, const void* C, void* x, hipStream_t stream);
+
 /* bmSensa over frames 0 .. nFrames-1 of the plan at once (nFrames <= 0: all;
  * no reference counterpart: the batched form of the serving loop).  Frame
  * f solves its own system

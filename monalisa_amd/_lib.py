@@ -73,6 +73,7 @@ SIGNATURES = {
     "mona_morphosia": (I, [P, I, I, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
     "mona_sensa_morphosia": (I, [P, P, P, LLP, P, LLP, F, I, I, DP, P]),
     "mona_teva_chain": (I, [P, P, P, P, I, P, LLP, P, LLP, F, FP, FP, I, I, DP, P]),
+    "mona_mathilda": (I, [P, P, P, P, P]),
     "mona_nasha": (I, [P, P, P, P, P, P]),
     "mona_sensa_batch": (I, [P, I, P, P, LLP, FP, LLP, FP, I, I, DP, P]),
     "mona_plan_memory": (I, [P, LLP]),

@@ -5,6 +5,7 @@
 //                         frame chains and sheets, motion-compensated by deformation
 //                         matrices Tu/Tut (imDim_morphosia/nonCartesian/*.m)
 //   bmNasha               n2u with Gn + centred ifft + deapodisation (bmNasha.m:6-91)
+//   bmMathilda            the same tail after a direct gridder (bmMathilda.m:14-125)
 //
 // The finite differences, their transposes and the soft-threshold prox are
 // fused elementwise kernels; inner CG scalars are FP64 on the device.
@@ -757,16 +758,13 @@ void sensa_morphosia(Plan& P, float2* x, const float2* y, const long long* y_off
     }
 }
 
-// ================================================================ bmNasha
-void nasha(Plan& P, const SparseDev& Gn, const float2* y, const float2* C, float2* x, hipStream_t s) {
-    MONA_CHECK(P.coils_set, MONA_ERR_STATE, "coil maps not set (the deapodisation is computed there)");
-    MONA_CHECK(Gn.l_size == P.Nu, MONA_ERR_ARG, "Gn has l_size != prod(N_u)");
-    P.ensure_grid();
+// ================================================================ bmNasha / bmMathilda
+// Image from the gridded data in P.grid: centred inverse FFT times
+// prod(N_u) prod(dK_u) (bmIDF*), times K, crop to n_u, coil combine with C
+// (bmCoilSense_pinv) or per-coil images when C is NULL (bmNasha.m:66-87,
+// bmMathilda.m:94-123).
+void image_from_grid(Plan& P, const float2* C, float2* x, hipStream_t s) {
     float2* gr = P.grid.as<float2>();
-    {
-        TimedRegion t(1, s);
-        spmv_c2_rowmajor(Gn, y, gr, P.nCh, nullptr, s);
-    }
     {
         TimedRegion t(2, s);
         cfft_nd(gr, P.imDim, P.N, P.nCh, true, s);
@@ -777,6 +775,26 @@ void nasha(Plan& P, const SparseDev& Gn, const float2* y, const float2* C, float
     Geom g = P.geom();
     k_nasha_epilogue<<<nb(P.nu), 256, 0, s>>>(gr, P.K.as<float>(), C, x, g, F);
     MONA_HIP(hipGetLastError());
+}
+
+void nasha(Plan& P, const SparseDev& Gn, const float2* y, const float2* C, float2* x, hipStream_t s) {
+    MONA_CHECK(P.coils_set, MONA_ERR_STATE, "coil maps not set (the deapodisation is computed there)");
+    MONA_CHECK(Gn.l_size == P.Nu, MONA_ERR_ARG, "Gn has l_size != prod(N_u)");
+    P.ensure_grid();
+    {
+        TimedRegion t(1, s);
+        spmv_c2_rowmajor(Gn, y, P.grid.as<float2>(), P.nCh, nullptr, s);
+    }
+    image_from_grid(P, C, x, s);
+}
+
+// bmMathilda (bmMathilda.m:14-125) after its gridder: grid [Nu][nCh]
+// (device, the normalised n2u of bmGridder_n2u[_leight]) to the image.
+void mathilda(Plan& P, const float2* grid, const float2* C, float2* x, hipStream_t s) {
+    MONA_CHECK(P.coils_set, MONA_ERR_STATE, "coil maps not set (the deapodisation is computed there)");
+    P.ensure_grid();
+    MONA_HIP(hipMemcpyAsync(P.grid.p, grid, (size_t)P.Nu * P.nCh * sizeof(float2), hipMemcpyDeviceToDevice, s));
+    image_from_grid(P, C, x, s);
 }
 
 }  // namespace mona
@@ -848,6 +866,14 @@ int mona_plan_set_deform(mona_plan P, int set, int frame, mona_sparse Tu, mona_s
         auto& d = pl.deform[set];
         if ((int)d.size() <= frame) d.resize(frame + 1, {nullptr, nullptr});
         d[frame] = {Tu ? &Tu->S : nullptr, Tut ? &Tut->S : nullptr};
+    });
+}
+
+int mona_mathilda(mona_plan P, const void* grid, const void* C, void* x, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(P && grid && x, MONA_ERR_ARG, "NULL argument");
+        mathilda(P->P, static_cast<const float2*>(grid), static_cast<const float2*>(C), static_cast<float2*>(x),
+                 stream);
     });
 }
 

@@ -29,6 +29,11 @@ Every function cites the reference file:line it follows.  What pins each one
   shift -> permutation, fractional 1-D shift -> the two normalised bump
   weights), a brute-force loop construction (``deform_dense_bruteforce``),
   deletion of points that leave the grid, Gut == Gu^T exactly.
+* ``gridder_n2u`` / ``gridder_n2u_leight`` / ``mathilda``
+  (tests/test_oracle_mathilda.py): constant data grids to exactly 1 on the
+  touched cells; the reconstruction through bmGridder_n2u's weights equals
+  the one through bmTraj2SparseMat's Gn (bmNasha); the leight restatement
+  equals the C port of the leight MEX.
 * ``steva``/``teva_chain``: parity unpinned beyond the above building blocks
   and monotone objective checks (no reference run exists to compare with);
   with deformations, identity matrices reproduce the undeformed chain.
@@ -1018,6 +1023,117 @@ def sensa_morphosia(x, y, ve, C, Gu, Gut, N_u, n_u, dK_u, nCGD, ve_max, nIter=1,
             p = dag + b * p
         hist.append(sq([yy - m for yy, m in zip(y, A(x))]))
     return x, hist
+
+
+def gridder_prep(t, Dn, N_u, dK_u, single):
+    """Grid coordinates of bmGridder_n2u(_leight).m:52-93: t / dK_u +
+    fix(N_u / 2 + 1) per axis (1-based), Dn / prod(dK_u), points outside
+    [1, N_u] removed.  single: the leight wrapper's float32 arithmetic."""
+    dt = np.float32 if single else np.float64
+    t = np.asarray(t, dtype=dt).copy()
+    Dn = np.ravel(np.asarray(Dn, dtype=dt)).copy()
+    keep = np.ones(t.shape[1], bool)
+    for d in range(t.shape[0]):
+        t[d] = t[d] / dt(dK_u[d])
+        Dn = Dn / dt(dK_u[d])
+        t[d] = t[d] + dt(np.fix(N_u[d] / 2 + 1))
+        keep &= (t[d] >= 1) & (t[d] <= N_u[d])
+    return t[:, keep], Dn[keep], keep
+
+
+def _normalised_accumulate(y, cells, w, Nu):
+    """b[cell] = sum w a / sum w with sum w < 1e-5 read as 1
+    (bmGridder_n2u_mex.cpp:72-201, leight3_mex.cpp:116-358)."""
+    nCh = y.shape[1]
+    b = np.zeros((Nu, nCh), complex)
+    nrm = np.zeros(Nu)
+    for k in range(nCh):
+        np.add.at(b[:, k], cells.ravel(), (w * y[:, k][:, None]).ravel())
+    np.add.at(nrm, cells.ravel(), w.ravel())
+    nrm[nrm < 1e-5] = 1.0
+    return b / nrm[:, None]
+
+
+def gridder_n2u(y, t, Dn, N_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """bmGridder_n2u (src/gridding123/m/bmGridder_n2u.m:15-216 with
+    bmGridder_n2u_mex.cpp:72-201), float64: neighbour offsets
+    -fix(nWin/2)..fix(nWin/2) (odd nWin, floor shift 0.5) or -nWin/2-1..nWin/2
+    (even), radial KB / Gauss weights times Dn, indices wrapped mod N_u,
+    normalised accumulation.  y [nPt, nCh]; returns [prod(N_u), nCh]."""
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    nWin = int(nWin)
+    N_u = [int(a) for a in N_u]
+    tg, Dn, keep = gridder_prep(t, Dn, N_u, dK_u, False)
+    y = np.asarray(y)[keep]
+    imDim, nPt = tg.shape
+    if nWin % 2 == 0:
+        r, shift = np.arange(-nWin // 2 - 1, nWin // 2 + 1), 0.0
+    else:
+        r, shift = np.arange(-(nWin // 2), nWin // 2 + 1), 0.5
+    c = np.stack([g.reshape(-1, order='F') for g in np.meshgrid(*([r] * imDim), indexing='ij')])
+    tf = np.floor(tg + shift)
+    d = np.sqrt((((tg - tf)[:, None, :] - c[:, :, None]) ** 2).sum(axis=0))
+    w = kernel_weight(d, kernelType, kernelParam) * Dn[None, :]
+    n = tf[:, None, :] + c[:, :, None]
+    lin = np.zeros(d.shape, np.int64)
+    stride = 1
+    for a in range(imDim):
+        lin += np.mod(n[a] - 1, N_u[a]).astype(np.int64) * stride
+        stride *= N_u[a]
+    return _normalised_accumulate(y, lin.T, w.T, int(np.prod(N_u)))
+
+
+def gridder_n2u_leight(y, t, Dn, N_u, dK_u, kernelType=None, nWin=None, kernelParam=None):
+    """bmGridder_n2u_leight (src/gridding123/m/bmGridder_n2u_leight.m:15-141
+    with leight3_mex.cpp:116-358), float64 after the wrapper's float32
+    coordinates: Gaussian weights (sigma = kernelParam[0], normalised with
+    the reference's 3.141592) on the box of nWin cells from trunc(t + 0.5)
+    (odd nWin) or trunc(t) (even), clipped at the grid edge."""
+    kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
+    nWin = int(nWin)
+    N_u = [int(a) for a in N_u]
+    tg, Dn, keep = gridder_prep(t, Dn, N_u, dK_u, True)
+    tg = tg.astype(np.float64)
+    Dn = Dn.astype(np.float64)
+    y = np.asarray(y)[keep]
+    imDim, nPt = tg.shape
+    shift, nh = (0.5, (nWin - 1) // 2) if nWin % 2 else (0.0, nWin // 2 - 1)
+    sigma = float(np.float32(kernelParam[0]))
+    g1 = 1.0 / (np.sqrt(2 * 3.141592) * sigma)
+    g2 = -1.0 / (2 * sigma * sigma)
+    u = np.arange(-nh, -nh + nWin)
+    offs = np.stack([g.reshape(-1, order='F') for g in np.meshgrid(*([u] * imDim), indexing='ij')])  # [d, nNb]
+    cf = np.trunc(tg + shift)
+    rest = tg - cf
+    cell0 = cf.astype(np.int64) - 1
+    nb = cell0[:, None, :] + offs[:, :, None]                       # [d, nNb, nPt] 0-based
+    inside = np.ones(nb.shape[1:], bool)
+    for a in range(imDim):
+        inside &= (nb[a] >= 0) & (nb[a] <= N_u[a] - 1)
+    dsq = ((rest[:, None, :] - offs[:, :, None]) ** 2).sum(axis=0)
+    w = g1 * np.exp(g2 * dsq) * Dn[None, :] * inside
+    lin = np.zeros(dsq.shape, np.int64)
+    stride = 1
+    for a in range(imDim):
+        lin += np.clip(nb[a], 0, N_u[a] - 1) * stride
+        stride *= N_u[a]
+    return _normalised_accumulate(y, lin.T, w.T, int(np.prod(N_u)))
+
+
+def mathilda(y, t, v, N_u, n_u, dK_u, C=None, kernel=(None, None, None), leight_flag=True):
+    """bmMathilda (src/fourier123/map_function/nonCartesian/bmMathilda.m:14-125):
+    gridder, bmIDF (centred ifft times prod(N_u dK_u)), .* K of the requested
+    kernel, crop, optional coil combination (bmCoilSense_pinv.m:6-12)."""
+    d = len(N_u)
+    grid = (gridder_n2u_leight if leight_flag else gridder_n2u)(y, t, v, N_u, dK_u, *kernel)
+    K = bmK(N_u, dK_u, *kernel)
+    g = _unflatF(grid, N_u)
+    g = cifft(g, axes=range(d)) * (np.prod(N_u) * np.prod(dK_u))
+    g = g * K[..., None]
+    g = crop(g, N_u, n_u)
+    if C is not None:
+        g = (np.conj(C) * g).sum(-1) / (np.abs(C) ** 2).sum(-1)
+    return g
 
 
 def nasha(y, Gn, N_u, n_u, dK_u, C=None, kernel=(None, None, None)):
