@@ -16,6 +16,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import lib, check, ptr, stream_of
+from .optim import n_iter
 from .sparse import BmSparseMat, KERNELS, kernel_defaults
 
 
@@ -147,6 +148,7 @@ class Plan:
     def bmSensa(self, x, y, ve, nCGD, ve_max=None, nIter=1, f=0, stream=None):
         """CG-SENSE in place on x (bmSensa.m:6-96); returns the per-outer-
         iteration residuals <y - Mx, HY (y - Mx)>."""
+        nIter = n_iter(nIter)
         _require_cuda(x, "x")
         _require_cuda(y, "y")
         _require_cuda(ve, "ve")
@@ -161,6 +163,7 @@ class Plan:
         CG), nFr = x.shape[0] <= the plan's frames: x [nFr, nu] in place;
         y_list / ve_list per frame; returns the residuals [nFr][nIter].  Each
         frame's result equals bmSensa on that frame alone."""
+        nIter = n_iter(nIter)
         import torch
         _require_cuda(x, "x")
         nFr = x.shape[0]
@@ -225,6 +228,7 @@ class Plan:
     def bmSteva(self, x, z, u, y, ve, delta, rho, nCGD, ve_max=None, nIter=1, f=0, stream=None):
         """Spatial-TV ADMM in place on x, z, u (bmSteva.m:6-138).  z, u are
         [imDim, nu] complex64 (given) or None (z = grad x, u = 0)."""
+        nIter = n_iter(nIter)
         import torch
         _require_cuda(x, "x")
         given = z is not None
@@ -247,6 +251,7 @@ class Plan:
         z = F x, u = 0).  Tu, Tut: lists of deformation matrices per frame
         (entries may be None), F x_f = Tu_f x_f - x_{f-1}; when not given,
         the ones set by set_deform (none: identity, Tu = [])."""
+        nIter = n_iter(nIter)
         import torch
         nFr = x.shape[0]
         self._set_deforms(nFr, 0, Tu, Tut)
@@ -274,6 +279,7 @@ class Plan:
     MORPH_SENSITIVA_CHAIN, MORPH_SENSITIVA_SHEET, MORPH_SENSITIVA_DUO_CHAIN = 3, 4, 5
 
     def _morph(self, solver, nFr1, x, zs, us, y_list, ve_list, delta, rho, nCGD, ve_max, nIter, stream, nT):
+        nIter = n_iter(nIter)
         import torch
         nFr = x.shape[0]
         if nFr1 is None:
@@ -363,6 +369,7 @@ class Plan:
         """bmSensaMorphosia_chain.m:6-104: one image x [nu] (complex64,
         updated in place) seen by every frame through Tu_f; returns the
         residual R after each iteration."""
+        nIter = n_iter(nIter)
         import torch
         nFr = len(y_list)
         self._set_deforms(nFr, 0, Tu, Tut)
