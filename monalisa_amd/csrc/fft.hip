@@ -760,8 +760,9 @@ void launch_reg_split(const PassArgs& a, int mode, bool bf_in, bool bf_out, unsi
                 launch_reg_mode<N2, 2, bf16x2, float2, SPLIT>(a, nblocks, nt, lds, tw, s);
                 break;
             case 0:
-                MONA_CHECK(bf_in && bf_out, MONA_ERR_INTERNAL, "bad bf16 pass");
-                launch_reg_mode<N2, 0, bf16x2, bf16x2, SPLIT>(a, nblocks, nt, lds, tw, s);
+                if (bf_in && bf_out) launch_reg_mode<N2, 0, bf16x2, bf16x2, SPLIT>(a, nblocks, nt, lds, tw, s);
+                else if (bf_in) launch_reg_mode<N2, 0, bf16x2, float2, SPLIT>(a, nblocks, nt, lds, tw, s);
+                else launch_reg_mode<N2, 0, float2, bf16x2, SPLIT>(a, nblocks, nt, lds, tw, s);
                 break;
             default: MONA_CHECK(false, MONA_ERR_INTERNAL, "bf16 storage has no per-coil epilogue");
         }

@@ -811,8 +811,11 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
         p.out_off = 0;
         p.out_cnt = N[a];
         p.inverse = false;
-        p.bf_in = bf && a > 0;
-        p.bf_out = bf;
+        // bf16 storage for the grid only: the pruned intermediates stay
+        // float2 (the passes are not bandwidth-bound; bf16 intermediates
+        // made them slower)
+        p.bf_in = false;
+        p.bf_out = bf && a == imDim - 1;
         if (a == imDim - 1 && (out_mask || out_ival)) {
             p.out_mask = out_mask;
             p.out_ival = out_ival;
@@ -861,8 +864,8 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         p.out_os = (int64_t)n[a] * S;
         p.inverse = true;
         p.scale = 1.0f / (float)N[a];
-        p.bf_in = bf;
-        p.bf_out = bf && a > 0;
+        p.bf_in = bf && a == imDim - 1;
+        p.bf_out = false;
         if (a == imDim - 1 && (in_mask || in_ival)) {
             p.in_mask = in_mask;
             p.in_ival = in_ival;
