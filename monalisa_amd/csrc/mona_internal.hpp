@@ -115,6 +115,22 @@ void scan_i32_to_i64_inclusive(const int* in, int64_t* out, int64_t n, hipStream
 void fill_f32(float* p, float v, int64_t n, hipStream_t s);
 void fill_i32(int* p, int v, int64_t n, hipStream_t s);
 
+// ---------------------------------------------------------------- launch order
+// Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs, each
+// with its own L2).  With G > 1 the logical block of workgroup b is chosen so
+// that each XCD receives runs of G consecutive logical blocks: neighbouring
+// blocks (gridding rows that gather the same points, FFT tiles on the same
+// rows) then share one L2, while the work still spreads over every XCD.
+// Blocks past the last full run of 8 G keep their index.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks, int G) {
+    if (G <= 1) return b;
+    const int64_t span = 8LL * G;
+    const int64_t full = nblocks / span * span;
+    if (b >= full) return b;
+    const int64_t xcd = b & 7, k = b >> 3;
+    return ((k / G) * 8 + xcd) * G + (k % G);
+}
+
 // ---------------------------------------------------------------- storage types
 // A complex value stored as two bfloat16 (real in the low half), the
 // reduced-precision storage of the mixed-precision solver.  Arithmetic is

@@ -266,21 +266,6 @@ __device__ __forceinline__ void row_sum_cv(const SpEntry* __restrict__ cv, int64
     }
 }
 
-// Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs, each
-// with its own L2).  With G > 1 the logical block of workgroup b is chosen so
-// that each XCD receives runs of G consecutive logical blocks: neighbouring
-// rows (which gather the same points) then share one L2, while the runs of
-// the dense k-space centre still spread over every XCD.  Blocks past the
-// last full run of 8 G keep their index.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks, int G) {
-    if (G <= 1) return b;
-    const int64_t span = 8LL * G;
-    const int64_t full = nblocks / span * span;
-    if (b >= full) return b;
-    const int64_t xcd = b & 7, k = b >> 3;
-    return ((k / G) * 8 + xcd) * G + (k % G);
-}
-
 // spmv_c2 with CPT vectors per work-item and packed entries: a quarter of
 // the vector-memory instructions of spmv_c2 at CPT = 4 (the metadata load
 // is shared by CPT vectors and the gathers are 16 bytes wide).
