@@ -34,9 +34,12 @@ Every function cites the reference file:line it follows.  What pins each one
   touched cells; the reconstruction through bmGridder_n2u's weights equals
   the one through bmTraj2SparseMat's Gn (bmNasha); the leight restatement
   equals the C port of the leight MEX.
-* ``steva``/``teva_chain``: parity unpinned beyond the above building blocks
-  and monotone objective checks (no reference run exists to compare with);
-  with deformations, identity matrices reproduce the undeformed chain.
+* ``steva``/``teva_chain``/``teva_duo_chain``/``teva_sheet`` (ADMM-L1,
+  tests/test_oracle_admm_limit.py): on tiny problems the ADMM iterates reach
+  the minimiser of 0.5 ||M x - y||^2_HY + delta HZ ||F x||_1 computed
+  independently (the bounded least-squares dual solved by BVLS, with F
+  built from the reference's private_F definitions); with deformations,
+  identity matrices reproduce the undeformed chain.
 * ``morphosia`` and its solvers (tests/test_oracle_morphosia.py): every
   difference term is the adjoint of its dagF; the duo chain with identities
   is the chain at rho/2; a one-column sheet is the chain; the sheet is
