@@ -57,7 +57,11 @@ const char* mona_version(void);
  * deltas or NULL when rows are not squeezed.
  * Replaces src/sparseMat/cpp/bmSparseMat_cC_oBlock_omp.h:9-14 (library
  * symbol) and the inner function of
- * src/sparseMat/mex/bmSparseMat_vec/bmSparseMat_cC_oBlock_omp/bmSparseMat_cC_oBlock_omp_mex.cpp:106-117. */
+ * src/sparseMat/mex/bmSparseMat_vec/bmSparseMat_cC_oBlock_omp/bmSparseMat_cC_oBlock_omp_mex.cpp:106-117.
+ * The reference's header declares the library symbol with C++ linkage; the
+ * library also defines that mangled twin
+ * (_Z25bmSparseMat_cC_oBlock_ompiPiS_PfiS_iS0_S0_iS0_S0_, cxx_linkage.cpp),
+ * so C++ code built against that header links unchanged. */
 void bmSparseMat_cC_oBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,
                                int l_size, int* l_jump, int l_nJump,
                                float* v_real, float* v_imag, int n_vec,

@@ -86,6 +86,11 @@ SIGNATURES = {
     "mona_timing_read": (I, [I, LLP, DP]),
 }
 
+# C++-linkage twin of bmSparseMat_cC_oBlock_omp (the reference's header
+# declares that one library symbol without extern "C"), by its mangled name.
+CXX_CC_OBLOCK_OMP = "_Z25bmSparseMat_cC_oBlock_ompiPiS_PfiS_iS0_S0_iS0_S0_"
+CXX_SIGNATURES = {CXX_CC_OBLOCK_OMP: SIGNATURES["bmSparseMat_cC_oBlock_omp"]}
+
 
 def _load():
     if not os.path.exists(LIB_PATH):
@@ -100,7 +105,7 @@ def _load():
     except ImportError:
         pass
     h = C.CDLL(LIB_PATH)
-    for name, (res, args) in SIGNATURES.items():
+    for name, (res, args) in list(SIGNATURES.items()) + list(CXX_SIGNATURES.items()):
         fn = getattr(h, name)
         fn.restype = res
         fn.argtypes = args

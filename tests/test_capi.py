@@ -27,6 +27,14 @@ def test_symbol_exported(name):
     assert hasattr(lib, name), name
 
 
+def test_cxx_linkage_twin_exported():
+    """C++ code compiled against the reference's bmSparseMat_cC_oBlock_omp.h
+    references the mangled name; the library defines it."""
+    from monalisa_amd import lib
+    from monalisa_amd._lib import CXX_CC_OBLOCK_OMP
+    assert getattr(lib, CXX_CC_OBLOCK_OMP) is not None
+
+
 def test_binding_table_covers_header():
     from monalisa_amd._lib import SIGNATURES
     assert set(declared_functions()) == set(SIGNATURES)

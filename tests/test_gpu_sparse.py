@@ -31,7 +31,9 @@ def test_cC_oBlock_omp_matches_reference_golden(case):
     name, M, vr, vi, wr, wi = case
     print(f"[gpu] cC_oBlock_omp golden case {name}", flush=True)
     L = lib()
-    for fn in (L.bmSparseMat_cC_oBlock_omp, L.bmSparseMat_cC_oBlock, L.bmSparseMat_cC_oBlock_2_omp):
+    from monalisa_amd._lib import CXX_CC_OBLOCK_OMP
+    for fn in (L.bmSparseMat_cC_oBlock_omp, getattr(L, CXX_CC_OBLOCK_OMP), L.bmSparseMat_cC_oBlock,
+               L.bmSparseMat_cC_oBlock_2_omp):
         gr, gi = gpu_cC(fn, M, vr, vi)
         assert np.array_equal(gr.view(np.int32), wr.view(np.int32))
         assert np.array_equal(gi.view(np.int32), wi.view(np.int32))
