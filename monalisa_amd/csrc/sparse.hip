@@ -226,12 +226,55 @@ __device__ __forceinline__ void st_n(bf16x2* p, const float* ar, const float* ai
     else p->v = u[0];
 }
 
-// Row sum over CPT vectors at once: v[col][k0 .. k0+CPT-1], entries packed
-// as (col, val).  Per vector the entries are added one by one in stored
-// order, exactly as row_sum does.
+// Vector offsets of work-item kq (of n_vec / CPT per row) for float2
+// vectors at CPT = 4: values k0, k0 + 1 and k1, k1 + 1.  With halves
+// (k0 = 2 kq, k1 = 2 kq + n_vec / 2) the first 16-byte loads of a row's
+// work-items cover one contiguous half of the n_vec values and the second
+// loads the other half, so each load instruction touches half the cache
+// lines of the plain order (k0 = 4 kq, k1 = k0 + 2).
+template <int CPT, class TV, class TW>
+struct VecSplit {
+    static constexpr bool halves = CPT == 4 && std::is_same<TV, float2>::value && std::is_same<TW, float2>::value;
+    __device__ static int k0(int kq, int) { return halves ? 2 * kq : kq * CPT; }
+    __device__ static int k1(int kq, int n_vec) { return halves ? 2 * kq + n_vec / 2 : kq * CPT + 2; }
+};
+// CPT values of one vector row: at k0 .. (plain), or the pairs at k0 and k1
+// for float2 at CPT = 4.
+template <int CPT>
+__device__ __forceinline__ void ld_v(const float2* p, int k0, int k1, float2* x) {
+    if constexpr (CPT == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(p + k0);
+        const float4 b = *reinterpret_cast<const float4*>(p + k1);
+        x[0] = make_float2(a.x, a.y);
+        x[1] = make_float2(a.z, a.w);
+        x[2] = make_float2(b.x, b.y);
+        x[3] = make_float2(b.z, b.w);
+    } else {
+        ld_n<CPT>(p + k0, x);
+    }
+}
+template <int CPT>
+__device__ __forceinline__ void ld_v(const bf16x2* p, int k0, int, float2* x) { ld_n<CPT>(p + k0, x); }
+template <int CPT>
+__device__ __forceinline__ void st_v(float2* p, int k0, int k1, const float* ar, const float* ai) {
+    if constexpr (CPT == 4) {
+        *reinterpret_cast<float4*>(p + k0) = make_float4(ar[0], ai[0], ar[1], ai[1]);
+        *reinterpret_cast<float4*>(p + k1) = make_float4(ar[2], ai[2], ar[3], ai[3]);
+    } else {
+        st_n<CPT>(p + k0, ar, ai);
+    }
+}
+template <int CPT>
+__device__ __forceinline__ void st_v(bf16x2* p, int k0, int, const float* ar, const float* ai) {
+    st_n<CPT>(p + k0, ar, ai);
+}
+
+// Row sum over CPT vectors at once: v[col] at the offsets k0 / k1 (ld_v),
+// entries packed as (col, val).  Per vector the entries are added one by one
+// in stored order, exactly as row_sum does.
 template <bool SCALE, int CPT, int U, class TV>
 __device__ __forceinline__ void row_sum_cv(const SpEntry* __restrict__ cv, int64_t jb, int64_t je, int n_vec,
-                                           int k0, const TV* __restrict__ v, const float* __restrict__ scale,
+                                           int k0, int k1, const TV* __restrict__ v, const float* __restrict__ scale,
                                            float* ar, float* ai) {
     for (int64_t j = jb; j < je; j += U) {
         const int cnt = (int)((je - j) < U ? (je - j) : U);
@@ -243,7 +286,7 @@ __device__ __forceinline__ void row_sum_cv(const SpEntry* __restrict__ cv, int64
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (u < cnt) {
-                ld_n<CPT>(v + (int64_t)e[u].col * n_vec + k0, x[u]);
+                ld_v<CPT>(v + (int64_t)e[u].col * n_vec, k0, k1, x[u]);
                 if (SCALE) {
                     const float sc = scale[e[u].col];
 #pragma unroll
@@ -283,9 +326,11 @@ spmv_cv(const int64_t* __restrict__ rowptr, const SpEntry* __restrict__ cv, cons
     float ar[CPT], ai[CPT];
 #pragma unroll
     for (int c = 0; c < CPT; c++) ar[c] = ai[c] = 0.0f;
-    row_sum_cv<SCALE, CPT, U>(cv, jb, je, n_vec, kq * CPT, v, scale, ar, ai);
+    using VS = VecSplit<CPT, TV, TW>;
+    const int k0 = VS::k0(kq, n_vec), k1 = VS::k1(kq, n_vec);
+    row_sum_cv<SCALE, CPT, U>(cv, jb, je, n_vec, k0, k1, v, scale, ar, ai);
     const int row = rowidx ? rowidx[i] : i;
-    st_n<CPT>(w + (int64_t)row * n_vec + kq * CPT, ar, ai);
+    st_v<CPT>(w + (int64_t)row * n_vec, k0, k1, ar, ai);
 }
 
 template <bool SCALE, int CPT, int U, class TV>
@@ -300,8 +345,10 @@ spmv_cv_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const SpEn
     float ar[CPT], ai[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; q++) ar[q] = ai[q] = 0.0f;
-    row_sum_cv<SCALE, CPT, U>(cv, chunk_be[2 * c], chunk_be[2 * c + 1], n_vec, kq * CPT, v, scale, ar, ai);
-    st_n<CPT>(partial + c * n_vec + kq * CPT, ar, ai);
+    using VS = VecSplit<CPT, TV, float2>;
+    const int k0 = VS::k0(kq, n_vec), k1 = VS::k1(kq, n_vec);
+    row_sum_cv<SCALE, CPT, U>(cv, chunk_be[2 * c], chunk_be[2 * c + 1], n_vec, k0, k1, v, scale, ar, ai);
+    st_v<CPT>(partial + c * n_vec, k0, k1, ar, ai);
 }
 
 // Partial sums of the chunks of long rows: chunk c covers entries
