@@ -60,6 +60,7 @@ struct PassArgs {
     int64_t mask_cpl;          // mask columns per line block: S / mask_nch
     const int2* in_ival;       // register kernel, mode 0: row intervals per column (see AxisPass)
     const int2* out_ival;
+    int64_t in_rs, out_rs;     // row strides of in / out (S unless the layout interleaves lines)
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
             int r = r0 + i * rstep;
-            if (r < a.in_cnt) v[i] = src[(int64_t)r * a.S];
+            if (r < a.in_cnt) v[i] = src[(int64_t)r * a.in_rs];
         }
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
                     int64_t ii = o * a.in_cnt + r;
                     v[i] = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + w0 + w]);
                 } else {
-                    v[i] = a.in[o * a.in_os + (int64_t)r * a.S + w0 + w];
+                    v[i] = a.in[o * a.in_os + (int64_t)r * a.in_rs + w0 + w];
                 }
             }
         }
@@ -429,7 +430,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
             float2 p = cmul(v, a.kfc[(i % a.kfc_period) * a.S + w0 + w]);
             a.out[i * a.S + w0 + w] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
         } else {
-            a.out[(o0 + ob) * a.out_os + (int64_t)kl * a.S + w0 + w] = v;
+            a.out[(o0 + ob) * a.out_os + (int64_t)kl * a.out_rs + w0 + w] = v;
         }
     }
 }
@@ -527,17 +528,17 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             if (MODE == 0 && a.in_ival) {
                 // rows outside the column's interval are zero and not loaded
                 const int row = j + N2 * n1;
-                if (take && row >= ilo && row < ihi) x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
+                if (take && row >= ilo && row < ihi) x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
             } else if (MODE == 0 && a.in_mask) {
                 // masked-off rows (not written by the producer) are zero and
                 // not loaded
-                if (take && bit(j + N2 * n1)) x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
+                if (take && bit(j + N2 * n1)) x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
             } else if (take) {
                 if (MODE == 1) {
                     const int64_t ii = o * a.in_cnt + r;
                     x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
                 } else {
-                    x = ld_c(in + o * a.in_os + (int64_t)r * a.S + ws);
+                    x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
                 }
             }
             v[n1] = x;
@@ -644,7 +645,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
                 const float2 p = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
                 st_c(out + i * a.S + ws, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
             } else {
-                st_c(out + o * a.out_os + (int64_t)kl * a.S + ws, y);
+                st_c(out + o * a.out_os + (int64_t)kl * a.out_rs + ws, y);
             }
         }
     }
@@ -808,6 +809,8 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     PassArgs a{};
     a.N = N;
     a.S = p.S;
+    a.in_rs = p.in_rs > 0 ? p.in_rs : p.S;
+    a.out_rs = p.out_rs > 0 ? p.out_rs : p.S;
     a.outer = p.outer;
     if (p.S <= 64) {
         // whole inner dimension in a tile, >= 16 lines where the outer
@@ -929,6 +932,8 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     while (n % 5 == 0) { a.radix[ns++] = 5; n /= 5; }
     a.nstage = ns;
     a.S = p.S;
+    a.in_rs = p.in_rs > 0 ? p.in_rs : p.S;
+    a.out_rs = p.out_rs > 0 ? p.out_rs : p.S;
     a.outer = p.outer;
     const int Lmax = TILE / N;
     if (p.S <= Lmax) {

@@ -15,9 +15,9 @@ bool fft_plain_pass_is_reg(int N, int lines);
 
 // One pass along an axis of length N over an [outer][rows][S] array (S
 // contiguous).  Input rows in_off .. in_off+in_cnt-1 of the length-N line
-// are read from in + o*in_os + r*S (r = row - in_off), the others are zero;
-// output rows out_off .. out_off+out_cnt-1 are written to
-// out + o*out_os + (row - out_off)*S.  presign multiplies input row n by
+// are read from in + o*in_os + r*in_rs (r = row - in_off), the others are
+// zero; output rows out_off .. out_off+out_cnt-1 are written to
+// out + o*out_os + (row - out_off)*out_rs (row strides default to S).  presign multiplies input row n by
 // (-1)^n, postsign output row k by (-1)^(k+N/2), scale multiplies the
 // output.  mode 1: the input is img[o*in_cnt + r] * kfc[(o*in_cnt + r)*S + w];
 // mode 2: out is replaced by img_out[o*out_cnt + k] = epi_scale *
@@ -57,6 +57,11 @@ struct AxisPass {
     // out_ival[c]; rows outside are read as 0 without a load / not stored.
     const int2* in_ival = nullptr;
     const int2* out_ival = nullptr;
+    // Row strides (elements between rows n and n+1 of a line) of in / out;
+    // 0 = S.  Mode 0 only: a pass can then read or write lines whose rows
+    // are interleaved with other lines' rows ([row][outer][S] instead of
+    // [outer][row][S]).
+    int64_t in_rs = 0, out_rs = 0;
 };
 
 void fft_pass(const AxisPass& p, hipStream_t s);

@@ -784,6 +784,19 @@ void Plan::ensure_fft_scratch() {
     fft_b.ensure((size_t)mx * sizeof(float2));
 }
 
+// 3-D single-frame passes keep the middle intermediate as [N_y][n_z][N_x
+// nCh] instead of [n_z][N_y][N_x nCh] (MONA_FFT_YZ=0: the plain order), so
+// that the z passes read (forward) or write (adjoint) it with rows N_x nCh
+// values apart instead of N_y N_x nCh: each pass then has only its grid side
+// on the large stride.
+bool yz_interleave() {
+    static const bool on = [] {
+        const char* e = std::getenv("MONA_FFT_YZ");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
 // Forward: x .* KFC zero-filled to N_u and transformed, one pass per axis;
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
@@ -791,10 +804,15 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
                               bool bf, const uint32_t* out_mask, const int2* out_ival, const int2* mid_ival) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
+    const bool yz = imDim == 3 && nb == 1 && yz_interleave();
     int64_t S = nCh;
     for (int a = 0; a < imDim; a++) {
         int64_t outer = nb;
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
+        if (yz && a == 2) {
+            S /= N[1];       // lines (y, x, c): outer y, inner x c
+            outer = N[1];
+        }
         AxisPass p;
         p.tile = fft_tile;
         p.lines = fft_lines;
@@ -825,6 +843,16 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
             p.out_ival = mid_ival;
             p.mask_nch = nCh;
         }
+        if (yz && a == 1) {  // out [N_y][n_z][S]
+            p.out_os = S;
+            p.out_rs = (int64_t)n[2] * S;
+        }
+        if (yz && a == 2) {  // in [N_y][n_z][S], out the grid [N_z][N_y][S]
+            p.in_os = (int64_t)n[2] * S;
+            p.in_rs = S;
+            p.out_os = S;
+            p.out_rs = (int64_t)N[1] * S;
+        }
         if (a == 0) {
             p.mode = 1;
             p.img = x;
@@ -844,10 +872,15 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
                               const int2* mid_ival) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
+    const bool yz = imDim == 3 && nb == 1 && yz_interleave();
     for (int a = imDim - 1; a >= 0; a--) {
         int64_t S = nCh, outer = nb;
         for (int d = 0; d < a; d++) S *= N[d];
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
+        if (yz && a == 2) {
+            S /= N[1];
+            outer = N[1];
+        }
         AxisPass p;
         p.tile = fft_tile;
         p.lines = fft_lines;
@@ -874,6 +907,16 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
         if (a == 1 && imDim == 3 && mid_ival) {
             p.in_ival = mid_ival;
             p.mask_nch = nCh;
+        }
+        if (yz && a == 2) {  // in the grid [N_z][N_y][S], out [N_y][n_z][S]
+            p.in_os = S;
+            p.in_rs = (int64_t)N[1] * S;
+            p.out_os = (int64_t)n[2] * S;
+            p.out_rs = S;
+        }
+        if (yz && a == 1) {  // in [N_y][n_z][S]
+            p.in_os = S;
+            p.in_rs = (int64_t)n[2] * S;
         }
         if (a == 0) {
             p.mode = combine ? 2 : 3;
