@@ -6,7 +6,8 @@ scalars, on one MI355X.
 Prints ONE JSON line (rank 0).  value = CG-SENSE reconstructions per second
 with all inputs resident in HBM.  Extra fields: NUFFT pairs/s, the roofline
 of the dominant gridding kernel (HIP events around every launch in the timed
-region), and the CPU baseline (the reference's own sparse kernel built from
+region) with the FFT op's beside it (roofline.fft_op: one transform, all
+passes, the largest share of a recon), and the CPU baseline (the reference's own sparse kernel built from
 its sources, plus scipy FFT standing in for MATLAB's fft) timed on this
 host on a bounded sample.
 """
@@ -417,17 +418,28 @@ def bench_serving(args, cfg, quiet=False):
     return out
 
 
-def measured_traffic(config):
-    """HBM bytes per launch of the adjoint gridding op (n2u) from the
-    committed rocprofv3 counter summary of this workload: FETCH_SIZE (scaled
-    by the factor a plain copy of known size gives) + WRITE_SIZE."""
-    path = os.path.join(ROOT, "profiles", f"{config}_n2u_traffic.json")
+def measured_traffic(config, op="n2u"):
+    """HBM bytes per launch of an op (n2u: one adjoint gridding; fft: one
+    transform, all passes) from the committed rocprofv3 counter summary of
+    this workload: FETCH_SIZE (scaled by the factor a plain copy of known size
+    gives) + WRITE_SIZE."""
+    path = os.path.join(ROOT, "profiles", f"{config}_{op}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     return {"bytes": d["fetch_bytes"] + d["write_bytes"],
-            "source": f"profiles/{config}_n2u_traffic.json (rocprofv3 FETCH_SIZE x {d['fetch_factor']} + WRITE_SIZE)"}
+            "source": f"profiles/{config}_{op}_traffic.json (rocprofv3 FETCH_SIZE x {d['fetch_factor']} + WRITE_SIZE)"}
+
+
+def fft_bytes(prob):
+    """Algorithmic HBM bytes of one centred 3-D transform with its fused
+    prologue / epilogue: one read and one write of the nCh-coil grid plus the
+    image and coil-map side (16 nCh prod(N_u) + 8 prod(n_u) (1 + nCh))."""
+    nCh = prob["nCh"]
+    G = int(np.prod(prob["N_u"]))
+    nu = int(np.prod(prob["n_u"]))
+    return 16 * nCh * G + 8 * nu * (1 + nCh)
 
 
 def gridding_bytes(prob):
@@ -591,6 +603,12 @@ def main(argv=None):
         dom, bytes_, avg = "u2n (Gu product, jump-CSR gather)", u2n_b, avg_u2n
     achieved = bytes_ / (avg * 1e-3) / 1e9
     ms_per = elapsed / args.steps * 1e3
+    # the FFT op (the largest share of a recon): each timed region is one
+    # transform (all its passes)
+    n_tr = n_fft
+    fft_b = fft_bytes(prob)
+    avg_tr = ms_fft / max(n_tr, 1)
+    fft_ach = fft_b / (avg_tr * 1e-3) / 1e9 if n_tr else None
     out = {
         "metric": "NUFFT pairs/s + CG-SENSE recons/s at 3D N=128 20ch; p50/p99 served latency",
         "value": args.steps / elapsed,
@@ -619,13 +637,20 @@ def main(argv=None):
                      "bytes_per_launch": bytes_, "avg_launch_ms": avg,
                      "u2n_GBps": u2n_b / (avg_u2n * 1e-3) / 1e9 if n_u2n else None,
                      "n2u_GBps": n2u_b / (avg_n2u * 1e-3) / 1e9 if n_n2u else None,
-                     "touched_cells_U": U},
+                     "touched_cells_U": U,
+                     "fft_op": {"achieved": fft_ach, "peak": 8000.0, "unit": "GB/s",
+                                "frac": fft_ach / 8000.0 if fft_ach else None, "traffic": None,
+                                "bytes_per_transform": fft_b, "avg_transform_ms": avg_tr, "transforms": n_tr}},
         "cpu_baseline": None,
     }
     tr = measured_traffic(args.config)
     if tr is not None:
         out["roofline"]["traffic"] = tr["bytes"]
         out["roofline"]["traffic_source"] = tr["source"]
+    tr = measured_traffic(args.config, "fft")
+    if tr is not None:
+        out["roofline"]["fft_op"]["traffic"] = tr["bytes"]
+        out["roofline"]["fft_op"]["traffic_source"] = tr["source"]
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         try:

@@ -394,7 +394,8 @@ int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* ch
 /* Kernel timing: when enabled, every gridding launch (forward Gu product,
  * adjoint Gut product) records HIP events on its stream; the accumulated
  * launch count and total milliseconds per kind are read with
- * mona_timing_read (kind 0 = u2n, 1 = n2u, 2 = fft pass, 3 = elementwise). */
+ * mona_timing_read (kind 0 = u2n, 1 = n2u, 2 = one whole transform, all its FFT
+ * passes, 3 = elementwise). */
 int mona_timing_enable(int on);
 int mona_timing_read(int kind, long long* launches, double* total_ms);
 
