@@ -61,7 +61,17 @@ struct PassArgs {
     const int2* in_ival;       // register kernel, mode 0: row intervals per column (see AxisPass)
     const int2* out_ival;
     int64_t in_rs, out_rs;     // row strides of in / out (S unless the layout interleaves lines)
+    int64_t in_on, in_os2;     // two-level line offsets (line_off); in_on = 0: o * in_os
+    int64_t out_on, out_os2;
 };
+
+// Offset of line o: o * os, or with on > 0 (o / on) * os2 + (o % on) * os
+// (lines of several frames whose per-frame layout interleaves rows).
+__device__ __forceinline__ int64_t line_off(int64_t o, int64_t os, int64_t on, int64_t os2) {
+    if (on <= 0) return o * os;
+    const int64_t hi = o / on;
+    return hi * os2 + (o - hi * on) * os;
+}
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -322,7 +332,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
         const int w = threadIdx.x % Wt;
         const int rstep = NT / Wt;
         const int r0 = threadIdx.x / Wt;
-        const float2* src = a.in + o0 * a.in_os + w0 + w;
+        const float2* src = a.in + line_off(o0, a.in_os, a.in_on, a.in_os2) + w0 + w;
         float2 v[EPT];
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
@@ -354,7 +364,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
                     int64_t ii = o * a.in_cnt + r;
                     v[i] = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + w0 + w]);
                 } else {
-                    v[i] = a.in[o * a.in_os + (int64_t)r * a.in_rs + w0 + w];
+                    v[i] = a.in[line_off(o, a.in_os, a.in_on, a.in_os2) + (int64_t)r * a.in_rs + w0 + w];
                 }
             }
         }
@@ -430,7 +440,7 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
             float2 p = cmul(v, a.kfc[(i % a.kfc_period) * a.S + w0 + w]);
             a.out[i * a.S + w0 + w] = make_float2(p.x * a.epi_scale, p.y * a.epi_scale);
         } else {
-            a.out[(o0 + ob) * a.out_os + (int64_t)kl * a.out_rs + w0 + w] = v;
+            a.out[line_off(o0 + ob, a.out_os, a.out_on, a.out_os2) + (int64_t)kl * a.out_rs + w0 + w] = v;
         }
     }
 }
@@ -466,6 +476,8 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int64_t o = to * a.Bo + ob;
     const int64_t ws = tile_w * W + w;
     const bool live = o < a.outer && ws < a.S;
+    const int64_t ibase = line_off(o, a.in_os, a.in_on, a.in_os2);
+    const int64_t obase = line_off(o, a.out_os, a.out_on, a.out_os2);
     const float sg = a.inverse ? 1.0f : -1.0f;
 
     // masked passes: the bits of this work-item's column, in registers
@@ -528,17 +540,17 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             if (MODE == 0 && a.in_ival) {
                 // rows outside the column's interval are zero and not loaded
                 const int row = j + N2 * n1;
-                if (take && row >= ilo && row < ihi) x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
+                if (take && row >= ilo && row < ihi) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
             } else if (MODE == 0 && a.in_mask) {
                 // masked-off rows (not written by the producer) are zero and
                 // not loaded
-                if (take && bit(j + N2 * n1)) x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
+                if (take && bit(j + N2 * n1)) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
             } else if (take) {
                 if (MODE == 1) {
                     const int64_t ii = o * a.in_cnt + r;
                     x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
                 } else {
-                    x = ld_c(in + o * a.in_os + (int64_t)r * a.in_rs + ws);
+                    x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
                 }
             }
             v[n1] = x;
@@ -645,7 +657,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
                 const float2 p = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
                 st_c(out + i * a.S + ws, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
             } else {
-                st_c(out + o * a.out_os + (int64_t)kl * a.out_rs + ws, y);
+                st_c(out + obase + (int64_t)kl * a.out_rs + ws, y);
             }
         }
     }
@@ -811,6 +823,10 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     a.S = p.S;
     a.in_rs = p.in_rs > 0 ? p.in_rs : p.S;
     a.out_rs = p.out_rs > 0 ? p.out_rs : p.S;
+    a.in_on = p.in_on;
+    a.in_os2 = p.in_os2;
+    a.out_on = p.out_on;
+    a.out_os2 = p.out_os2;
     a.outer = p.outer;
     if (p.S <= 64) {
         // whole inner dimension in a tile, >= 16 lines where the outer
@@ -934,6 +950,10 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     a.S = p.S;
     a.in_rs = p.in_rs > 0 ? p.in_rs : p.S;
     a.out_rs = p.out_rs > 0 ? p.out_rs : p.S;
+    a.in_on = p.in_on;
+    a.in_os2 = p.in_os2;
+    a.out_on = p.out_on;
+    a.out_os2 = p.out_os2;
     a.outer = p.outer;
     const int Lmax = TILE / N;
     if (p.S <= Lmax) {

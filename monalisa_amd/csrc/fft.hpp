@@ -62,6 +62,10 @@ struct AxisPass {
     // are interleaved with other lines' rows ([row][outer][S] instead of
     // [outer][row][S]).
     int64_t in_rs = 0, out_rs = 0;
+    // Two-level line offsets (mode 0): with in_on > 0 line o starts at
+    // (o / in_on) * in_os2 + (o % in_on) * in_os (frames of a batch whose
+    // own layout interleaves rows); likewise out_on / out_os2.
+    int64_t in_on = 0, in_os2 = 0, out_on = 0, out_os2 = 0;
 };
 
 void fft_pass(const AxisPass& p, hipStream_t s);

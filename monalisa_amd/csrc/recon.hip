@@ -475,6 +475,22 @@ const Frame& Plan::frame(int f) const {
 // Tile-order copies of a frame's matrices (host decode of the reference
 // encoding, stable sort of the points by 8^3 cell tile, upload).
 // Row intervals for dense frames (MONA_GRID_INTERVALS=0 turns them off).
+// y-major layouts of the 3-D pruned transforms (MONA_FFT_YZ=0: the plain
+// [z][y][x] order).  Each frame's grid is stored [N_y][N_z][N_x][nCh]
+// (the gridding matrices address cells in that order) and the middle
+// intermediate [N_y][n_z][N_x nCh], so that every z pass reads and writes
+// rows N_x nCh values apart; only the y passes keep one large-stride side,
+// the forward one on its writes and the adjoint one on its reads.
+bool yz_interleave() {
+    static const bool on = [] {
+        const char* e = std::getenv("MONA_FFT_YZ");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
+bool Plan::grid_yz() const { return imDim == 3 && yz_interleave() && fft_pass_fusable(N[0], nCh); }
+
 bool grid_intervals() {
     static const bool on = [] {
         const char* e = std::getenv("MONA_GRID_INTERVALS");
@@ -540,6 +556,16 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
             }
         }
     }
+    // cell index in the grid's storage order (y-major for the 3-D pruned
+    // transforms)
+    const bool yzg = grid_yz();
+    auto stored_cell = [&](int64_t c) {
+        if (!yzg) return (int)c;
+        const int64_t x = c % N[0], y = (c / N[0]) % N[1], z = c / ((int64_t)N[0] * N[1]);
+        return (int)(x + (int64_t)N[0] * (z + (int64_t)N[2] * y));
+    };
+    if (yzg)
+        for (auto& c : scol) c = stored_cell(c);
     auto Gu_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gu_s, Gu->r_size, Gu->l_size, scol, sval, snj, nullptr, s);
     // Gut with relabelled point columns (entries of each cell keep their order)
@@ -642,6 +668,8 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         dnj.resize(q);
         drow.resize(q);
     }
+    if (yzg)
+        for (auto& c : drow) c = stored_cell(c);
     auto Gut_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
     Gut_s->covers_all = !sparse_frame && !use_ival;
@@ -784,19 +812,6 @@ void Plan::ensure_fft_scratch() {
     fft_b.ensure((size_t)mx * sizeof(float2));
 }
 
-// 3-D single-frame passes keep the middle intermediate as [N_y][n_z][N_x
-// nCh] instead of [n_z][N_y][N_x nCh] (MONA_FFT_YZ=0: the plain order), so
-// that the z passes read (forward) or write (adjoint) it with rows N_x nCh
-// values apart instead of N_y N_x nCh: each pass then has only its grid side
-// on the large stride.
-bool yz_interleave() {
-    static const bool on = [] {
-        const char* e = std::getenv("MONA_FFT_YZ");
-        return e ? std::atoi(e) != 0 : true;
-    }();
-    return on;
-}
-
 // Forward: x .* KFC zero-filled to N_u and transformed, one pass per axis;
 // pass a reads the compact [nb][prod(n_d, d>a)][n_a][S_a] array, S_a =
 // nCh*prod(N_d, d<a), and writes [nb][prod(n_d, d>a)][N_a][S_a].
@@ -804,14 +819,14 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
                               bool bf, const uint32_t* out_mask, const int2* out_ival, const int2* mid_ival) {
     const float2* src = nullptr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
-    const bool yz = imDim == 3 && nb == 1 && yz_interleave();
+    const bool yz = grid_yz();
     int64_t S = nCh;
     for (int a = 0; a < imDim; a++) {
         int64_t outer = nb;
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         if (yz && a == 2) {
-            S /= N[1];       // lines (y, x, c): outer y, inner x c
-            outer = N[1];
+            S /= N[1];       // lines (frame, y, x, c): outer frame * N_y + y, inner x c
+            outer = (int64_t)nb * N[1];
         }
         AxisPass p;
         p.tile = fft_tile;
@@ -843,15 +858,17 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
             p.out_ival = mid_ival;
             p.mask_nch = nCh;
         }
-        if (yz && a == 1) {  // out [N_y][n_z][S]
+        if (yz && a == 1) {  // out per frame [N_y][n_z][S]; line o = frame * n_z + z
+            p.out_on = n[2];
+            p.out_os2 = (int64_t)N[1] * n[2] * S;
             p.out_os = S;
             p.out_rs = (int64_t)n[2] * S;
         }
-        if (yz && a == 2) {  // in [N_y][n_z][S], out the grid [N_z][N_y][S]
+        if (yz && a == 2) {  // in per frame [N_y][n_z][S], out the grid [N_y][N_z][S]
             p.in_os = (int64_t)n[2] * S;
             p.in_rs = S;
-            p.out_os = S;
-            p.out_rs = (int64_t)N[1] * S;
+            p.out_os = (int64_t)N[2] * S;
+            p.out_rs = S;
         }
         if (a == 0) {
             p.mode = 1;
@@ -872,14 +889,14 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
                               const int2* mid_ival) {
     const float2* src = gr;
     float2* bufs[2] = {ba.as<float2>(), bb.as<float2>()};
-    const bool yz = imDim == 3 && nb == 1 && yz_interleave();
+    const bool yz = grid_yz();
     for (int a = imDim - 1; a >= 0; a--) {
         int64_t S = nCh, outer = nb;
         for (int d = 0; d < a; d++) S *= N[d];
         for (int d = a + 1; d < imDim; d++) outer *= n[d];
         if (yz && a == 2) {
             S /= N[1];
-            outer = N[1];
+            outer = (int64_t)nb * N[1];
         }
         AxisPass p;
         p.tile = fft_tile;
@@ -908,13 +925,15 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
             p.in_ival = mid_ival;
             p.mask_nch = nCh;
         }
-        if (yz && a == 2) {  // in the grid [N_z][N_y][S], out [N_y][n_z][S]
-            p.in_os = S;
-            p.in_rs = (int64_t)N[1] * S;
+        if (yz && a == 2) {  // in the grid [N_y][N_z][S], out per frame [N_y][n_z][S]
+            p.in_os = (int64_t)N[2] * S;
+            p.in_rs = S;
             p.out_os = (int64_t)n[2] * S;
             p.out_rs = S;
         }
-        if (yz && a == 1) {  // in [N_y][n_z][S]
+        if (yz && a == 1) {  // in per frame [N_y][n_z][S]; line o = frame * n_z + z
+            p.in_on = n[2];
+            p.in_os2 = (int64_t)N[1] * n[2] * S;
             p.in_os = S;
             p.in_rs = (int64_t)n[2] * S;
         }

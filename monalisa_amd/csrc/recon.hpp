@@ -95,6 +95,9 @@ struct Plan {
     // FFT stages of the operators over nb frames stored back to back
     // (images [nb][nu], grids [nb][Nu][nCh]); scratch a/b sized for nb.
     // (bf: grid-side arrays stored as bf16x2)
+    // 3-D pruned transforms with y-major grid and intermediate layouts (the
+    // gridding matrices of set_frame then address cells y-major)
+    bool grid_yz() const;
     void forward_fft_pruned(const float2* x, float2* grid, int nb, DevBuf& a, DevBuf& b, hipStream_t s,
                             bool bf = false, const uint32_t* out_mask = nullptr, const int2* out_ival = nullptr,
                             const int2* mid_ival = nullptr);
