@@ -529,13 +529,15 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         }
     }
     // tile key of each point: the tile of its first (lowest) cell
-    const int tx = (N[0] + 7) / 8, ty = (N[1] + 7) / 8;
+    // (tiles in the grid's storage order: y before z for y-major grids)
+    const bool yzg = grid_yz();
+    const int tx = (N[0] + 7) / 8, ty = (N[1] + 7) / 8, tz = (N[2] + 7) / 8;
     std::vector<int64_t> key(P);
     for (int64_t p = 0; p < P; p++) {
         if (rowptr[p + 1] == rowptr[p]) { key[p] = INT64_MAX; continue; }
         int64_t cell = col[rowptr[p]];
         int64_t x = cell % N[0], y = (cell / N[0]) % N[1], z = cell / ((int64_t)N[0] * N[1]);
-        key[p] = ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
+        key[p] = yzg ? ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3) : ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
     }
     std::vector<int> perm(P), inv(P);
     for (int64_t p = 0; p < P; p++) perm[p] = (int)p;
@@ -558,7 +560,6 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     }
     // cell index in the grid's storage order (y-major for the 3-D pruned
     // transforms)
-    const bool yzg = grid_yz();
     auto stored_cell = [&](int64_t c) {
         if (!yzg) return (int)c;
         const int64_t x = c % N[0], y = (c / N[0]) % N[1], z = c / ((int64_t)N[0] * N[1]);
@@ -647,13 +648,19 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     std::vector<float> dval(Gut->nnz);
     {
         int64_t o = 0, q = 0;
-        for (int z0 = 0; z0 < N[2]; z0 += 8)
-            for (int y0 = 0; y0 < N[1]; y0 += 8)
+        // slowest axis sa, then sb, then x (storage order of the grid)
+        const int sa = yzg ? 1 : 2, sb = yzg ? 2 : 1;
+        int c3[3] = {0, 0, 0};
+        for (int a0 = 0; a0 < N[sa]; a0 += 8)
+            for (int b0 = 0; b0 < N[sb]; b0 += 8)
                 for (int x0 = 0; x0 < N[0]; x0 += 8)
-                    for (int z = z0; z < std::min(z0 + 8, N[2]); z++)
-                        for (int y = y0; y < std::min(y0 + 8, N[1]); y++)
+                    for (int a = a0; a < std::min(a0 + 8, N[sa]); a++)
+                        for (int b = b0; b < std::min(b0 + 8, N[sb]); b++)
                             for (int x = x0; x < std::min(x0 + 8, N[0]); x++) {
-                                const int64_t cell = ((int64_t)z * N[1] + y) * N[0] + x;
+                                c3[0] = x;
+                                c3[sa] = a;
+                                c3[sb] = b;
+                                const int64_t cell = ((int64_t)c3[2] * N[1] + c3[1]) * N[0] + c3[0];
                                 const int64_t r = cell_row[cell];
                                 const int n = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
                                 if (n == 0 && (sparse_frame || (use_ival && !inside(cell)))) continue;
