@@ -537,14 +537,14 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             const int r = j + N2 * n1 - a.in_off;
             float2 x = make_float2(0.f, 0.f);
             const bool take = live && !dead && r >= 0 && r < a.in_cnt;
-            if (MODE == 0 && a.in_ival) {
-                // rows outside the column's interval are zero and not loaded
+            if (MODE == 0 && (a.in_ival || a.in_mask)) {
+                // rows outside the column's interval, or masked off (not
+                // written by the producer), are zero and not loaded
                 const int row = j + N2 * n1;
-                if (take && row >= ilo && row < ihi) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
-            } else if (MODE == 0 && a.in_mask) {
-                // masked-off rows (not written by the producer) are zero and
-                // not loaded
-                if (take && bit(j + N2 * n1)) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
+                bool ok = take;
+                if (a.in_ival) ok = ok && row >= ilo && row < ihi;
+                if (a.in_mask) ok = ok && bit(row);
+                if (ok) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
             } else if (take) {
                 if (MODE == 1) {
                     const int64_t ii = o * a.in_cnt + r;
@@ -872,8 +872,6 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     const bool limited = p.in_mask || p.out_mask || p.in_ival || p.out_ival;
     MONA_CHECK(!limited || (p.mode == 0 && p.mask_nch >= 1 && p.S % p.mask_nch == 0), MONA_ERR_INTERNAL,
                "cell masks / intervals need a plain pass and S divisible by mask_nch");
-    MONA_CHECK(!((p.in_mask || p.out_mask) && (p.in_ival || p.out_ival)), MONA_ERR_INTERNAL,
-               "a pass takes cell masks or intervals, not both");
     const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     const int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");

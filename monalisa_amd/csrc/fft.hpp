@@ -54,7 +54,9 @@ struct AxisPass {
     int mask_nch = 1;
     // Row intervals (register kernel, mode 0), the contiguous form of the
     // masks: column c (as above) holds rows [x, y) of in_ival[c] /
-    // out_ival[c]; rows outside are read as 0 without a load / not stored.
+    // out_ival[c]; rows outside are read as 0 without a load / not stored,
+    // and a column (-1, -1) is skipped whole (dead).  Given with masks, an
+    // element is read / stored only when both allow it.
     const int2* in_ival = nullptr;
     const int2* out_ival = nullptr;
     // Row strides (elements between rows n and n+1 of a line) of in / out;

@@ -606,8 +606,12 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     // [lo, hi) of column c span every cell of c that Gu reads or Gut writes
     const int64_t NLa = N[imDim - 1], colsa = G / NLa;
     const bool use_ival = !sparse_frame && imDim >= 2 && grid_intervals();
+    // sparse 3-D frames keep their cell masks and also carry the intervals:
+    // the columns outside the y span of their x are then skipped by every
+    // pass (dead columns), the rows inside stay masked
+    const bool sparse_ival = sparse_frame && imDim == 3 && grid_intervals();
     std::vector<int2> ival;
-    if (use_ival) {
+    if (use_ival || sparse_ival) {
         ival.assign(colsa, make_int2((int)NLa, 0));
         auto widen = [&](int64_t cell) {
             const int64_t z = cell / colsa, c = cell - z * colsa;
@@ -623,7 +627,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     // 3-D: the span over y of the non-empty columns of each x; columns
     // outside it are dead (-1, -1): no pass loads or stores them
     std::vector<int2> ivaly;
-    if (use_ival && imDim == 3) {
+    if ((use_ival || sparse_ival) && imDim == 3) {
         std::vector<int2> span(N[0], make_int2(N[1], 0));
         for (int y = 0; y < N[1]; y++)
             for (int x = 0; x < N[0]; x++)
@@ -703,7 +707,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         MONA_HIP(hipMemcpyAsync(dam->p, amask.data(), (size_t)nw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     }
     std::shared_ptr<DevBuf> div, divy;
-    if (use_ival) {
+    if (use_ival || sparse_ival) {
         div = std::make_shared<DevBuf>((size_t)colsa * sizeof(int2));
         MONA_HIP(hipMemcpyAsync(div->p, ival.data(), (size_t)colsa * sizeof(int2), hipMemcpyHostToDevice, s));
     }
@@ -768,14 +772,17 @@ const uint32_t* Plan::adj_mask(int f) const {
     if (f < 0) return masked_grid(-1) && mask_ab.p ? mask_ab.as<uint32_t>() : nullptr;
     return masked_grid(f) ? frame(f).amask->as<uint32_t>() : nullptr;
 }
+// Intervals of the passes: dense frames (ival_grid), and sparse 3-D frames
+// that carry them beside their masks (dead columns only).
 const int2* Plan::ivals_of(int f) const {
-    if (f < 0) return ival_grid(-1) && ival_b.p ? ival_b.as<int2>() : nullptr;
-    return ival_grid(f) ? frame(f).ival->as<int2>() : nullptr;
+    if (f < 0) return (ival_grid(-1) || masked_grid(-1)) && ival_b.p ? ival_b.as<int2>() : nullptr;
+    return (ival_grid(f) || masked_grid(f)) && frame(f).ival ? frame(f).ival->as<int2>() : nullptr;
 }
 // (with ivals_of: the two go together, dead columns rely on both)
 const int2* Plan::mid_ivals_of(int f) const {
-    if (f < 0) return ival_grid(-1) && ival_b.p && ivaly_b.p ? ivaly_b.as<int2>() : nullptr;
-    return ival_grid(f) && frame(f).ivaly ? frame(f).ivaly->as<int2>() : nullptr;
+    if (f < 0) return (ival_grid(-1) || masked_grid(-1)) && ival_b.p && ivaly_b.p ? ivaly_b.as<int2>() : nullptr;
+    return (ival_grid(f) || masked_grid(f)) && frame(f).ival && frame(f).ivaly ? frame(f).ivaly->as<int2>()
+                                                                               : nullptr;
 }
 
 void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
@@ -1055,7 +1062,7 @@ void Plan::build_batch(hipStream_t s) {
     ivaly_b.release();
     {
         bool all = true;
-        for (int f = 0; f < nFr; f++) all = all && frame(f).ivals;
+        for (int f = 0; f < nFr; f++) all = all && frame(f).ival && (frame(f).ivaly != nullptr) == (frame(0).ivaly != nullptr);
         if (all) {
             const size_t ib = frame(0).ival->bytes;
             ival_b.alloc(ib * nFr);
