@@ -556,6 +556,18 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             v[n1] = x;
         }
     }
+    // MODE 2: the KFC_conj values of this work-item's kept output rows are
+    // loaded now, in flight with the input, instead of after the transform
+    float2 kf[MODE == 2 ? N2 : 1];
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+            const int kl = j + N1 * k2 - a.out_off;
+            kf[k2] = make_float2(0.f, 0.f);
+            if (live && j < N1 && kl >= 0 && kl < a.out_cnt)
+                kf[k2] = a.kfc[((o * a.out_cnt + kl) % a.kfc_period) * a.S + ws];
+        }
+    }
     for (int i = t; i < N; i += blockDim.x) tw[i] = twg[i];
     __syncthreads();
     if (j < N2) {
@@ -618,8 +630,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
                     float sc = a.scale;
                     if (a.postsign && ((k + half) & 1)) sc = -sc;
                     const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
-                    const int64_t i = o * a.out_cnt + kl;
-                    ex[(ob * a.out_cnt + kl) * Wp + w] = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
+                    ex[(ob * a.out_cnt + kl) * Wp + w] = cmul(y, kf[k2]);
                 }
             }
         }
