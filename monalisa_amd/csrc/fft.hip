@@ -890,11 +890,11 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     MONA_CHECK(nt <= 1024 && (p.mode != 2 || a.W == p.S), MONA_ERR_INTERNAL, "bad register FFT tile");
     const size_t coil_sum = p.mode == 2 ? (size_t)a.Bo * p.out_cnt * (a.W | 1) : 0;
     size_t ex = std::max<size_t>((size_t)16 * xs, coil_sum);
-    // passes whose exchange needs more than 40 KiB (fewer than four tiles
+    // passes whose exchange needs more than 20 KiB (fewer than eight tiles
     // on a CU) exchange in two halves (MONA_FFT_SPLIT_KB sets the bound)
     static const size_t split_bytes = [] {
         const char* e = std::getenv("MONA_FFT_SPLIT_KB");
-        return (size_t)(e ? std::atoi(e) : 40) * 1024;
+        return (size_t)(e ? std::atoi(e) : 20) * 1024;
     }();
     const bool split = split_exchange() && ((size_t)N + ex) * sizeof(float2) > split_bytes;
     if (split) ex = std::max<size_t>((size_t)8 * xs, coil_sum);
