@@ -395,7 +395,7 @@ int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* ch
  * adjoint Gut product) records HIP events on its stream; the accumulated
  * launch count and total milliseconds per kind are read with
  * mona_timing_read (kind 0 = u2n, 1 = n2u, 2 = one whole transform, all its FFT
- * passes, 3 = elementwise). */
+ * passes; kind 3 is reserved and reads zero). */
 int mona_timing_enable(int on);
 int mona_timing_read(int kind, long long* launches, double* total_ms);
 
