@@ -521,10 +521,12 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     std::vector<int64_t> rowptr(P + 1, 0);
     for (int64_t p = 0; p < P; p++) rowptr[p + 1] = rowptr[p] + Gu->h_r_nJump[p];
     std::vector<int> col(Gu->nnz);
+    MONA_CHECK(Gu->r_size == Nu, MONA_ERR_ARG, "Gu must have one column per grid cell (r_size == prod(N_u))");
     {
         int64_t c = 0;
         for (int64_t k = 0; k < Gu->nnz; k++) {
             c += Gu->h_r_jump[k];
+            MONA_CHECK(c >= 0 && c < Nu, MONA_ERR_ARG, "Gu column outside the grid");
             col[k] = (int)c;
         }
     }

@@ -161,3 +161,19 @@ def test_loaded_matrices_drive_the_operators(tmp_path):
         outs.append((y.cpu().numpy(), P.bmNakatsha(y).cpu().numpy()))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_set_frame_rejects_a_matrix_of_another_grid():
+    """Gu must address the plan's grid: a matrix built for another N_u is
+    refused with MONA_ERR_ARG before any device work."""
+    import torch
+    from monalisa_amd._lib import MonaError
+    from monalisa_amd.nufft import Plan
+    from monalisa_amd.sparse import bmTraj2SparseMat
+    p = make_problem("3d32", nCh=2)
+    G = bmTraj2SparseMat(p["t"], p["ve"], [24, 24, 24], p["dK_u"], p["kernel"])
+    P = Plan(p["N_u"], p["n_u"], p["dK_u"], 2, p["kernel"])
+    P.set_coils(torch.from_numpy(p["C"]).cuda())
+    with pytest.raises(MonaError) as e:
+        P.set_frame(0, G["Gu"], G["Gut"])
+    assert e.value.status == 1
