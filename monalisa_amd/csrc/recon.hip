@@ -186,6 +186,59 @@ __global__ void k_xpby_dev(float2* p, const float2* d, const double* scal, int64
         p[i] = make_float2(dv.x + b * pv.x, dv.y + b * pv.y);
     }
 }
+// The same updates on two complex values per work-item (16-byte accesses,
+// every pointer 16-byte aligned); the last value of an odd n is done alone
+// by the last work-item.  Each value gets the arithmetic of the one-value
+// kernels above.
+__global__ void k_sub2(const float2* a, const float2* b, float2* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (2 * i + 1 < n) {
+        const float4 x = reinterpret_cast<const float4*>(a)[i], y = reinterpret_cast<const float4*>(b)[i];
+        reinterpret_cast<float4*>(out)[i] = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+    } else if (2 * i < n) {
+        const float2 x = a[2 * i], y = b[2 * i];
+        out[2 * i] = make_float2(x.x - y.x, x.y - y.y);
+    }
+}
+__global__ void k_copy2(const float2* a, float2* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (2 * i + 1 < n) reinterpret_cast<float4*>(out)[i] = reinterpret_cast<const float4*>(a)[i];
+    else if (2 * i < n) out[2 * i] = a[2 * i];
+}
+__global__ void k_axpy_dev2(float2* y, const float2* x, const double* scal, float sign, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float a = sign * (float)(*scal);
+    if (2 * i + 1 < n) {
+        const float4 xv = reinterpret_cast<const float4*>(x)[i];
+        float4 yv = reinterpret_cast<float4*>(y)[i];
+        yv.x = yv.x + a * xv.x;
+        yv.y = yv.y + a * xv.y;
+        yv.z = yv.z + a * xv.z;
+        yv.w = yv.w + a * xv.w;
+        reinterpret_cast<float4*>(y)[i] = yv;
+    } else if (2 * i < n) {
+        const float2 xv = x[2 * i];
+        float2 yv = y[2 * i];
+        yv.x = yv.x + a * xv.x;
+        yv.y = yv.y + a * xv.y;
+        y[2 * i] = yv;
+    }
+}
+__global__ void k_xpby_dev2(float2* p, const float2* d, const double* scal, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float b = (float)(*scal);
+    if (2 * i + 1 < n) {
+        const float4 pv = reinterpret_cast<const float4*>(p)[i], dv = reinterpret_cast<const float4*>(d)[i];
+        reinterpret_cast<float4*>(p)[i] = make_float4(dv.x + b * pv.x, dv.y + b * pv.y, dv.z + b * pv.z,
+                                                      dv.w + b * pv.w);
+    } else if (2 * i < n) {
+        const float2 pv = p[2 * i], dv = d[2 * i];
+        p[2 * i] = make_float2(dv.x + b * pv.x, dv.y + b * pv.y);
+    }
+}
+bool al16(const void* a, const void* b = nullptr, const void* c = nullptr) {
+    return ((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) % 16 == 0;
+}
 // dst[s][c] = src[perm[s]][c]  (caller order -> tile order)
 __global__ void k_gather_pts(const float2* __restrict__ src, const int* __restrict__ perm, float2* __restrict__ dst,
                              int64_t P, int nCh) {
@@ -1131,19 +1184,23 @@ void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scal
 
 // ---------------------------------------------------------------- vector helpers
 void vsub(const float2* a, const float2* b, float2* out, int64_t n, hipStream_t s) {
-    k_sub<<<nb256(n), 256, 0, s>>>(a, b, out, n);
+    if (al16(a, b, out)) k_sub2<<<nb256((n + 1) / 2), 256, 0, s>>>(a, b, out, n);
+    else k_sub<<<nb256(n), 256, 0, s>>>(a, b, out, n);
     MONA_HIP(hipGetLastError());
 }
 void vcopy(const float2* a, float2* out, int64_t n, hipStream_t s) {
-    k_copy<<<nb256(n), 256, 0, s>>>(a, out, n);
+    if (al16(a, out)) k_copy2<<<nb256((n + 1) / 2), 256, 0, s>>>(a, out, n);
+    else k_copy<<<nb256(n), 256, 0, s>>>(a, out, n);
     MONA_HIP(hipGetLastError());
 }
 void vaxpy(float2* y, const float2* x, const double* a, float sign, int64_t n, hipStream_t s) {
-    k_axpy_dev<<<nb256(n), 256, 0, s>>>(y, x, a, sign, n);
+    if (al16(y, x)) k_axpy_dev2<<<nb256((n + 1) / 2), 256, 0, s>>>(y, x, a, sign, n);
+    else k_axpy_dev<<<nb256(n), 256, 0, s>>>(y, x, a, sign, n);
     MONA_HIP(hipGetLastError());
 }
 void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s) {
-    k_xpby_dev<<<nb256(n), 256, 0, s>>>(p, d, b, n);
+    if (al16(p, d)) k_xpby_dev2<<<nb256((n + 1) / 2), 256, 0, s>>>(p, d, b, n);
+    else k_xpby_dev<<<nb256(n), 256, 0, s>>>(p, d, b, n);
     MONA_HIP(hipGetLastError());
 }
 void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
