@@ -72,6 +72,23 @@ def test_all_layouts_bit_exact_vs_port(seed):
     assert np.array_equal(rrow.T, er)
 
 
+@pytest.mark.parametrize("n", [1, 4, 20])
+def test_long_rows_bit_exact_vs_port(n):
+    """Rows far longer than 64 entries: the drop-in products still sum each
+    row sequentially in stored order (the reference's kernel does), so they
+    equal the C restatement bit for bit."""
+    print(f"[gpu] long rows n_vec={n}", flush=True)
+    rng = np.random.default_rng(900 + n)
+    M = OL.random_matrix(rng, 60, 3000, 900, squeeze=False)
+    assert max(M["r_nJump"]) > 64
+    vr = rng.normal(size=(n, M["r_size"])).astype(np.float32)
+    vi = rng.normal(size=(n, M["r_size"])).astype(np.float32)
+    er, ei = OL.call_cC(OL.port().port_cC_oBlock_omp, M, vr, vi)
+    gr, gi = gpu_cC(lib().bmSparseMat_cC_oBlock_omp, M, vr, vi)
+    assert np.array_equal(gr.view(np.int32), er.view(np.int32))
+    assert np.array_equal(gi.view(np.int32), ei.view(np.int32))
+
+
 def test_multiblock_row_layout():
     """cR/rR_nBlock_omp: same sums with the column cursor restarting at each
     block (first entry of each block absolute), as cpp_prepare('multi_block')
