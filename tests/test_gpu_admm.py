@@ -82,12 +82,15 @@ def _teva_run(q, nIter=3, delta=0.05, rho=0.5, nCGD=3):
     return x.cpu().numpy(), z.cpu().numpy(), u.cpu().numpy(), hist
 
 
-def test_teva_chain_vs_oracle():
+@pytest.mark.parametrize("sched", ["constant", "ramped"])
+def test_teva_chain_vs_oracle(sched):
+    """Constant delta/rho, and [start, end] ranges spread linearly over the
+    iterations (private_init_regul_param, bmTevaMorphosia_chain.m:226-235)."""
     import torch
     from monalisa_amd import synth
     from monalisa_amd.nufft import Plan
     from monalisa_amd.sparse import bmTraj2SparseMat
-    print("[gpu] teva chain", flush=True)
+    print(f"[gpu] teva chain {sched}", flush=True)
     rng = np.random.default_rng(3)
     n_u, N_u, dK_u, nCh, nFr = [16, 16], [32, 32], [0.5, 0.5], 2, 3
     C = synth.coil_maps(n_u, nCh, rng)
@@ -109,6 +112,8 @@ def test_teva_chain_vs_oracle():
         ys.append((y + 0.01 * (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape))).astype(np.complex64))
     ve_max = float(max(v.max() for v in ves))
     delta, rho, nCGD, nIter = 0.05, 0.5, 3, 3
+    if sched == "ramped":
+        delta, rho, nIter = (0.02, 0.1), (0.3, 0.8), 4
     x = torch.zeros((nFr, 256), dtype=torch.complex64, device="cuda")
     z, u, hist = P.bmTevaMorphosia_chain(x, None, None, [torch.from_numpy(v).cuda() for v in ys],
                                          [torch.from_numpy(v).cuda() for v in ves], delta, rho, nCGD, ve_max,
