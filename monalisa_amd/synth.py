@@ -152,3 +152,85 @@ def dirichlet_bins(nSpoke, nBin, alpha, rng):
         out.append(np.sort(perm[o:o + c]))
         o += c
     return out
+
+
+def _keys_weight(s):
+    """Cubic convolution kernel of Keys (1981) with a = -1/2."""
+    s = np.abs(s)
+    return np.where(s <= 1, (1.5 * s - 2.5) * s * s + 1,
+                    np.where(s < 2, ((-0.5 * s + 2.5) * s - 4) * s + 2, 0.0))
+
+
+def cubic_interp(values, origin, step, points):
+    """Cubic-convolution interpolation on a uniform grid (the method MATLAB's
+    interpn 'cubic' uses on uniformly spaced grids; Keys 1981, a = -1/2,
+    with Keys' boundary rows f(-1) = 3 f(0) - 3 f(1) + f(2) at each end).
+    values: d-D array; origin, step: per-axis first coordinate and spacing;
+    points: [d, P].  Points outside the grid's coordinate range give NaN,
+    as interpn does.  Reproduces polynomials of degree <= 2 exactly."""
+    v = np.asarray(values)
+    d = v.ndim
+    pts = np.asarray(points, dtype=np.float64).reshape(d, -1)
+    for a in range(d):  # boundary rows on both ends of every axis
+        n = v.shape[a]
+        if n < 3:
+            raise ValueError("cubic interpolation needs at least 3 points per axis")
+        take = lambda i: np.take(v, [i], axis=a)
+        lo = 3 * take(0) - 3 * take(1) + take(2)
+        hi = 3 * take(n - 1) - 3 * take(n - 2) + take(n - 3)
+        v = np.concatenate([lo, v, hi], axis=a)
+    P = pts.shape[1]
+    out = np.zeros(P, dtype=np.result_type(v.dtype, np.float64))
+    inside = np.ones(P, bool)
+    base, frac = [], []
+    for a in range(d):
+        n = np.asarray(values).shape[a]
+        u = (pts[a] - origin[a]) / step[a]
+        inside &= (u >= 0) & (u <= n - 1)
+        i = np.clip(np.floor(u), 0, n - 2).astype(np.int64)
+        base.append(i)
+        frac.append(u - i)
+    for off in np.ndindex(*([4] * d)):
+        w = np.ones(P)
+        idx = []
+        for a in range(d):
+            o = off[a] - 1                       # neighbour offsets -1 .. 2
+            w = w * _keys_weight(frac[a] - o)
+            idx.append(np.clip(base[a] + o + 1, 0, v.shape[a] - 1))   # +1: the boundary row
+        out = out + w * v[tuple(idx)]
+    out[~inside] = np.nan
+    return out
+
+
+def bmSimulateMriData(h, C, t, N_u, n_u, dK_u):
+    """Simulated k-space data (src/mriRecon/function/bmSimulateMriData.m:8-86):
+    for every coil, C .* h zero-filled to N_u, the centred DFT along every
+    axis divided by N_u and dK_u, then interpolated at the trajectory points
+    with cubic convolution on the grid k = (-N/2 .. N/2-1) dK; points outside
+    the grid give 0.  h: image [prod(n_u)] (x fastest) or n_u-shaped; C:
+    [prod(n_u), nCh] or None (one coil of ones); t: [imDim, nPt].  Returns
+    [nPt, nCh] complex64."""
+    n_u = [int(a) for a in np.ravel(n_u)]
+    N_u = [int(a) for a in np.ravel(N_u)]
+    dK_u = [float(a) for a in np.ravel(dK_u)]
+    d = len(n_u)
+    h = np.asarray(h).reshape(n_u, order="F").astype(np.complex128)
+    C = np.ones((int(np.prod(n_u)), 1)) if C is None else np.asarray(C)
+    C = C.reshape(int(np.prod(n_u)), -1)
+    t = np.asarray(t, dtype=np.float64).reshape(d, -1)
+    y = np.zeros((t.shape[1], C.shape[1]), np.complex64)
+    origin = [-N / 2 * dk for N, dk in zip(N_u, dK_u)]
+    for c in range(C.shape[1]):
+        im = C[:, c].reshape(n_u, order="F") * h
+        if N_u != n_u:  # centred zero fill (bmImZeroFill.m: rows N/2+1-n/2 .. N/2+n/2)
+            z = np.zeros(N_u, np.complex128)
+            z[tuple(slice(N // 2 - n // 2, N // 2 - n // 2 + n) for N, n in zip(N_u, n_u))] = im
+            im = z
+        for a in range(d):
+            im = np.fft.fftshift(np.fft.fft(np.fft.ifftshift(im, axes=a), axis=a), axes=a) / N_u[a] / dK_u[a]
+        re = cubic_interp(im.real, origin, dK_u, t)
+        imag = cubic_interp(im.imag, origin, dK_u, t)
+        v = re + 1j * imag
+        v[np.isnan(v)] = 0
+        y[:, c] = v.astype(np.complex64)
+    return y

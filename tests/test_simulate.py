@@ -1,0 +1,75 @@
+"""synth.bmSimulateMriData (bmSimulateMriData.m:8-86) and its cubic
+convolution, pinned by what the mathematics fixes: cubic convolution
+reproduces polynomials of degree <= 2 exactly (boundary rows included); at
+trajectory points on grid nodes the data are the centred DFT of the
+zero-filled coil image divided by N dK, computed here as an explicit sum;
+points outside the grid give 0; coils are independent."""
+import numpy as np
+import pytest
+
+from monalisa_amd.synth import bmSimulateMriData, cubic_interp
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_cubic_convolution_reproduces_quadratics(d):
+    rng = np.random.default_rng(d)
+    shape = [7, 9, 6][:d]
+    origin = rng.uniform(-2, 2, d)
+    step = rng.uniform(0.2, 1.5, d)
+    axes = [origin[a] + step[a] * np.arange(shape[a]) for a in range(d)]
+    grids = np.meshgrid(*axes, indexing="ij")
+    c0 = rng.normal()
+    c1 = rng.normal(size=d)
+    c2 = rng.normal(size=(d, d))
+    f = lambda xs: c0 + sum(c1[a] * xs[a] for a in range(d)) + sum(
+        c2[a, b] * xs[a] * xs[b] for a in range(d) for b in range(d))
+    pts = np.stack([rng.uniform(axes[a][0], axes[a][-1], 200) for a in range(d)])
+    got = cubic_interp(f(grids), origin, step, pts)
+    np.testing.assert_allclose(got, f(pts), rtol=0, atol=1e-10)
+    # a cubic is not reproduced (the kernel is third order, not exact on cubics)
+    g = lambda xs: xs[0] ** 3
+    assert np.abs(cubic_interp(g(grids), origin, step, pts) - g(pts)).max() > 1e-6
+
+
+def _centred_dft(im, N, dK):
+    """fftshift(fft(ifftshift(.))) along every axis as an explicit sum over
+    centred indices, divided by N dK."""
+    out = im.astype(np.complex128)
+    for a, (n, dk) in enumerate(zip(N, dK)):
+        j = np.arange(n) - n // 2
+        F = np.exp(-2j * np.pi * np.outer(j, j) / n) / n / dk
+        out = np.moveaxis(np.tensordot(F, np.moveaxis(out, a, 0), axes=(1, 0)), 0, a)
+    return out
+
+
+@pytest.mark.parametrize("n_u,N_u,dK_u", [((6,), (12,), (0.5,)), ((4, 6), (8, 12), (0.5, 0.25)),
+                                          ((4, 4, 6), (8, 8, 12), (1.0, 0.5, 0.5))])
+def test_on_grid_points_are_the_scaled_centred_dft(n_u, N_u, dK_u):
+    rng = np.random.default_rng(5)
+    d = len(n_u)
+    nCh = 2
+    h = rng.normal(size=n_u) + 1j * rng.normal(size=n_u)
+    C = rng.normal(size=(int(np.prod(n_u)), nCh)) + 1j * rng.normal(size=(int(np.prod(n_u)), nCh))
+    # interior grid nodes (the cubic weights at a node are 1 there, 0 elsewhere)
+    idx = [rng.integers(1, N - 2, 15) for N in N_u]
+    t = np.stack([(i - N // 2) * dk for i, N, dk in zip(idx, N_u, dK_u)])
+    y = bmSimulateMriData(h.reshape(-1, order="F"), C, t, N_u, n_u, dK_u)
+    for c in range(nCh):
+        im = np.zeros(N_u, complex)
+        im[tuple(slice(N // 2 - n // 2, N // 2 - n // 2 + n) for N, n in zip(N_u, n_u))] = \
+            C[:, c].reshape(n_u, order="F") * h
+        ref = _centred_dft(im, N_u, dK_u)[tuple(idx)]
+        np.testing.assert_allclose(y[:, c], ref, rtol=2e-6, atol=2e-6 * np.abs(ref).max())
+
+
+def test_points_outside_the_grid_give_zero_and_coils_are_independent():
+    rng = np.random.default_rng(8)
+    n_u, N_u, dK_u = (4, 4), (8, 8), (0.5, 0.5)
+    h = rng.normal(size=16) + 1j * rng.normal(size=16)
+    t = np.array([[0.1, 5.0, -2.2, 1.0], [0.3, 0.0, 0.0, -1.99]])   # 5.0 and -2.2 are off the grid
+    y1 = bmSimulateMriData(h, None, t, N_u, n_u, dK_u)
+    assert y1.shape == (4, 1) and y1[1, 0] == 0 and y1[2, 0] == 0 and abs(y1[0, 0]) > 0
+    C = np.stack([np.ones(16), 2j * np.ones(16)], axis=1)
+    y2 = bmSimulateMriData(h, C, t, N_u, n_u, dK_u)
+    np.testing.assert_allclose(y2[:, 0], y1[:, 0], rtol=1e-6)
+    np.testing.assert_allclose(y2[:, 1], 2j * y1[:, 0], rtol=1e-6)
