@@ -61,6 +61,21 @@ def radial2d_golden(nSpoke, nSample, k_max, dK_n=1.0):
     return np.stack([x, y], 0).reshape(2, -1, order="F")
 
 
+def bmTraj_fullRadial2_lineAssym2(N, nSeg, dK_n):
+    """2-D full-radial trajectory with nSeg lines at equally spread angles
+    phi = 2 pi s / nSeg (src/traj2/bmTraj_fullRadial2_lineAssym2.m:7-35):
+    N points per line at r = (n - fix(N/2)) kMax / fix(N/2), n = 0 .. N-1,
+    kMax = N dK_n / 2.  Returns [2, N * nSeg], the points of a line
+    consecutive."""
+    N, nSeg = int(N), int(nSeg)
+    kMax = N * dK_n / 2
+    phi = np.linspace(0, 2 * math.pi, nSeg + 1)[:-1]
+    r = (np.arange(N) - N // 2) * kMax / (N // 2)
+    x = r[:, None] * np.cos(phi)[None, :]
+    y = r[:, None] * np.sin(phi)[None, :]
+    return np.stack([x, y], 0).reshape(2, -1, order="F")
+
+
 def ve_radial3(t, nSample, nSpoke):
     """Volume elements of a 3-D full-radial trajectory
     (src/geom3/bmVolumeElement_full_radial3.m:42-49): dr * 4 pi |k|^2 /

@@ -73,3 +73,24 @@ def test_points_outside_the_grid_give_zero_and_coils_are_independent():
     y2 = bmSimulateMriData(h, C, t, N_u, n_u, dK_u)
     np.testing.assert_allclose(y2[:, 0], y1[:, 0], rtol=1e-6)
     np.testing.assert_allclose(y2[:, 1], 2j * y1[:, 0], rtol=1e-6)
+
+
+def test_uniform_angle_radial_trajectory():
+    """bmTraj_fullRadial2_lineAssym2: nSeg lines through the origin at angles
+    2 pi s / nSeg, N points each at an arithmetic sequence of radii whose
+    sample fix(N/2) is the origin and whose last is kMax (N-1-fix(N/2)) /
+    fix(N/2)."""
+    from monalisa_amd.synth import bmTraj_fullRadial2_lineAssym2
+    N, nSeg, dK = 16, 6, 0.5
+    t = bmTraj_fullRadial2_lineAssym2(N, nSeg, dK)
+    assert t.shape == (2, N * nSeg)
+    kMax = N * dK / 2
+    for s in range(nSeg):
+        line = t[:, s * N:(s + 1) * N]
+        ang = 2 * np.pi * s / nSeg
+        # every point on the line at angle ang: cross product with the direction is 0
+        np.testing.assert_allclose(line[0] * np.sin(ang) - line[1] * np.cos(ang), 0, atol=1e-12)
+        r = line[0] * np.cos(ang) + line[1] * np.sin(ang)
+        np.testing.assert_allclose(np.diff(r), kMax / (N // 2), rtol=1e-12)
+        assert abs(r[N // 2]) < 1e-12
+        np.testing.assert_allclose(r[-1], kMax * (N - 1 - N // 2) / (N // 2), rtol=1e-12)
