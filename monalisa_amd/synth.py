@@ -100,6 +100,58 @@ def ve_radial2(t, nSample, nSpoke):
     return v.reshape(-1, order="F")
 
 
+def _volume_element_1d(x):
+    """Widths of the 1-D Voronoi cells of the values x (midpoints between
+    sorted neighbours; the two end cells mirror their inner half), in the
+    order of x (src/geom1/bmVolumeElement1.m)."""
+    x = np.asarray(x, dtype=np.float64)
+    order = np.argsort(x, kind="stable")
+    xs = x[order]
+    mid = (xs[1:] + xs[:-1]) / 2
+    edges = np.concatenate([[xs[0] - (mid[0] - xs[0])], mid, [xs[-1] + (xs[-1] - mid[-1])]])
+    w = np.empty_like(x)
+    w[order] = np.diff(edges)
+    return w
+
+
+def bmVolumeElement_voronoi_full_radial2(t, N):
+    """Volume elements of a 2-D full-radial trajectory of lines of N points
+    (N even, the zero at index N/2 of each line)
+    (src/geom2/bmVolumeElement_voronoi_full_radial2.m:11-77): per point the
+    radial Voronoi width along its line times |k| times the angular Voronoi
+    width of the line's direction among the 2 nLine directions +-e; the
+    centre point of every line gets pi (mean dr_0 / 2)^2 / (2 nLine).
+    t: [2, N * nLine], points of a line consecutive.  (The reference infers
+    N from t; here it is given.)"""
+    t = np.asarray(t, dtype=np.float64)
+    N = int(N)
+    if t.shape[0] != 2 or N % 2 or t.shape[1] % N:
+        raise ValueError("2-D trajectory of whole lines of an even number of points required")
+    nLine = t.shape[1] // N
+    tl = t.reshape(2, N, nLine, order="F")
+    # line directions (bmTraj_lineDirection.m): mean unit vector from the
+    # first point to the points of the second half
+    d = tl[:, (N + 1) // 2 - 1:, :] - tl[:, :1, :]
+    d = d / np.sqrt((d ** 2).sum(0, keepdims=True))
+    e = d.mean(1)                                               # [2, nLine]
+    dr = np.stack([_volume_element_1d(tl[:, :, i].T @ e[:, i]) for i in range(nLine)], 1)   # [N, nLine]
+    ee = np.concatenate([e, -e], 1)
+    ang = np.arctan2(ee[1], ee[0])
+    order = np.argsort(ang, kind="stable")
+    a = ang[order]
+    cut = (math.pi - a[-1]) + (a[0] + math.pi)
+    w = _volume_element_1d(a)
+    w[0] = (a[1] - a[0]) / 2 + cut / 2
+    w[-1] = (a[-1] - a[-2]) / 2 + cut / 2
+    wa = np.empty_like(w)
+    wa[order] = w
+    ds = np.sqrt((tl ** 2).sum(0)) * wa[None, :nLine]
+    v = (dr * ds).reshape(-1, order="F")
+    dr0 = dr[N // 2, :].mean() / 2
+    v[N // 2::N] = math.pi * dr0 ** 2 / (2 * nLine)
+    return v
+
+
 def coil_maps(n_u, nCh, rng):
     """Smooth Gaussian-magnitude blobs (centres on a ring/sphere around the
     FoV) times a low-order polynomial phase, normalised so that

@@ -94,3 +94,26 @@ def test_uniform_angle_radial_trajectory():
         np.testing.assert_allclose(np.diff(r), kMax / (N // 2), rtol=1e-12)
         assert abs(r[N // 2]) < 1e-12
         np.testing.assert_allclose(r[-1], kMax * (N - 1 - N // 2) / (N // 2), rtol=1e-12)
+
+
+@pytest.mark.parametrize("nLine", [7, 8, 31])
+def test_voronoi_volume_elements_of_uniform_lines(nLine):
+    """For nLine lines at equally spread angles over [0, 2 pi) the angular
+    Voronoi cells of the 2 nLine directions +-e share 2 pi; every line's
+    point then weighs dr |k| times its cell, and the elements of all points
+    tile the disc of radius kMax (area pi kMax^2, up to the end cells)."""
+    from monalisa_amd.synth import bmTraj_fullRadial2_lineAssym2, bmVolumeElement_voronoi_full_radial2
+    N, dK = 32, 1.0
+    t = bmTraj_fullRadial2_lineAssym2(N, nLine, dK)
+    v = bmVolumeElement_voronoi_full_radial2(t, N)
+    assert v.shape == (N * nLine,) and np.all(v > 0)
+    kMax = N * dK / 2
+    step = kMax / (N // 2)
+    k = np.sqrt((t ** 2).sum(0))
+    area = v.sum()
+    assert abs(area - np.pi * kMax ** 2) / (np.pi * kMax ** 2) < 1e-3
+    if nLine % 2:   # no direction repeats: each line's angular cell is pi / nLine
+        inner = np.ones(N * nLine, bool)
+        inner[N // 2::N] = False
+        np.testing.assert_allclose(v[inner], step * k[inner] * np.pi / nLine, rtol=1e-9)
+        np.testing.assert_allclose(v[N // 2::N], np.pi * (step / 2) ** 2 / (2 * nLine), rtol=1e-9)
