@@ -667,7 +667,7 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
         float2* part = S.partial.as<float2>();
         // Every chunk holds 64 entries, so a deeper unroll wastes no tail: at
         // CPT = 4, 8 entries per batch keep twice the gathers in flight (n2u
-        // long-row chunks of the 3D N=128 20-coil CG-SENSE: 363 -> ~330 us).
+        // long-row chunks of the 3D N=128 20-coil CG-SENSE: 358 -> 329 us).
         constexpr int UC = CPT == 4 ? 8 : U;
         if (yscale)
             spmv_cv_chunks<true, CPT, UC><<<blocks_for(tc, 256), 256, 0, s>>>(cb, V.n_chunk, cv, n_vec, v, yscale, part,
