@@ -5,22 +5,27 @@ tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py
 may import this module, and only as the checker (or, for the baseline, as the
 timed CPU stand-in); the product library never calls it.
 
-Every function cites the reference file:line it follows.  What pins each one
-(tests/test_oracle.py):
+Every function cites the reference file:line it follows.  What pins each one:
 
-* ``traj2sparsemat`` / ``to_bmsparse`` / ``cpp_prepare``: brute-force dense
-  construction that loops over grid cells instead of neighbour offsets
-  (``dense_gridding_matrix_bruteforce``), row sums == 1, Gut == Gu^T exactly,
-  ascending columns, jump decode, every ``check()`` rule, and the
-  bmSparseMat_r_nJump2index closed form ``np.repeat``.
+* ``traj2sparsemat`` / ``to_bmsparse`` / ``l_squeeze`` / ``cpp_prepare``
+  (tests/test_oracle_structure.py): a brute-force dense construction that
+  loops over grid cells and their periodic images instead of neighbour
+  offsets (``dense_gridding_matrix_bruteforce``; 1-D/2-D/3-D, Gauss and
+  Kaiser, odd and even nWin, deleted points, windows wider than the grid),
+  rows of Gu summing to 1, Gut == Gu^T exactly, ascending columns, the
+  absolute first jump, Gut's kept rows = the touched cells, every
+  ``check()`` rule; the bmSparseMat_r_nJump2index closed form ``np.repeat``
+  (tests/test_oracle_port.py).
 * ``spmv``: dense float64 product of the decoded matrix; the float32
   sequential form is the C port (port.c), pinned bit-exact against the
   reference's own compiled kernel.
 * ``bmK``: the closed form Sum_x w(x) cos(2 pi k.x / N) evaluated directly
-  (``bmK_direct``), which must equal the reference's oversampled-grid DFT.
-* ``cfft``/``cifft``: numpy.fft with explicit shifts vs the (-1)^n modulation
-  identity and vs a dense DFT matrix on tiny sizes.
-* ``shanna``/``nakatsha``: adjointness <A x, y>_HY = <x, A^H y>_HX.
+  (``bmK_direct``), which must equal the reference's oversampled-grid DFT,
+  and the continuous Gaussian transform for a wide window.
+* ``cfft``/``cifft``/``zero_fill``/``crop``: an explicit centred DFT matrix
+  on tiny sizes; the 1-based window N/2+1-n/2 : N/2+n/2.
+* ``shanna``/``nakatsha``: the dense composition Gu F Z diag(KFC) on tiny
+  problems, and adjointness <A x, y>_HY = <x, A^H y>_HX in float64.
 * ``sensa``: on a tiny problem, CG converges to the dense least-squares
   solution of the normal equations.
 * ``back_gradient``/``back_gradient_T``: the transpose identity on inputs whose
@@ -461,9 +466,15 @@ def check(G):
 def dense_gridding_matrix_bruteforce(t, N_u, dK_u, kernelType=None, nWin=None,
                                      kernelParam=None):
     """Independent restatement used to pin traj2sparsemat: loops over every
-    grid cell of the (periodic) grid and keeps the cells whose per-axis
-    wrapped offset from the point's rounded position is within the window.
-    Returns the un-normalised dense [nPt, Nu] weight matrix (w * Du)."""
+    cell of the periodic grid and, per axis, over every periodic image of the
+    cell (offset g + 1 - base + k N for integer k), keeping the images whose
+    offset lies in the window: [-h, h] around round(t) for odd nWin,
+    [-nWin/2 - 1, nWin/2] around floor(t) for even nWin
+    (bmTraj2SparseMat.m:134-164).  A window wider than the grid visits a
+    cell through several images, whose weights add (MATLAB's sparse() sums
+    duplicates).  Points outside [1, N] on any axis are dropped
+    (bmTraj2SparseMat.m:116-126).  Returns the un-normalised dense [nPt, Nu]
+    weight matrix (w * Du)."""
     kernelType, nWin, kernelParam = kernel_defaults(kernelType, nWin, kernelParam)
     t = np.atleast_2d(np.asarray(t, dtype=np.float64))
     imDim, nPt = t.shape
@@ -474,31 +485,39 @@ def dense_gridding_matrix_bruteforce(t, N_u, dK_u, kernelType=None, nWin=None,
     Du = 1.0
     for d in range(imDim):
         Du *= (1.0 / dK[d]) / N[d]
-    h = int(nWin) // 2
-    assert int(nWin) % 2 == 1
-    grids = np.meshgrid(*[np.arange(n) for n in N], indexing='ij')
-    cell_lin = np.zeros(grids[0].shape, dtype=np.int64)
-    stride = 1
-    for d in range(imDim):
-        cell_lin += grids[d] * stride
-        stride *= N[d]
+    nW = int(nWin)
+    if nW % 2 == 1:
+        lo, hi, fshift = -(nW // 2), nW // 2, 0.5
+    else:
+        lo, hi, fshift = -nW // 2 - 1, nW // 2, 0.0
     for p in range(nPt):
         tg = [t[d, p] / dK[d] + math.trunc(N[d] / 2 + 1) for d in range(imDim)]
         if any(tg[d] < 1 or tg[d] > N[d] for d in range(imDim)):
             continue
-        base = [math.floor(tg[d] + 0.5) for d in range(imDim)]   # 1-based
-        ok = np.ones(grids[0].shape, dtype=bool)
-        offs = []
+        base = [math.floor(tg[d] + fshift) for d in range(imDim)]   # 1-based
+        # per axis: the (cell, offset) pairs of every periodic image in the window
+        per_axis = []
         for d in range(imDim):
-            # signed wrapped offset of cell (1-based grids[d]+1) from base
-            o = np.mod(grids[d] + 1 - base[d] + N[d] // 2, N[d]) - N[d] // 2
-            ok &= np.abs(o) <= h
-            offs.append(o)
-        sq = np.zeros(grids[0].shape)
+            cells, offs = [], []
+            kmax = (hi - lo) // N[d] + 2
+            for g in range(N[d]):
+                for k in range(-kmax, kmax + 1):
+                    o = g + 1 - base[d] + k * N[d]
+                    if lo <= o <= hi:
+                        cells.append(g)
+                        offs.append(o)
+            per_axis.append((np.array(cells, np.int64), np.array(offs, np.float64)))
+        mesh_c = np.meshgrid(*[pa[0] for pa in per_axis], indexing='ij')
+        mesh_o = np.meshgrid(*[pa[1] for pa in per_axis], indexing='ij')
+        lin = np.zeros(mesh_c[0].shape, dtype=np.int64)
+        sq = np.zeros(mesh_c[0].shape)
+        stride = 1
         for d in range(imDim):
-            sq = sq + ((tg[d] - base[d]) - offs[d]) ** 2
+            lin += mesh_c[d] * stride
+            stride *= N[d]
+            sq = sq + ((tg[d] - base[d]) - mesh_o[d]) ** 2
         w = kernel_weight(np.sqrt(sq), kernelType, kernelParam)
-        A[p, cell_lin[ok]] += w[ok] * Du
+        np.add.at(A[p], lin.ravel(), w.ravel() * Du)
     return A
 
 
