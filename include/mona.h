@@ -187,6 +187,19 @@ int mona_sparse_export(mona_sparse S, int* r_jump, int* r_nJump, float* m_val,
  * v [r_size][n_vec] float2, w [l_size][n_vec] float2. */
 int mona_sparse_apply(mona_sparse S, const void* v, void* w, int n_vec,
                       hipStream_t stream);
+/* mona_sparse_apply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The partial sums of rows
+ * longer than 64 entries go to a file-scope device array of 64 MiB per GPU
+ * (held until the process ends) instead of a per-matrix buffer, so
+ * n_chunk * n_vec * 8 bytes must fit in 64 MiB (n_chunk = sum over rows of
+ * more than 64 entries of ceil(entries / 64)); larger calls return
+ * MONA_ERR_ARG.  S must already be on the device (mona_sparse_apply or
+ * mona_plan_set_frame uploads it), else MONA_ERR_STATE.  Results equal
+ * mona_sparse_apply's bit for bit.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (other streams or threads) race. */
+int mona_sparse_apply_noalloc(mona_sparse S, const void* v, void* w, int n_vec,
+                              hipStream_t stream);
 
 /* --- trajectory -> gridding matrices ----------------------------------
  * bmTraj2SparseMat (src/gridding123/m/bmTraj2SparseMat.m:12-314):

@@ -55,6 +55,7 @@ SIGNATURES = {
     "mona_sparse_info": (I, [P, LLP, IP, IP, IP, IP]),
     "mona_sparse_export": (I, [P, IP, IP, FP, IP]),
     "mona_sparse_apply": (I, [P, P, P, I, P]),
+    "mona_sparse_apply_noalloc": (I, [P, P, P, I, P]),
     "mona_traj2sparsemat": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP]),
     "mona_traj2sparsemat_device": (I, [I, I, DP, DP, IP, DP, I, I, DP, I, I, PP, PP, PP, P]),
     "mona_deform2sparsemat": (I, [I, IP, DP, DP, I, I, PP, PP, PP]),

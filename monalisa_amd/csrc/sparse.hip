@@ -646,7 +646,7 @@ int xcd_group(bool adjoint) {
 // 16-byte aligned, operator matrices only).
 template <int CPT, int U, class TV, class TW>
 void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
-                    hipStream_t s) {
+                    hipStream_t s, float2* part_ext) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
     const int64_t total = (int64_t)V.rows * (n_vec / CPT);
     const SpEntry* cv = S.cv.as<SpEntry>();
@@ -661,10 +661,13 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
         MONA_HIP(hipGetLastError());
     }
     if (V.n_long > 0) {
-        S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+        float2* part = part_ext;
+        if (!part) {
+            S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+            part = S.partial.as<float2>();
+        }
         const int64_t tc = V.n_chunk * (n_vec / CPT);
         const int64_t* cb = S.chunk_begin.as<int64_t>();
-        float2* part = S.partial.as<float2>();
         // Every chunk holds 64 entries, so a deeper unroll wastes no tail: at
         // CPT = 4, 8 entries per batch keep twice the gathers in flight (n2u
         // long-row chunks of the 3D N=128 20-coil CG-SENSE: 358 -> 329 us).
@@ -685,7 +688,7 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
 
 template <class TV, class TW>
 void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s,
-                   const SpView* view, bool zero_rest) {
+                   const SpView* view, bool zero_rest, float2* part_ext = nullptr) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
     if (S.cv.p && S.nnz > 0) {
         SpView V;
@@ -709,11 +712,11 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
         // (bf16 grids: four vectors per work-item, 8 gathers in flight,
         // 16-byte loads; 25% faster than the float2 choice on the same
         // 20-coil problem)
-        if (std::is_same<TV, bf16x2>::value && !yscale && cpt == 4) spmv_cv_launch<4, 8>(S, V, v, w, n_vec, yscale, s);
-        else if (!yscale && cpt >= 2) spmv_cv_launch<2, 16>(S, V, v, w, n_vec, yscale, s);
-        else if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s);
-        else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s);
-        else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s);
+        if (std::is_same<TV, bf16x2>::value && !yscale && cpt == 4) spmv_cv_launch<4, 8>(S, V, v, w, n_vec, yscale, s, part_ext);
+        else if (!yscale && cpt >= 2) spmv_cv_launch<2, 16>(S, V, v, w, n_vec, yscale, s, part_ext);
+        else if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s, part_ext);
+        else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s, part_ext);
+        else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s, part_ext);
         return;
     }
     MONA_CHECK(!view, MONA_ERR_INTERNAL, "matrix views need the packed entries");
@@ -733,10 +736,13 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
     }
     MONA_HIP(hipGetLastError());
     if (S.n_long > 0) {
-        S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+        float2* part = part_ext;
+        if (!part) {
+            S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
+            part = S.partial.as<float2>();
+        }
         int64_t tc = S.n_chunk * n_vec;
         const int64_t* cb = S.chunk_begin.as<int64_t>();
-        float2* part = S.partial.as<float2>();
         if (yscale)
             spmv_c2_chunks<true><<<blocks_for(tc, 256), 256, 0, s>>>(cb, S.n_chunk, S.col.as<int>(),
                                                                      S.val.as<float>(), n_vec, v, yscale, part);
@@ -985,6 +991,36 @@ HostProduct mk(int r_size, int* r_jump, int* r_nJump, float* m_val, int l_size, 
 }
 
 }  // namespace
+
+// ------------------------------------------ allocation-free product scratch
+// mona_sparse_apply_noalloc keeps the long-row partial sums in this
+// file-scope device array instead of a per-matrix buffer.  Every device the
+// library runs on holds its own copy until the process ends.  In host code
+// the name `scratch` is not a device address: the device address comes from
+// hipGetSymbolAddress, taken once per device (noalloc_scratch) and used for
+// every kernel argument.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) float2 scratch[kNoAllocScratchBytes / sizeof(float2)];
+
+static float2* noalloc_scratch() {
+    static std::mutex mu;
+    static std::vector<float2*> per_device;
+    int dev = -1;
+    MONA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            throw Error{MONA_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e)};
+        }
+        per_device[dev] = static_cast<float2*>(ptr);
+    }
+    return per_device[dev];
+}
+
 }  // namespace mona
 
 using namespace mona;
@@ -1163,6 +1199,26 @@ int mona_sparse_apply(mona_sparse S, const void* v, void* w, int n_vec, hipStrea
         sparse_upload(S->S, stream);
         spmv_c2_rowmajor(S->S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, nullptr,
                          stream);
+    });
+}
+
+int mona_sparse_apply_noalloc(mona_sparse S, const void* v, void* w, int n_vec, hipStream_t stream) {
+    return guarded([&] {
+        MONA_CHECK(S && v && w && n_vec > 0, MONA_ERR_ARG, "bad argument");
+        SparseDev& D = S->S;
+        {
+            std::lock_guard<std::mutex> lk(D.mu);
+            MONA_CHECK(D.on_device, MONA_ERR_STATE,
+                       "the matrix is not on the device yet (mona_sparse_apply or mona_plan_set_frame uploads it; "
+                       "this entry point allocates nothing)");
+        }
+        const size_t need = (size_t)D.n_chunk * (size_t)n_vec * sizeof(float2);
+        MONA_CHECK(need <= kNoAllocScratchBytes, MONA_ERR_ARG,
+                   "long-row partial sums need " + std::to_string(need) + " bytes of scratch, more than the " +
+                       std::to_string(kNoAllocScratchBytes) + "-byte maximum of mona_sparse_apply_noalloc");
+        float2* sc = noalloc_scratch();
+        spmv_c2_typed(D, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, nullptr, stream,
+                      nullptr, true, sc);
     });
 }
 
