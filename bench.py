@@ -29,10 +29,10 @@ CONFIGS = {
     # small shape for quick checks
     "small": dict(n=64, nCh=8, nSeg=10, nShot=200, nSample=128, nCGD=10),
     # C1 with bf16 storage: CG-SENSE by iterative refinement to a relative
-    # normal-equation residual < 1e-5
-    "c3": dict(n=128, nCh=20, nSeg=25, nShot=400, nSample=256, nCGD=20, mixed=True, nInner=10, nOuter=30,
+    # normal-equation residual < 1e-5 (consistent data; see bench_mixed)
+    "c3": dict(n=128, nCh=20, nSeg=25, nShot=400, nSample=256, nCGD=20, mixed=True, nInner=10, nOuter=40,
                tol=1e-5),
-    "c3small": dict(n=32, nCh=8, nSeg=10, nShot=100, nSample=64, nCGD=10, mixed=True, nInner=10, nOuter=30,
+    "c3small": dict(n=32, nCh=8, nSeg=10, nShot=100, nSample=64, nCGD=10, mixed=True, nInner=10, nOuter=40,
                     tol=1e-5),
     # recon serving: Poisson arrivals over a 2D / 3D protocol mix, 10-step CG-SENSE each
     "c4": dict(serve=True, nCGD=10, n_sat=96, n_poisson=240, load=0.8, max_batch=4, workers=3,
@@ -169,9 +169,10 @@ def bench_binned(args, cfg):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, x
 
+    # the tuned FFT variant serves both forms, so they compute the same sums
+    tile = P.autotune(0)
     P.set_options(batch_frames=False, graph=False)
     s_plain, x_plain = timed(max(1, args.steps // 2), args.warmup)
-    tile = P.autotune(0)
     P.set_options(batch_frames=True, graph=True)
     s_batch, x_batch = timed(args.steps, args.warmup)
     a, b = x_batch.cpu().numpy(), x_plain.cpu().numpy()
@@ -209,26 +210,65 @@ def bench_binned(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def _fp64_operators(P, G, x, yk, n_u, N_u, nCh):
+    """Float64 host products of the forward operator at x and the adjoint at
+    yk with the plan's own gridding matrices and coil weights (KFC and
+    KFC_conj read back from the plan): Gu F Z (x .* KFC) and
+    sum_c KFC_conj .* crop(F^-1 Gut yk) (bmShanna_MATLAB.m:6-56,
+    bmNakatsha_MATLAB.m:6-60), one coil at a time."""
+    import scipy.fft as sfft
+    from scipy import sparse
+
+    def csr(M):
+        col, rowptr, rows = M.decode()
+        rid = np.repeat(rows, np.diff(rowptr))
+        return sparse.csr_matrix((M.m_val.astype(np.float64), (rid, col)), shape=(M.l_size, M.r_size))
+
+    Gu, Gut = csr(G["Gu"]), csr(G["Gut"])
+    kfc, kfcc = P.KFC()
+    kfc, kfcc = kfc.cpu().numpy().astype(np.complex128), kfcc.cpu().numpy().astype(np.complex128)
+    x = x.cpu().numpy().astype(np.complex128)
+    yk = yk.cpu().numpy().astype(np.complex128)
+    shp = tuple(N_u[::-1])               # C order: x fastest
+    sl = tuple(slice(N // 2 - n // 2, N // 2 - n // 2 + n) for N, n in zip(N_u[::-1], n_u[::-1]))
+    w = os.cpu_count() or 8
+    yf = np.zeros((G["Gu"].l_size, nCh), np.complex128)
+    xa = np.zeros(int(np.prod(n_u)), np.complex128)
+    for c in range(nCh):
+        z = np.zeros(shp, np.complex128)
+        z[sl] = (x * kfc[:, c]).reshape(tuple(n_u[::-1]))
+        g = sfft.fftshift(sfft.fftn(sfft.ifftshift(z), workers=w))
+        yf[:, c] = Gu @ g.reshape(-1)
+        g = (Gut @ yk[:, c]).reshape(shp)
+        g = sfft.fftshift(sfft.ifftn(sfft.ifftshift(g), workers=w))
+        xa += kfcc[:, c] * g[sl].reshape(-1)
+    return yf, xa
+
+
 def bench_mixed(args, cfg):
-    """Mixed precision at the full workload: the float CG-SENSE recon
-    (nCGD steps) sets the target, tol = max(cfg tol, its relative
-    normal-equation residual); CG with bf16-storage products and float
-    residual replacement every nInner steps runs to that target, and so does
-    the same refinement with float products.  Same inputs; plus the bf16
-    operators' error against the float operators."""
+    """Mixed precision at the full workload (3D N=128, 20 coils).
+
+    * Target (the configuration's): iterative refinement until the relative
+      normal-equation residual is below 1e-5, CG with bf16-storage products
+      and float residual replacement every nInner steps; timed beside the
+      same refinement with float products.  The noisy benchmark data make
+      these normal equations converge far too slowly for 1e-5 (the float
+      refinement is still above 5e-4 after 6,000 steps), so the target is
+      run on consistent data, y = M x_phantom.
+    * Noisy data: both refinements after the same fixed number of steps.
+    * bf16 storage without residual replacement: where it stalls.
+    * Forward and adjoint operators (FFT + gridding, bf16 and float storage)
+      against float64 host products with the same Gu, Gut and coil weights."""
     import torch
     prob = build_problem(cfg)
-    P, y, ve = prob["P"], prob["y"], prob["ve"]
+    P, yn, ve = prob["P"], prob["y"], prob["ve"]
     nu = int(np.prod(prob["n_u"]))
+    y0 = P.bmShanna(prob["xd"])          # consistent data
+    tol = cfg["tol"]
 
-    def sensa32():
+    def refined(y, bf16, tol, nOuter=None, nInner=None):
         x = torch.zeros(nu, dtype=torch.complex64, device="cuda")
-        P.bmSensa(x, y, ve, cfg["nCGD"], nIter=1)
-        return x, None
-
-    def refined(bf16, tol):
-        x = torch.zeros(nu, dtype=torch.complex64, device="cuda")
-        h = P.sensa_refined(x, y, ve, cfg["nInner"], cfg["nOuter"], tol, bf16=bf16)
+        h = P.sensa_refined(x, y, ve, nInner or cfg["nInner"], nOuter or cfg["nOuter"], tol, bf16=bf16)
         return x, h
 
     def timed(fn, steps):
@@ -241,19 +281,30 @@ def bench_mixed(args, cfg):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, x, h
 
-    s_cg, x_cg, _ = timed(sensa32, max(1, args.steps // 2))
-    rho32 = P.sensa_refined(x_cg.clone(), y, ve, 1, 0)[0]
-    tol = max(cfg["tol"], rho32)
-    s16, x16, h16 = timed(lambda: refined(True, tol), args.steps)
-    s32, x32, h32 = timed(lambda: refined(False, tol), max(1, args.steps // 2))
-    a, b, c = x16.cpu().numpy(), x32.cpu().numpy(), x_cg.cpu().numpy()
+    s16, x16, h16 = timed(lambda: refined(y0, True, tol), args.steps)
+    s32, x32, h32 = timed(lambda: refined(y0, False, tol), max(1, args.steps // 2))
+    # bf16 products with no residual replacement: the level it stalls at
+    _, hplain = refined(y0, True, 0.0, nOuter=1, nInner=h16.steps + 100)
+    # noisy benchmark data: the same fixed budget of steps for both
+    budget = cfg["nInner"] * 10
+    _, hn16 = refined(yn, True, 0.0, nOuter=10)
+    _, hn32 = refined(yn, False, 0.0, nOuter=10)
+    a, b = x16.cpu().numpy(), x32.cpu().numpy()
+    # operators against float64 host products (same matrices and coil weights)
     xo = prob["xd"]
-    y32 = P.bmShanna(xo)
-    y16 = P.bmShanna_bf16(xo)
-    ea = P.bmNakatsha(y32)
-    eb = P.bmNakatsha_bf16(y32)
-    fwd_err = float(torch.linalg.norm(y16 - y32) / torch.linalg.norm(y32))
-    adj_err = float(torch.linalg.norm(eb - ea) / torch.linalg.norm(ea))
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    yk = torch.complex(torch.randn(yn.shape, generator=gen, device="cuda"),
+                       torch.randn(yn.shape, generator=gen, device="cuda")).contiguous()
+    t0 = time.time()
+    yf64, xa64 = _fp64_operators(P, prob["G"], xo, yk, prob["n_u"], prob["N_u"], prob["nCh"])
+    log(f"[bench] float64 host operators in {time.time() - t0:.1f}s")
+
+    def err(g, r):
+        g = g.cpu().numpy().astype(np.complex128)
+        return float(np.linalg.norm(g.reshape(r.shape) - r) / np.linalg.norm(r))
+
+    ops = {"forward_bf16": err(P.bmShanna_bf16(xo), yf64), "forward_fp32": err(P.bmShanna(xo), yf64),
+           "adjoint_bf16": err(P.bmNakatsha_bf16(yk), xa64), "adjoint_fp32": err(P.bmNakatsha(yk), xa64)}
     ph = prob["phantom"]
 
     def img_err(v):
@@ -261,8 +312,8 @@ def bench_mixed(args, cfg):
         return float(np.linalg.norm(fit * v - ph) / np.linalg.norm(ph))
 
     out = {
-        "metric": "mixed-precision CG-SENSE recons/s at 3D N=%d %dch (bf16 storage, refined to the fp32 "
-                  "residual level)" % (cfg["n"], cfg["nCh"]),
+        "metric": "mixed-precision CG-SENSE recons/s at 3D N=%d %dch (bf16 storage, float residual "
+                  "replacement until relative residual < %g)" % (cfg["n"], cfg["nCh"], tol),
         "value": 1.0 / s16,
         "unit": "recons/s",
         "n_gpus": 1,
@@ -273,25 +324,24 @@ def bench_mixed(args, cfg):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16 storage / fp32 arithmetic (fp64 CG scalars, fp32 residual replacement)",
-        "data": "synthetic (seeded koosh-ball trajectory, Gaussian-blob coil maps, ellipsoid phantom, 30 dB noise)",
+        "data": "synthetic (seeded koosh-ball trajectory, Gaussian-blob coil maps, ellipsoid phantom; "
+                "consistent data y = M x for the 1e-5 target, 30 dB noise for the fixed-budget comparison)",
         "config": {"workload": f"3D koosh-ball CG-SENSE n_u={cfg['n']}^3 N_u={2 * cfg['n']}^3 "
                                f"{cfg['nSeg'] * cfg['nShot']} spokes x {cfg['nSample']} samples, {cfg['nCh']} coils; "
-                               f"target = residual of {cfg['nCGD']} fp32 CG steps (floor {cfg['tol']}), float "
-                               f"residual replacement every {cfg['nInner']} steps",
+                               f"refinement to relative normal-equation residual < {tol}, float residual "
+                               f"replacement every {cfg['nInner']} steps",
                    "parallelism": "single GPU"},
         "target_rel_residual": tol,
-        "fp32_cg_rel_residual": rho32,
-        "fp32_cg_recons_per_s": 1.0 / s_cg,
         "cg_steps_bf16": h16.steps,
         "cg_steps_fp32_refined": h32.steps,
         "final_rel_residual_bf16": h16[-1],
         "final_rel_residual_fp32_refined": h32[-1],
         "fp32_refined_recons_per_s": 1.0 / s32,
+        "bf16_without_replacement_rel_residual": {"steps": hplain.steps, "value": hplain[-1]},
+        "noisy_data_fixed_budget": {"steps": budget, "rel_residual_bf16": hn16[-1], "rel_residual_fp32": hn32[-1]},
         "image_rel_l2_bf16_vs_fp32_refined": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
-        "image_rel_l2_bf16_vs_fp32_cg": float(np.linalg.norm(a - c) / np.linalg.norm(c)),
-        "forward_op_rel_err_bf16_vs_fp32": fwd_err,
-        "adjoint_op_rel_err_bf16_vs_fp32": adj_err,
-        "image_rel_err_vs_phantom": {"bf16_refined": img_err(a), "fp32_refined": img_err(b), "fp32_cg": img_err(c)},
+        "operator_rel_err_vs_fp64_host": ops,
+        "image_rel_err_vs_phantom": {"bf16_refined": img_err(a), "fp32_refined": img_err(b)},
     }
     print(json.dumps(out), flush=True)
 

@@ -106,8 +106,10 @@ void bmSparseMat_rR_oBlock(int r_size, int* r_jump, int* r_nJump, float* m_val,
 /* Multi-block row-layout products: the block partition only changes how
  * the reference splits the work over threads; entries are still a single
  * running delta sequence, except that the first entry of every block (at
- * m_block_start[b]) and the first output row of every block (at
- * l_block_start[b]) are absolute 0-based.
+ * m_block_start[b]) is absolute 0-based, and, for a squeezed matrix
+ * (l_jump != NULL, the reference's l_sparsity branch), so is the first
+ * output row of every block (l_jump[l_block_start[b]]).  Rows not stored
+ * are zero in w.
  * Replace .../bmSparseMat_cR_nBlock_omp/bmSparseMat_cR_nBlock_omp_mex.cpp:97-483
  * and .../bmSparseMat_rR_nBlock_omp/bmSparseMat_rR_nBlock_omp_mex.cpp:97-436. */
 void bmSparseMat_cR_nBlock_omp(int r_size, int* r_jump, int* r_nJump, float* m_val,

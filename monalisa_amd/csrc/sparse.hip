@@ -461,12 +461,10 @@ static void decode_device(SparseDev& S, const int* d_r_jump, const int* d_r_nJum
     MONA_HIP(hipMemcpyAsync(S.val.p, d_m_val, S.nnz * sizeof(float), hipMemcpyDeviceToDevice, s));
     scan_i32_to_i64_exclusive(d_r_nJump, S.rowptr.as<int64_t>(), S.l_nJump, s);
     if (S.squeezed) {
+        // l_jump is one running-delta sequence (multi-block callers re-encode
+        // their block-absolute starts before the upload)
         S.rowidx.alloc((size_t)S.l_nJump * sizeof(int));
-        if (nBlock <= 1) {
-            scan_i32_to_i32_inclusive(d_l_jump, S.rowidx.as<int>(), S.l_nJump, s);
-        } else {
-            MONA_CHECK(false, MONA_ERR_ARG, "multi-block squeezed matrices are not supported");
-        }
+        scan_i32_to_i32_inclusive(d_l_jump, S.rowidx.as<int>(), S.l_nJump, s);
     }
     // range checks: every decoded column must address v, every row w
     DevBuf flag(sizeof(int) * 2);
@@ -903,8 +901,33 @@ void host_product(const HostProduct& a, Layout lay, const float* v0, const float
     MONA_HIP(hipMemcpyAsync(C.slot[0].p, a.r_jump, nnz * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipMemcpyAsync(C.slot[1].p, a.r_nJump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipMemcpyAsync(C.slot[2].p, a.m_val, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+    // Multi-block squeezed matrices (cpp_prepare('multi_block'),
+    // bmSparseMat.m:131-165): the first l_jump of every block, at
+    // l_block_start[b], is the absolute 0-based output row.  Re-encoded here
+    // as one running-delta sequence so the one-block row decode applies.
+    std::vector<int> lj_one;
+    if (squeezed && a.nBlock > 1) {
+        MONA_CHECK(a.l_block_start, MONA_ERR_ARG, "NULL l_block_start");
+        lj_one.resize(a.l_nJump);
+        int64_t row = 0, prev = 0;
+        int b = 0;
+        for (int i = 0; i < a.l_nJump; i++) {
+            if (b < a.nBlock && i == a.l_block_start[b]) {
+                row = a.l_jump[i];
+                b++;
+            } else {
+                row += a.l_jump[i];
+            }
+            const int64_t d = i == 0 ? row : row - prev;
+            MONA_CHECK(row >= 0 && row < a.l_size && d >= INT32_MIN && d <= INT32_MAX, MONA_ERR_ARG,
+                       "l_jump decodes to a row outside [0, l_size)");
+            lj_one[i] = (int)d;
+            prev = row;
+        }
+    }
     if (squeezed)
-        MONA_HIP(hipMemcpyAsync(C.slot[3].p, a.l_jump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(C.slot[3].p, lj_one.empty() ? a.l_jump : lj_one.data(), (size_t)a.l_nJump * sizeof(int),
+                                hipMemcpyHostToDevice, s));
     MONA_HIP(hipMemcpyAsync(C.slot[4].p, v0, vlen * sizeof(float), hipMemcpyHostToDevice, s));
     if (v1) MONA_HIP(hipMemcpyAsync(C.slot[5].p, v1, vlen * sizeof(float), hipMemcpyHostToDevice, s));
 
@@ -913,7 +936,6 @@ void host_product(const HostProduct& a, Layout lay, const float* v0, const float
     S.squeezed = squeezed;
     DevBuf mstart;
     if (a.nBlock > 1) {
-        MONA_CHECK(!squeezed, MONA_ERR_ARG, "multi-block squeezed matrices are not supported");
         MONA_CHECK(a.m_block_start && a.l_block_start && a.block_length, MONA_ERR_ARG,
                    "NULL block arrays");
         // block starts must tile the stored rows in order

@@ -34,3 +34,7 @@ def test_bench_config_runs(config):
     assert d["value"] > 0 and d["n_gpus"] == 1
     if config == "small":
         assert d["roofline"]["achieved"] > 0 and 0 < d["roofline"]["frac"] < 1
+    if config == "c2small":
+        # one batched, graph-replayed launch per gridding op over every bin
+        # gives the per-bin eager results bit for bit
+        assert d["batched_vs_per_bin_rel_l2"] == 0.0

@@ -63,3 +63,54 @@ def test_full_size_operators_are_linear_and_adjoint(c1):
     lhs = torch.vdot(y.reshape(-1).to(torch.complex128), Mx1.reshape(-1).to(torch.complex128))
     rhs = torch.vdot(Mhy.to(torch.complex128), x1.to(torch.complex128))
     assert (abs(lhs - rhs) / abs(lhs)).item() < 1e-5
+
+
+def test_full_size_noise_free_cg_residuals(c1):
+    """bmSensa at the full workload on noise-free data y = M x_phantom, from
+    x0 = 0 with nCGD = 1 .. 20 (bmSensa.m:39-85): CG on the normal
+    equations minimises <r, HY r> over a growing Krylov space, so the final
+    residual of k steps never increases with k.  The image error against
+    the phantom is printed beside the one of 30 dB noisy data."""
+    import math
+    import torch
+    from monalisa_amd import synth
+    from monalisa_amd.nufft import Plan
+    print("[gpu] full-size CG-SENSE on noise-free data, nCGD = 1..20", flush=True)
+    n, nCh = c1["n"], 20
+    n_u, N_u = [n] * 3, [2 * n] * 3
+    rng = np.random.default_rng(1234)
+    C = synth.coil_maps(n_u, nCh, rng)
+    ph = synth.ellipsoid_phantom(n_u, rng)
+    P = Plan(N_u, n_u, [0.5] * 3, nCh, "kaiser")
+    P.set_coils(torch.from_numpy(C).cuda())
+    P.set_frame(0, c1["H"]["Gu"], c1["H"]["Gut"])
+    y = P.bmShanna(torch.from_numpy(ph).cuda())
+    ve = torch.from_numpy(c1["ve"].astype(np.float32)).cuda()
+    yy = float(torch.sum(ve[:, None] * torch.abs(y) ** 2).item())   # <y, HY y> (ve_max = max ve)
+
+    def err(x):
+        v = x.cpu().numpy()
+        fit = np.vdot(v, ph) / np.vdot(v, v)
+        return float(np.linalg.norm(fit * v - ph) / np.linalg.norm(ph))
+
+    res, errs = [], []
+    for k in range(1, 21):
+        x = torch.zeros(n ** 3, dtype=torch.complex64, device="cuda")
+        res.append(P.bmSensa(x, y, ve, k)[0])
+        errs.append(err(x))
+    rel = [r / yy for r in res]
+    print("noise-free relative residual <r,HY r>/<y,HY y> for k = 1..20:", " ".join(f"{v:.3e}" for v in rel))
+    print("noise-free image error vs phantom for k = 1..20:", " ".join(f"{v:.3f}" for v in errs))
+    for k in range(1, 20):
+        assert res[k] <= res[k - 1] * (1 + 1e-5), (k, res[k - 1], res[k])
+    assert rel[-1] < 0.1 * rel[0]
+    # the same 20 steps on 30 dB noisy data (the benchmark's input)
+    gen = torch.Generator(device="cuda").manual_seed(1234)
+    rms = torch.sqrt(torch.mean(torch.abs(y) ** 2)).item()
+    sigma = rms * 10 ** (-30 / 20) / math.sqrt(2)
+    yn = (y + torch.complex(torch.randn(y.shape, generator=gen, device="cuda"),
+                            torch.randn(y.shape, generator=gen, device="cuda")) * sigma).contiguous()
+    x = torch.zeros(n ** 3, dtype=torch.complex64, device="cuda")
+    P.bmSensa(x, yn, ve, 20)
+    print(f"image error vs phantom after 20 steps: noise-free {errs[-1]:.3f}, 30 dB noise {err(x):.3f}")
+    assert errs[-1] < errs[0]

@@ -1,7 +1,7 @@
 """Mixed precision on the GPU: the operators with bf16 storage of the
 grid-side arrays and k-space against the float64 oracle, and CG-SENSE by
 iterative refinement whose result is checked with the oracle's float64
-normal-equation residual."""
+normal-equation residual and its float64 converged solution."""
 import numpy as np
 import pytest
 
@@ -55,33 +55,74 @@ def _normal_residual(x, y, ve, p, R, KFC, KFCc):
     return float(np.linalg.norm(g) / np.linalg.norm(g0))
 
 
-@pytest.mark.parametrize("bf16", [True, False])
-def test_sensa_refined_restores_fp32_level(bf16):
-    """The refinement with bf16 products reaches the relative normal-equation
-    residual of 20 float CG-SENSE steps (or 1e-5 if that is lower), as the
-    float64 oracle measures it on the returned image."""
+def test_sensa_refined_reaches_1e5():
+    """Iterative refinement to a relative normal-equation residual below
+    1e-5 (bmSensa.m:39-85's system; the mixed-precision target) on
+    consistent data y = M x, with bf16 and with float storage: the solver's
+    residual and the float64 oracle's residual of the returned image are
+    both below 1e-5.  With bf16 storage and no residual replacement the same
+    CG stalls above 1e-5: the float replacement is what restores the
+    float-level residual.  The system is ill-conditioned (a residual of
+    1e-5 pins the image only to about cond * 1e-5), so the image gaps --
+    bf16 against float refinement, and against a float64 oracle CG of the
+    same step count -- are printed and bounded loosely."""
     import torch
-    print(f"[gpu] sensa refined bf16={bf16}", flush=True)
+    print("[gpu] sensa refined to 1e-5: bf16 and float", flush=True)
+    p, P, R, KFC, KFCc = _setup()
+    xt = torch.from_numpy(p["x"]).cuda()
+    y = P.bmShanna(xt).cpu().numpy()
+    ve = p["ve"].astype(np.float32)
+    yd, ved = torch.from_numpy(y).cuda(), torch.from_numpy(ve).cuda()
+    tol = 1e-5
+    out = {}
+    for bf16 in (True, False):
+        x = torch.zeros_like(xt)
+        hist = P.sensa_refined(x, yd, ved, nInner=10, nOuter=60, tol=tol, bf16=bf16)
+        assert hist[0] == 1.0
+        assert hist[-1] < tol, hist
+        xg = x.cpu().numpy()
+        res64 = _normal_residual(xg, y, ve, p, R, KFC, KFCc)
+        assert res64 < 1.05 * tol, res64
+        out[bf16] = (xg, hist, res64)
+    # float64 oracle CG-SENSE with as many steps as the float refinement took
+    # (data scaled by 1e6 so bmSensa's absolute early exit, bmSensa.m:54,
+    # does not stop it first; CG is otherwise scale invariant)
+    Cn = nd(p["C"].astype(np.complex128), p["n_u"], p["nCh"])
+    k = out[False][1].steps
+    x64, _ = O.sensa(np.zeros(p["n_u"], complex), 1e6 * y.astype(np.complex128), ve.astype(np.float64), Cn,
+                     R["Gu"], R["Gut"], p["N_u"], p["n_u"], p["dK_u"], k, None, 1, kernel=(p["kernel"], None, None))
+    x64 = flat(x64, p["n_u"]) / 1e6
+    gap16_32 = rel(out[True][0], out[False][0])
+    gap32_64 = rel(out[False][0], x64)
+    gap16_64 = rel(out[True][0], x64)
+    print(f"steps bf16 {out[True][1].steps} float {k}; float64 residuals {out[True][2]:.2e} {out[False][2]:.2e}; "
+          f"image gaps bf16-float {gap16_32:.2e}, float-float64 {gap32_64:.2e}, bf16-float64 {gap16_64:.2e}",
+          flush=True)
+    assert gap16_32 < 5e-3 and gap32_64 < 5e-3 and gap16_64 < 5e-3
+    x = torch.zeros_like(xt)
+    plain = P.sensa_refined(x, yd, ved, nInner=out[True][1].steps + 100, nOuter=1, tol=0.0, bf16=True)
+    assert plain[-1] > tol, plain
+
+
+def test_sensa_refined_bf16_tracks_fp32_on_noisy_data():
+    """On noisy data (the normal equations converge slowly) the bf16 and the
+    float refinement reach the same residual after the same replacements."""
+    import torch
+    print("[gpu] sensa refined, noisy data: bf16 vs float", flush=True)
     p, P, R, KFC, KFCc = _setup()
     xt = torch.from_numpy(p["x"]).cuda()
     y = P.bmShanna(xt).cpu().numpy()
     rng = np.random.default_rng(11)
     y = (y + 0.01 * np.abs(y).mean() * (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape))).astype(
         np.complex64)
-    ve = p["ve"].astype(np.float32)
-    yd, ved = torch.from_numpy(y).cuda(), torch.from_numpy(ve).cuda()
-    x32 = torch.zeros_like(xt)
-    P.bmSensa(x32, yd, ved, 20, nIter=1)
-    rho32 = P.sensa_refined(x32.clone(), yd, ved, nInner=1, nOuter=0)[0]
-    # the solver's residual of a given image is the oracle's float64 one
-    assert abs(_normal_residual(x32.cpu().numpy(), y, ve, p, R, KFC, KFCc) - rho32) < 0.02 * rho32
-    tol = max(1e-5, rho32)
-    x = torch.zeros_like(xt)
-    hist = P.sensa_refined(x, yd, ved, nInner=10, nOuter=40, tol=tol, bf16=bf16)
-    assert hist[0] == 1.0
-    assert hist[-1] < tol, (tol, hist)
-    assert hist.steps <= 40 * 10
-    assert _normal_residual(x.cpu().numpy(), y, ve, p, R, KFC, KFCc) < 1.05 * tol
+    yd, ved = torch.from_numpy(y).cuda(), torch.from_numpy(p["ve"].astype(np.float32)).cuda()
+    hs = []
+    for bf16 in (False, True):
+        x = torch.zeros_like(xt)
+        hs.append(P.sensa_refined(x, yd, ved, nInner=10, nOuter=10, tol=0.0, bf16=bf16))
+    h32, h16 = np.array(hs[0]), np.array(hs[1])
+    assert len(h32) == len(h16) == 11
+    assert np.all(np.abs(h16[1:] - h32[1:]) <= 0.05 * h32[1:]), (h32, h16)
 
 
 def test_sensa_refined_deterministic():

@@ -89,15 +89,23 @@ def test_long_rows_bit_exact_vs_port(n):
     assert np.array_equal(gi.view(np.int32), ei.view(np.int32))
 
 
-def test_multiblock_row_layout():
-    """cR/rR_nBlock_omp: same sums with the column cursor restarting at each
-    block (first entry of each block absolute), as cpp_prepare('multi_block')
-    writes it (bmSparseMat.m:156-170)."""
-    rng = np.random.default_rng(5)
-    M = OL.random_matrix(rng, 120, 90, 10)
+@pytest.mark.parametrize("squeeze", [False, True])
+def test_multiblock_row_layout(squeeze):
+    """cR/rR_nBlock_omp (bmSparseMat_cR_nBlock_omp_mex.cpp:97-483,
+    bmSparseMat_rR_nBlock_omp_mex.cpp:97-436) on a matrix encoded as
+    cpp_prepare('multi_block') writes it (bmSparseMat.m:131-165): the column
+    cursor restarts at every block (first r_jump of a block absolute) and,
+    for a squeezed matrix, so does the output-row cursor (first l_jump of a
+    block absolute; the reference's l_sparsity branch).  Every row is still
+    summed in stored order, so the products equal the one-block product bit
+    for bit; they also agree with the dense float64 product."""
+    print(f"[gpu] multi-block products squeeze={squeeze}", flush=True)
+    rng = np.random.default_rng(5 + squeeze)
+    M = OL.random_matrix(rng, 120, 90, 10, squeeze=squeeze)
     n = 2
-    bl = [37, 50, 33]
-    lbs = np.array([0, 37, 87], np.int32)
+    nl = M["l_nJump"]
+    bl = [nl // 3, nl // 2 - nl // 3, nl - nl // 2]
+    lbs = np.array([0, bl[0], bl[0] + bl[1]], np.int32)
     rowptr = np.concatenate([[0], np.cumsum(M["r_nJump"])])
     mbs = rowptr[lbs].astype(np.int64)
     col = np.cumsum(M["r_jump"].astype(np.int64))
@@ -105,17 +113,34 @@ def test_multiblock_row_layout():
     for m in mbs:
         if m < col.size:
             rj[m] = col[m]
+    lj = None
+    rows = np.arange(nl)
+    if squeeze:
+        rows = np.cumsum(M["l_jump"].astype(np.int64))
+        lj = M["l_jump"].copy()
+        lj[lbs] = rows[lbs]
+        assert not np.array_equal(lj, M["l_jump"])
     vr = rng.normal(size=(M["r_size"], n)).astype(np.float32)
     vi = rng.normal(size=(M["r_size"], n)).astype(np.float32)
     er, ei = OL.call_cC(OL.port().port_cC_oBlock_omp, M, np.ascontiguousarray(vr.T), np.ascontiguousarray(vi.T))
     L = lib()
+    blk = (3, OL.ip(np.array(bl, np.int32)), OL.ip(lbs), mbs.ctypes.data_as(C.POINTER(C.c_longlong)))
     wr = np.full((M["l_size"], n), np.nan, np.float32)
     wi = np.full((M["l_size"], n), np.nan, np.float32)
     L.bmSparseMat_cR_nBlock_omp(M["r_size"], OL.ip(rj), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]), M["l_size"],
-                                None, M["l_nJump"], 3, OL.ip(np.array(bl, np.int32)), OL.ip(lbs),
-                                mbs.ctypes.data_as(C.POINTER(C.c_longlong)), OL.fp(vr), OL.fp(vi), n,
-                                OL.fp(wr), OL.fp(wi))
-    assert np.array_equal(wr.T, er) and np.array_equal(wi.T, ei)
+                                OL.ip(lj), nl, *blk, OL.fp(vr), OL.fp(vi), n, OL.fp(wr), OL.fp(wi))
+    assert np.array_equal(wr.T.view(np.int32), er.view(np.int32))
+    assert np.array_equal(wi.T.view(np.int32), ei.view(np.int32))
+    rr = np.full((M["l_size"], n), np.nan, np.float32)
+    L.bmSparseMat_rR_nBlock_omp(M["r_size"], OL.ip(rj), OL.ip(M["r_nJump"]), OL.fp(M["m_val"]), M["l_size"],
+                                OL.ip(lj), nl, *blk, OL.fp(vr), n, OL.fp(rr))
+    assert np.array_equal(rr.T.view(np.int32), er.view(np.int32))
+    # dense float64 product of the same matrix
+    A = np.zeros((M["l_size"], M["r_size"]))
+    np.add.at(A, (np.repeat(rows, M["r_nJump"]), col), M["m_val"].astype(np.float64))
+    ref = A @ (vr.astype(np.float64) + 1j * vi.astype(np.float64))
+    got = wr.astype(np.float64) + 1j * wi.astype(np.float64)
+    assert np.linalg.norm(got - ref) <= 1e-6 * np.linalg.norm(ref)
 
 
 def test_r_nJump2index_gpu():
