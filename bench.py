@@ -530,7 +530,7 @@ def cpu_baseline(prob, threads):
               l_nJump=Gu.l_nJump, l_jump=None)
     Mt = dict(r_size=Gut.r_size, l_size=Gut.l_size, r_jump=Gut.r_jump, r_nJump=Gut.r_nJump,
               m_val=Gut.m_val, l_nJump=Gut.l_nJump, l_jump=Gut.l_jump)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    omp_env = os.environ.get("OMP_NUM_THREADS")
 
     def cfft_cpu(g, inverse):
         axes = (1, 2, 3)
@@ -538,35 +538,57 @@ def cpu_baseline(prob, threads):
         g = (sfft.ifftn if inverse else sfft.fftn)(g, axes=axes, workers=threads, overwrite_x=True)
         return sfft.fftshift(g, axes=axes)
 
-    # forward: x.*KFC -> zero-fill -> cfft -> Gu
-    t0 = time.perf_counter()
-    xc = (x[:, None] * KFC).T.reshape((nCh,) + tuple(n_u[::-1]))   # [c, z, y, x]
-    g = np.zeros((nCh,) + tuple(N_u[::-1]), np.complex64)
     sl = tuple(slice(N // 2 - n // 2, N // 2 - n // 2 + n) for N, n in zip(N_u[::-1], n_u[::-1]))
-    g[(slice(None),) + sl] = xc
-    g = cfft_cpu(g, False).astype(np.complex64)
-    gr = np.ascontiguousarray(g.real.reshape(nCh, -1))
-    gi = np.ascontiguousarray(g.imag.reshape(nCh, -1))
-    del g
-    yr, yi = OL.call_cC(fn, Mu, gr, gi)
-    t_fwd = time.perf_counter() - t0
-    del gr, gi
-    # adjoint: Gut -> cifft -> crop -> .*KFC_conj -> coil sum
-    t0 = time.perf_counter()
-    wr, wi = OL.call_cC(fn, Mt, yr, yi)
-    g = (wr + 1j * wi).astype(np.complex64).reshape((nCh,) + tuple(N_u[::-1]))
-    del wr, wi
-    g = cfft_cpu(g, True)
-    g = g[(slice(None),) + sl].reshape(nCh, -1).T
-    xa = (g * KFCc).sum(1)
-    t_adj = time.perf_counter() - t0
+
+    def forward():
+        # x.*KFC -> zero-fill -> cfft -> Gu
+        xc = (x[:, None] * KFC).T.reshape((nCh,) + tuple(n_u[::-1]))   # [c, z, y, x]
+        g = np.zeros((nCh,) + tuple(N_u[::-1]), np.complex64)
+        g[(slice(None),) + sl] = xc
+        g = cfft_cpu(g, False).astype(np.complex64)
+        gr = np.ascontiguousarray(g.real.reshape(nCh, -1))
+        gi = np.ascontiguousarray(g.imag.reshape(nCh, -1))
+        del g
+        return OL.call_cC(fn, Mu, gr, gi)
+
+    def adjoint(yr, yi):
+        # Gut -> cifft -> crop -> .*KFC_conj -> coil sum
+        wr, wi = OL.call_cC(fn, Mt, yr, yi)
+        g = (wr + 1j * wi).astype(np.complex64).reshape((nCh,) + tuple(N_u[::-1]))
+        del wr, wi
+        g = cfft_cpu(g, True)
+        g = g[(slice(None),) + sl].reshape(nCh, -1).T
+        return (g * KFCc).sum(1)
+
+    def timed(fn_, *a):
+        t0 = time.perf_counter()
+        r = fn_(*a)
+        return time.perf_counter() - t0, r
+
+    # one warm-up of each, then the median of 5 timed runs (monotonic clock)
+    _, (yr, yi) = timed(forward)
+    timed(adjoint, yr, yi)
+    tf = [timed(forward)[0] for _ in range(5)]
+    ta = [timed(adjoint, yr, yi)[0] for _ in range(5)]
+    t_fwd, t_adj = float(np.median(tf)), float(np.median(ta))
     recon_s = 22 * t_fwd + 20 * t_adj
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(value=1.0 / recon_s, unit="recons/s", cores=threads, kind=kind,
-                sample=(f"one forward ({t_fwd:.2f} s) + one adjoint ({t_adj:.2f} s) at the full workload, "
-                        f"scaled to a 20-iteration CG-SENSE (22 forward + 20 adjoint, vector updates "
-                        f"not counted); sparse products: {'the reference kernel built from its sources' if kind == 'reference' else 'the oracle C restatement'} "
+                sample=(f"forward and adjoint at the full workload, one warm-up each then the median of 5 "
+                        f"(forward {t_fwd:.2f} s, adjoint {t_adj:.2f} s), scaled to a 20-iteration CG-SENSE "
+                        f"(22 forward + 20 adjoint, vector updates not counted); sparse products: "
+                        f"{'the reference kernel built from its sources' if kind == 'reference' else 'the oracle C restatement'} "
                         f"(OpenMP, min(2*nCh, threads) threads), FFT: scipy.fft with {threads} workers"),
-                nufft_pairs_per_s=1.0 / (t_fwd + t_adj))
+                nufft_pairs_per_s=1.0 / (t_fwd + t_adj),
+                forward_s=tf, adjoint_s=ta, nproc=os.cpu_count(), cpu_model=model,
+                omp_num_threads=omp_env, fft_workers=threads)
 
 
 def main(argv=None):

@@ -9,6 +9,7 @@
 // (src/sparseMat/cpp/bmSparseMat_cC_oBlock_omp.cpp:99-110), hence
 // bit-identical results with no atomics.
 #include <algorithm>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -871,6 +872,41 @@ struct HostProduct {
     const long long* m_block_start = nullptr;
 };
 
+// Decoded device copies of the matrices the drop-in entry points saw last.
+// A caller that applies the same bmSparseMat again (a CG loop calling
+// bmSparseMat_vec with the same Gu and Gut) skips the upload and decode of
+// the matrix.  The key is the caller's host pointers, the sizes and a 64-bit
+// hash of every matrix array (values included), recomputed on every call,
+// so a matrix edited in place is decoded again.
+uint64_t hash_words(const void* p, int64_t n_words, uint64_t seed) {
+    if (!p || n_words <= 0) return seed;
+    const uint32_t* w = static_cast<const uint32_t*>(p);
+    uint64_t acc = 0;
+#pragma omp parallel for reduction(+ : acc) schedule(static)
+    for (int64_t i = 0; i < n_words; i++) {
+        uint64_t z = ((uint64_t)w[i] << 32 | (uint64_t)(uint32_t)i) ^ seed;
+        z += (uint64_t)i * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        acc += z ^ (z >> 31);
+    }
+    return acc * 0x2545F4914F6CDD1Dull + (uint64_t)n_words + seed * 31;
+}
+
+struct CachedMat {
+    const void* ptr[7];
+    int r_size = 0, l_size = 0, l_nJump = 0, nBlock = 0;
+    int64_t nnz = 0;
+    uint64_t hash = 0, last_use = 0;
+    std::unique_ptr<SparseDev> S;
+};
+constexpr int kMatCacheEntries = 4;
+
+std::vector<CachedMat>& mat_cache() {
+    static std::vector<CachedMat> c;
+    return c;
+}
+
 void host_product(const HostProduct& a, Layout lay, const float* v0, const float* v1, float* w0,
                   float* w1) {
     HostCallCtx& C = host_ctx();
@@ -898,64 +934,108 @@ void host_product(const HostProduct& a, Layout lay, const float* v0, const float
     C.slot[5].ensure(std::max<int64_t>(vlen, 1) * sizeof(float));
     C.slot[6].ensure(std::max<int64_t>(wlen, 1) * sizeof(float));
     C.slot[7].ensure(std::max<int64_t>(wlen, 1) * sizeof(float));
-    MONA_HIP(hipMemcpyAsync(C.slot[0].p, a.r_jump, nnz * sizeof(int), hipMemcpyHostToDevice, s));
-    MONA_HIP(hipMemcpyAsync(C.slot[1].p, a.r_nJump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
-    MONA_HIP(hipMemcpyAsync(C.slot[2].p, a.m_val, nnz * sizeof(float), hipMemcpyHostToDevice, s));
-    // Multi-block squeezed matrices (cpp_prepare('multi_block'),
-    // bmSparseMat.m:131-165): the first l_jump of every block, at
-    // l_block_start[b], is the absolute 0-based output row.  Re-encoded here
-    // as one running-delta sequence so the one-block row decode applies.
-    std::vector<int> lj_one;
-    if (squeezed && a.nBlock > 1) {
-        MONA_CHECK(a.l_block_start, MONA_ERR_ARG, "NULL l_block_start");
-        lj_one.resize(a.l_nJump);
-        int64_t row = 0, prev = 0;
-        int b = 0;
-        for (int i = 0; i < a.l_nJump; i++) {
-            if (b < a.nBlock && i == a.l_block_start[b]) {
-                row = a.l_jump[i];
-                b++;
-            } else {
-                row += a.l_jump[i];
-            }
-            const int64_t d = i == 0 ? row : row - prev;
-            MONA_CHECK(row >= 0 && row < a.l_size && d >= INT32_MIN && d <= INT32_MAX, MONA_ERR_ARG,
-                       "l_jump decodes to a row outside [0, l_size)");
-            lj_one[i] = (int)d;
-            prev = row;
-        }
-    }
-    if (squeezed)
-        MONA_HIP(hipMemcpyAsync(C.slot[3].p, lj_one.empty() ? a.l_jump : lj_one.data(), (size_t)a.l_nJump * sizeof(int),
-                                hipMemcpyHostToDevice, s));
+    // the vectors
     MONA_HIP(hipMemcpyAsync(C.slot[4].p, v0, vlen * sizeof(float), hipMemcpyHostToDevice, s));
     if (v1) MONA_HIP(hipMemcpyAsync(C.slot[5].p, v1, vlen * sizeof(float), hipMemcpyHostToDevice, s));
 
-    SparseDev S;
-    S.r_size = a.r_size; S.l_size = a.l_size; S.l_nJump = a.l_nJump; S.nnz = nnz;
-    S.squeezed = squeezed;
-    DevBuf mstart;
+    // the matrix: a cached decoded copy, or upload + decode
+    uint64_t h = 0x6A09E667F3BCC908ull;
+    h = hash_words(a.r_jump, nnz, h);
+    h = hash_words(a.r_nJump, a.l_nJump, h);
+    h = hash_words(a.m_val, nnz, h);
+    if (squeezed) h = hash_words(a.l_jump, a.l_nJump, h);
     if (a.nBlock > 1) {
-        MONA_CHECK(a.m_block_start && a.l_block_start && a.block_length, MONA_ERR_ARG,
-                   "NULL block arrays");
-        // block starts must tile the stored rows in order
-        int64_t msum = 0;
-        int lsum = 0;
-        for (int b = 0; b < a.nBlock; b++) {
-            MONA_CHECK(a.l_block_start[b] == lsum && a.m_block_start[b] == msum, MONA_ERR_ARG,
-                       "block partition does not match r_nJump");
-            for (int i = 0; i < a.block_length[b]; i++) {
-                MONA_CHECK(lsum < a.l_nJump, MONA_ERR_ARG, "block lengths exceed l_nJump");
-                msum += a.r_nJump[lsum++];
+        h = hash_words(a.l_block_start, a.l_block_start ? a.nBlock : 0, h);
+        h = hash_words(a.block_length, a.block_length ? a.nBlock : 0, h);
+        h = hash_words(a.m_block_start, a.m_block_start ? 2 * (int64_t)a.nBlock : 0, h);
+    }
+    const void* key[7] = {a.r_jump, a.r_nJump, a.m_val, a.l_jump, a.l_block_start, a.block_length,
+                          a.m_block_start};
+    static uint64_t tick = 0;
+    std::vector<CachedMat>& cache = mat_cache();
+    const SparseDev* Sp = nullptr;
+    for (CachedMat& e : cache) {
+        if (e.hash == h && e.nnz == nnz && e.r_size == a.r_size && e.l_size == a.l_size && e.l_nJump == a.l_nJump &&
+            e.nBlock == a.nBlock && std::equal(key, key + 7, e.ptr)) {
+            e.last_use = ++tick;
+            Sp = e.S.get();
+            break;
+        }
+    }
+    if (!Sp) {
+        MONA_HIP(hipMemcpyAsync(C.slot[0].p, a.r_jump, nnz * sizeof(int), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(C.slot[1].p, a.r_nJump, (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(C.slot[2].p, a.m_val, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+        // Multi-block squeezed matrices (cpp_prepare('multi_block'),
+        // bmSparseMat.m:131-165): the first l_jump of every block, at
+        // l_block_start[b], is the absolute 0-based output row.  Re-encoded
+        // here as one running-delta sequence so the one-block row decode
+        // applies.
+        std::vector<int> lj_one;
+        if (squeezed && a.nBlock > 1) {
+            MONA_CHECK(a.l_block_start, MONA_ERR_ARG, "NULL l_block_start");
+            lj_one.resize(a.l_nJump);
+            int64_t row = 0, prev = 0;
+            int b = 0;
+            for (int i = 0; i < a.l_nJump; i++) {
+                if (b < a.nBlock && i == a.l_block_start[b]) {
+                    row = a.l_jump[i];
+                    b++;
+                } else {
+                    row += a.l_jump[i];
+                }
+                const int64_t d = i == 0 ? row : row - prev;
+                MONA_CHECK(row >= 0 && row < a.l_size && d >= INT32_MIN && d <= INT32_MAX, MONA_ERR_ARG,
+                           "l_jump decodes to a row outside [0, l_size)");
+                lj_one[i] = (int)d;
+                prev = row;
             }
         }
-        MONA_CHECK(lsum == a.l_nJump, MONA_ERR_ARG, "block lengths do not cover l_nJump");
-        mstart.alloc((size_t)a.nBlock * sizeof(int64_t));
-        std::vector<int64_t> ms(a.m_block_start, a.m_block_start + a.nBlock);
-        MONA_HIP(hipMemcpyAsync(mstart.p, ms.data(), ms.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        if (squeezed)
+            MONA_HIP(hipMemcpyAsync(C.slot[3].p, lj_one.empty() ? a.l_jump : lj_one.data(),
+                                    (size_t)a.l_nJump * sizeof(int), hipMemcpyHostToDevice, s));
+        auto S = std::make_unique<SparseDev>();
+        S->r_size = a.r_size; S->l_size = a.l_size; S->l_nJump = a.l_nJump; S->nnz = nnz;
+        S->squeezed = squeezed;
+        DevBuf mstart;
+        if (a.nBlock > 1) {
+            MONA_CHECK(a.m_block_start && a.l_block_start && a.block_length, MONA_ERR_ARG,
+                       "NULL block arrays");
+            // block starts must tile the stored rows in order
+            int64_t msum = 0;
+            int lsum = 0;
+            for (int b = 0; b < a.nBlock; b++) {
+                MONA_CHECK(a.l_block_start[b] == lsum && a.m_block_start[b] == msum, MONA_ERR_ARG,
+                           "block partition does not match r_nJump");
+                for (int i = 0; i < a.block_length[b]; i++) {
+                    MONA_CHECK(lsum < a.l_nJump, MONA_ERR_ARG, "block lengths exceed l_nJump");
+                    msum += a.r_nJump[lsum++];
+                }
+            }
+            MONA_CHECK(lsum == a.l_nJump, MONA_ERR_ARG, "block lengths do not cover l_nJump");
+            mstart.alloc((size_t)a.nBlock * sizeof(int64_t));
+            std::vector<int64_t> ms(a.m_block_start, a.m_block_start + a.nBlock);
+            MONA_HIP(hipMemcpyAsync(mstart.p, ms.data(), ms.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        }
+        decode_device(*S, C.slot[0].as<int>(), C.slot[1].as<int>(), C.slot[2].as<float>(),
+                      squeezed ? C.slot[3].as<int>() : nullptr, a.nBlock, mstart.as<int64_t>(), s);
+        // keep it; the least recently used entry makes room
+        if ((int)cache.size() >= kMatCacheEntries) {
+            auto lru = std::min_element(cache.begin(), cache.end(), [](const CachedMat& x, const CachedMat& y) {
+                return x.last_use < y.last_use;
+            });
+            cache.erase(lru);
+        }
+        CachedMat e;
+        std::copy(key, key + 7, e.ptr);
+        e.r_size = a.r_size; e.l_size = a.l_size; e.l_nJump = a.l_nJump; e.nBlock = a.nBlock; e.nnz = nnz;
+        e.hash = h;
+        e.last_use = ++tick;
+        e.S = std::move(S);
+        Sp = e.S.get();
+        cache.push_back(std::move(e));
     }
-    decode_device(S, C.slot[0].as<int>(), C.slot[1].as<int>(), C.slot[2].as<float>(),
-                  squeezed ? C.slot[3].as<int>() : nullptr, a.nBlock, mstart.as<int64_t>(), s);
+    const SparseDev& S = *Sp;
 
     float* dw0 = C.slot[6].as<float>();
     float* dw1 = C.slot[7].as<float>();

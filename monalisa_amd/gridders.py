@@ -183,11 +183,16 @@ def bmMathilda(y, t, v, Cmaps=None, N_u=None, n_u=None, dK_u=None, kernelType=No
         cm = Cmaps if Cmaps is not None else np.ones((int(np.prod(n_u)), nCh), np.complex64)
         cm = torch.as_tensor(np.asarray(cm, dtype=np.complex64) if not torch.is_tensor(cm) else cm)
         plan.set_coils(cm.to("cuda", torch.complex64).contiguous())
-    g = torch.from_numpy(np.ascontiguousarray(grid, dtype=np.complex64)).cuda()
     nu = int(np.prod(n_u))
+    if plan.nCh != nCh or plan.nu != nu or plan.Nu != int(np.prod(N_u)):
+        raise ValueError(f"plan geometry (nCh {plan.nCh}, prod(n_u) {plan.nu}, prod(N_u) {plan.Nu}) does not match "
+                         f"the data (nCh {nCh}, prod(n_u) {nu}, prod(N_u) {int(np.prod(N_u))})")
     cm_dev = None
     if Cmaps is not None:
         cm_dev = torch.as_tensor(Cmaps).to("cuda", torch.complex64).contiguous()
+        if cm_dev.numel() != nu * nCh:
+            raise ValueError(f"Cmaps must have prod(n_u) * nCh = {nu * nCh} elements, got {cm_dev.numel()}")
+    g = torch.from_numpy(np.ascontiguousarray(grid, dtype=np.complex64)).cuda()
     out = torch.empty((nu,) if cm_dev is not None else (nu, nCh), dtype=torch.complex64, device="cuda")
     check(lib.mona_mathilda(plan._h, ptr(g), ptr(cm_dev), ptr(out), stream_of(stream)))
     return out

@@ -28,6 +28,22 @@ def _require_cuda(t, name):
         raise ValueError(f"{name} must be contiguous")
 
 
+def _arg(t, name, numel, real=False):
+    """Refuses, before any library call, a tensor the kernels would misread
+    through its raw pointer: a dtype other than complex64 (float32 for
+    real=True, e.g. volume elements), an element count other than numel, a
+    non-contiguous layout, or host memory.  numel None skips the count."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    want = torch.float32 if real else torch.complex64
+    if t.dtype != want:
+        raise ValueError(f"{name} must be {want}, got {t.dtype}")
+    if numel is not None and t.numel() != int(numel):
+        raise ValueError(f"{name} must have {int(numel)} elements, got {t.numel()}")
+    _require_cuda(t, name)
+
+
 class Plan:
     """Geometry + gridding kernel + coil maps + frames (Gu, Gut pairs)."""
 
@@ -59,7 +75,7 @@ class Plan:
             self._h = None
 
     def set_coils(self, Cmaps, stream=None):
-        _require_cuda(Cmaps, "C")
+        _arg(Cmaps, "C", self.nu * self.nCh)
         check(lib.mona_plan_set_coils(self._h, ptr(Cmaps), stream_of(stream)))
 
     def set_frame(self, f, Gu: BmSparseMat, Gut: BmSparseMat):
@@ -86,7 +102,16 @@ class Plan:
             self.set_deform(f, Tu[f], Tut[f], set=set)
 
     def nPt(self, f=0):
+        if f not in self._frames:
+            raise ValueError(f"frame {f} has no gridding matrices (set_frame)")
         return self._frames[f][0].l_size
+
+    def _check_frames(self, nFr, y_list, ve_list):
+        if len(y_list) != nFr or len(ve_list) != nFr:
+            raise ValueError(f"{nFr} frames need {nFr} data and weight vectors, got {len(y_list)} and {len(ve_list)}")
+        for f in range(nFr):
+            _arg(y_list[f], f"y[{f}]", self.nPt(f) * self.nCh)
+            _arg(ve_list[f], f"ve[{f}]", self.nPt(f), real=True)
 
     # -- execution options (no reference counterpart) ----------------------
     OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE, OPT_FFT_LINES = 1, 2, 3, 4
@@ -128,19 +153,21 @@ class Plan:
     def bmShanna(self, x, f=0, out=None, stream=None):
         """y = Gu * cfft(zerofill(x .* KFC))  (bmShanna_MATLAB.m:6-56)."""
         import torch
-        _require_cuda(x, "x")
+        _arg(x, "x", self.nu)
         if out is None:
             out = torch.empty((self.nPt(f), self.nCh), dtype=torch.complex64, device=x.device)
+        _arg(out, "out", self.nPt(f) * self.nCh)
         check(lib.mona_shanna(self._h, int(f), ptr(x), ptr(out), stream_of(stream)))
         return out
 
     def bmNakatsha(self, y, f=0, C_flag=True, out=None, stream=None):
         """x = sum_c KFC_conj .* crop(cifft(Gut * y))  (bmNakatsha_MATLAB.m:6-60)."""
         import torch
-        _require_cuda(y, "y")
+        _arg(y, "y", self.nPt(f) * self.nCh)
+        shape = (self.nu,) if C_flag else (self.nu, self.nCh)
         if out is None:
-            shape = (self.nu,) if C_flag else (self.nu, self.nCh)
             out = torch.empty(shape, dtype=torch.complex64, device=y.device)
+        _arg(out, "out", int(np.prod(shape)))
         check(lib.mona_nakatsha(self._h, int(f), ptr(y), ptr(out), 1 if C_flag else 0, stream_of(stream)))
         return out
 
@@ -149,9 +176,9 @@ class Plan:
         """CG-SENSE in place on x (bmSensa.m:6-96); returns the per-outer-
         iteration residuals <y - Mx, HY (y - Mx)>."""
         nIter = n_iter(nIter)
-        _require_cuda(x, "x")
-        _require_cuda(y, "y")
-        _require_cuda(ve, "ve")
+        _arg(x, "x", self.nu)
+        _arg(y, "y", self.nPt(f) * self.nCh)
+        _arg(ve, "ve", self.nPt(f), real=True)
         res = (C.c_double * max(nIter, 1))()
         check(lib.mona_sensa(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
                              float(ve_max) if ve_max else 0.0, int(nIter), int(nCGD), res,
@@ -165,10 +192,16 @@ class Plan:
         frame's result equals bmSensa on that frame alone."""
         nIter = n_iter(nIter)
         import torch
-        _require_cuda(x, "x")
-        nFr = x.shape[0]
+        nFr = x.shape[0] if isinstance(x, torch.Tensor) and x.dim() > 0 else 0
+        if nFr < 1 or any(f not in self._frames for f in range(nFr)):
+            raise ValueError(f"x must be [nFr, nu] with nFr frames set (got {tuple(getattr(x, 'shape', ()))})")
+        _arg(x, "x", nFr * self.nu)
+        if not isinstance(y_list, torch.Tensor):
+            self._check_frames(nFr, y_list, ve_list)
         ycat = y_list if isinstance(y_list, torch.Tensor) else torch.cat([v.reshape(-1) for v in y_list])
         vcat = ve_list if isinstance(ve_list, torch.Tensor) else torch.cat([v.reshape(-1) for v in ve_list])
+        _arg(ycat, "y", sum(self.nPt(f) for f in range(nFr)) * self.nCh)
+        _arg(vcat, "ve", sum(self.nPt(f) for f in range(nFr)), real=True)
         yo = np.cumsum([0] + [self.nPt(f) * self.nCh for f in range(nFr)])[:-1]
         vo = np.cumsum([0] + [self.nPt(f) for f in range(nFr)])[:-1]
         yoff = (C.c_longlong * nFr)(*[int(v) for v in yo])
@@ -197,9 +230,9 @@ class Plan:
         nOuter periodic replacements.  x in place (any start; nOuter = 0 only
         evaluates the relative residual of x); returns the relative residual
         at the start and after every replacement."""
-        _require_cuda(x, "x")
-        _require_cuda(y, "y")
-        _require_cuda(ve, "ve")
+        _arg(x, "x", self.nu)
+        _arg(y, "y", self.nPt(f) * self.nCh)
+        _arg(ve, "ve", self.nPt(f), real=True)
         rel = (C.c_double * (2 * nOuter + 1))()
         n_outer = C.c_int()
         n_steps = C.c_int()
@@ -211,7 +244,7 @@ class Plan:
     def bmShanna_bf16(self, x, f=0, stream=None):
         """bmShanna with bf16 storage of the grid-side arrays and k-space."""
         import torch
-        _require_cuda(x, "x")
+        _arg(x, "x", self.nu)
         out = torch.empty((self.nPt(f), self.nCh), dtype=torch.complex64, device=x.device)
         check(lib.mona_shanna_bf16(self._h, int(f), ptr(x), ptr(out), stream_of(stream)))
         return out
@@ -220,7 +253,7 @@ class Plan:
         """bmNakatsha (coil-combined) with bf16 storage of the grid-side
         arrays and k-space."""
         import torch
-        _require_cuda(y, "y")
+        _arg(y, "y", self.nPt(f) * self.nCh)
         out = torch.empty((self.nu,), dtype=torch.complex64, device=y.device)
         check(lib.mona_nakatsha_bf16(self._h, int(f), ptr(y), ptr(out), stream_of(stream)))
         return out
@@ -230,11 +263,15 @@ class Plan:
         [imDim, nu] complex64 (given) or None (z = grad x, u = 0)."""
         nIter = n_iter(nIter)
         import torch
-        _require_cuda(x, "x")
+        _arg(x, "x", self.nu)
+        _arg(y, "y", self.nPt(f) * self.nCh)
+        _arg(ve, "ve", self.nPt(f), real=True)
         given = z is not None
         if z is None:
             z = torch.zeros((self.imDim, self.nu), dtype=torch.complex64, device=x.device)
             u = torch.zeros_like(z)
+        _arg(z, "z", self.imDim * self.nu)
+        _arg(u, "u", self.imDim * self.nu)
         d = (C.c_float * 2)(*_sched(delta))
         r = (C.c_float * 2)(*_sched(rho))
         hist = (C.c_double * (2 * max(nIter, 1)))()
@@ -254,11 +291,15 @@ class Plan:
         nIter = n_iter(nIter)
         import torch
         nFr = x.shape[0]
+        _arg(x, "x", nFr * self.nu)
+        self._check_frames(nFr, y_list, ve_list)
         self._set_deforms(nFr, 0, Tu, Tut)
         given = z is not None
         if z is None:
             z = torch.zeros_like(x)
             u = torch.zeros_like(x)
+        _arg(z, "z", nFr * self.nu)
+        _arg(u, "u", nFr * self.nu)
         ycat = torch.cat([yy.reshape(-1) for yy in y_list])
         vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
         yo = np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]
@@ -282,6 +323,8 @@ class Plan:
         nIter = n_iter(nIter)
         import torch
         nFr = x.shape[0]
+        _arg(x, "x", nFr * self.nu)
+        self._check_frames(nFr, y_list, ve_list)
         if nFr1 is None:
             nFr1 = nFr
         l1 = rho is not None
@@ -292,6 +335,8 @@ class Plan:
             z = torch.stack(list(zs)).contiguous() if given else torch.zeros((nT,) + tuple(x.shape), dtype=x.dtype,
                                                                             device=x.device)
             u = torch.stack(list(us)).contiguous() if given else torch.zeros_like(z)
+            _arg(z, "z", nT * nFr * self.nu)
+            _arg(u, "u", nT * nFr * self.nu)
         ycat = torch.cat([yy.reshape(-1) for yy in y_list])
         vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
         yo = np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]
@@ -372,6 +417,8 @@ class Plan:
         nIter = n_iter(nIter)
         import torch
         nFr = len(y_list)
+        _arg(x, "x", self.nu)
+        self._check_frames(nFr, y_list, ve_list)
         self._set_deforms(nFr, 0, Tu, Tut)
         ycat = torch.cat([yy.reshape(-1) for yy in y_list])
         vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
@@ -385,6 +432,9 @@ class Plan:
     def bmNasha(self, y, Gn: BmSparseMat, Cmaps=None, stream=None):
         """Gridded recon (bmNasha.m:6-91)."""
         import torch
+        _arg(y, "y", Gn.r_size * self.nCh)
+        if Cmaps is not None:
+            _arg(Cmaps, "C", self.nu * self.nCh)
         shape = (self.nu,) if Cmaps is not None else (self.nu, self.nCh)
         out = torch.empty(shape, dtype=torch.complex64, device=y.device)
         check(lib.mona_nasha(self._h, Gn.handle(), ptr(y), ptr(Cmaps), ptr(out), stream_of(stream)))
@@ -410,8 +460,10 @@ def _sched(v):
 def cfft(data, N, inverse=False, stream=None):
     """Centred n-D DFT in place on a [prod(N), nBatch] complex64 device
     tensor (fftshift(fft(ifftshift(.))) per axis; inverse scaled 1/N)."""
-    _require_cuda(data, "data")
     N = [int(v) for v in np.ravel(N)]
+    if not hasattr(data, "numel") or data.numel() == 0 or data.numel() % int(np.prod(N)):
+        raise ValueError(f"data must hold a whole number of {int(np.prod(N))}-element grids")
+    _arg(data, "data", None)
     nb = data.numel() // int(np.prod(N))
     Na = (C.c_int * len(N))(*N)
     check(lib.mona_cfft(len(N), Na, nb, ptr(data), 1 if inverse else 0, stream_of(stream)))

@@ -179,13 +179,27 @@ class ReconServer:
             self.completed.append(r)
             r.done.set()
             return None
-        B = min(cap, len(q))
+        # a batch runs one CG step count: the head's nCGD, same-nCGD
+        # requests of the protocol in queue order (the others stay queued)
+        B = min(cap, sum(1 for r in q if r.nCGD == q[0].nCGD))
         plan = self._idle_plans[key][-1] if self._idle_plans[key] else None
         need = (0 if plan is not None else self.backend.plan_bytes_estimate(proto, cap)) + \
             B * self.backend.request_bytes(proto)
         if self._evict_for(need, keep=plan):
             return key, B
         return None
+
+    @staticmethod
+    def _take(q, B):
+        """The first B requests of q with the head's nCGD, in queue order;
+        the others keep their places."""
+        n = q[0].nCGD
+        out, keep = [], []
+        while q and len(out) < B:
+            r = q.popleft()
+            (out if r.nCGD == n else keep).append(r)
+        q.extendleft(reversed(keep))
+        return out
 
     def _dispatch(self):
         with self._cv:
@@ -196,7 +210,7 @@ class ReconServer:
                     if nb is not None:
                         key, B = nb
                         proto = self.protocols[key]
-                        reqs = [self._queues[key].popleft() for _ in range(B)]
+                        reqs = self._take(self._queues[key], B)
                         plan = self._idle_plans[key].pop() if self._idle_plans[key] else None
                         if plan is not None:
                             pb = self._plan_bytes[id(plan)]
@@ -363,6 +377,14 @@ class GpuServer(ReconServer):
 
     def submit(self, key: str, y, nCGD: int = 10, ready=None) -> Request:
         import torch
+        from .nufft import _arg
+        if key not in self.protocols:
+            raise KeyError(f"unknown protocol {key!r}")
+        pr = self.protocols[key]
+        # a mis-sized y would shift every later request of a batch
+        _arg(y, "y", pr.ve.numel() * pr.nCh)
+        if int(nCGD) < 1:
+            raise ValueError("nCGD must be >= 1")
         if ready is None:
             ready = torch.cuda.Event()
             ready.record(torch.cuda.current_stream())

@@ -83,15 +83,16 @@ def test_served_results_equal_alone():
     print("[gpu] serving mix", flush=True)
     protos = _protocols()
     reqs = _requests(protos, 16)
-    nCGD = 5
-    alone = [_alone(protos, k, y, nCGD) for k, y in reqs]
+    # mixed CG step counts within each protocol: a batch never mixes them
+    nCGDs = [5 if i % 4 else 3 for i in range(len(reqs))]
+    alone = [_alone(protos, k, y, n) for (k, y), n in zip(reqs, nCGDs)]
     srv = GpuServer(hbm_budget=8 << 30, max_batch=3, workers=3)
     for key, (_, pr) in protos.items():
         srv.register(key, pr)
     srv.start()
     handles = []
     for i, (k, y) in enumerate(reqs):
-        handles.append(srv.submit(k, y, nCGD))
+        handles.append(srv.submit(k, y, nCGDs[i]))
         if i % 4 == 3:
             time.sleep(0.002)
     assert srv.drain(timeout=120)

@@ -170,3 +170,35 @@ def test_sparse_apply_device_and_determinism():
     er, ei = OL.call_cC(OL.port().port_cC_oBlock_omp, M, np.ascontiguousarray(v.real.T),
                         np.ascontiguousarray(v.imag.T))
     assert np.array_equal(a.real.T, er) and np.array_equal(a.imag.T, ei)
+
+
+def test_dropin_matrix_cache_follows_in_place_edits():
+    """The drop-in products keep the decoded device copy of a matrix between
+    calls (keyed by its host arrays and a hash of their contents): repeated
+    calls give identical results, and a value or structure edited in place
+    in the same host arrays is decoded again (each result bit-exact against
+    the C restatement)."""
+    print("[gpu] drop-in matrix cache", flush=True)
+    rng = np.random.default_rng(17)
+    M = OL.random_matrix(rng, 400, 300, 20, squeeze=True)
+    vr = rng.normal(size=(3, M["r_size"])).astype(np.float32)
+    vi = rng.normal(size=(3, M["r_size"])).astype(np.float32)
+    P = OL.port()
+    L = lib()
+    for step in range(4):
+        er, ei = OL.call_cC(P.port_cC_oBlock_omp, M, vr, vi)
+        for _ in range(2):
+            gr, gi = gpu_cC(L.bmSparseMat_cC_oBlock_omp, M, vr, vi)
+            assert np.array_equal(gr.view(np.int32), er.view(np.int32)), step
+            assert np.array_equal(gi.view(np.int32), ei.view(np.int32)), step
+        if step == 0:
+            M["m_val"][17] *= -3.0                      # a value, same arrays
+        elif step == 1:
+            c = np.cumsum(M["r_jump"].astype(np.int64))
+            k = int(np.argmax(c < M["r_size"] - 1))     # an entry whose column can move right by one
+            M["r_jump"][k] += 1
+            M["r_jump"][k + 1] -= 1
+        elif step == 2:
+            k = int(np.argmax(M["l_jump"][1:] >= 2))   # an output row that can move down by one
+            M["l_jump"][k] += 1
+            M["l_jump"][k + 1] -= 1

@@ -145,3 +145,32 @@ def test_request_larger_than_budget_fails_instead_of_blocking():
     s.stop()
     assert isinstance(r_big.error, MemoryError)
     assert r_small.error is None and r_small.x[0] == 2
+
+
+def test_batches_never_mix_cg_step_counts():
+    """A batch runs one nCGD (the backend takes it from the batch), so
+    requests with different nCGD for one protocol go to different batches,
+    and each result is the one the request would get alone."""
+    be = FakeBackend()
+    seen = []
+    run0 = be.run
+
+    def run(plan, proto, reqs, stream):
+        seen.append([r.nCGD for r in reqs])
+        run0(plan, proto, reqs, stream)
+        for r in reqs:
+            r.x = r.x * r.nCGD          # the result depends on the request's own nCGD
+
+    be.run = run
+    s = ReconServer(be, 10_000, max_batch=4, workers=1)
+    s.register("small", {"plan": 100, "req": 1, "t": 0.002})
+    reqs = [s.submit("small", np.full(2, i), nCGD=(5 if i % 3 else 10)) for i in range(30)]
+    s.start()
+    assert s.drain(timeout=30)
+    s.stop()
+    assert all(len(set(b)) == 1 for b in seen), seen
+    assert max(len(b) for b in seen) > 1
+    for i, r in enumerate(reqs):
+        assert r.error is None
+        np.testing.assert_array_equal(r.x, np.full(2, 2 * i) * r.nCGD)
+    assert sorted(r.rid for r in s.completed) == list(range(30))
