@@ -400,6 +400,9 @@ int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream
 #define MONA_OPT_FFT_TILE 3     /* plain FFT pass tile: 2048, 4096 or 8192 complex values, 0 = default */
 #define MONA_OPT_FFT_LINES 4    /* register-resident FFT passes (N = 16 * N2, N2 <= 16): lines per tile
                                    on strided axes 16, 32 or 64, 0 = default, -1 = LDS kernel only */
+#define MONA_OPT_MATRIX_FREE 5  /* forward gridding computed from the trajectory that mona_traj2sparsemat[_device]
+                                   keeps with Gu (frames whose Gu has one and whose geometry and kernel match the
+                                   plan; nWin 3): 1 = on (default), 0 = the stored entries */
 int mona_plan_set_option(mona_plan P, int option, long long value);
 /* Autotuner: times forward + adjoint of one frame for each FFT pass variant
  * and keeps the fastest; *chosen (may be NULL) receives the register kernel's

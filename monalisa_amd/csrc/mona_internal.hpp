@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -160,6 +161,35 @@ struct SpEntry {
 };
 
 // Device-resident, decoded form of a cpp_prepared one-block bmSparseMat.
+// The trajectory a gridding matrix was built from (bmTraj2SparseMat
+// arguments), kept by the library's own builders so that operator plans can
+// grid without the stored entries.
+struct TrajInfo {
+    int imDim = 0, nPt = 0, ktype = 0, nWin = 0, nParam = 0;
+    int N[3] = {1, 1, 1};
+    double dK[3] = {1, 1, 1};
+    double kp[3] = {0, 0, 0};
+    std::vector<double> t;  // [nPt][imDim], physical k units
+};
+
+// Per-point data of the matrix-free gridding of a trajectory, in the
+// builder's float64 semantics (bmTraj2SparseMat.m:58-230): the base cell
+// (0-based, wrapped, nearest grid point for odd nWin), the offset of the
+// point from it (t - floor(t + 1/2), in [-1/2, 1/2)), and 1 / sum of the
+// point's nWin^imDim kernel weights taken without the kernel's constant
+// factor -- I0(alpha sqrt(1 - d^2/tau^2)) for Kaiser, exp(-d^2/(2 sigma^2))
+// for Gauss -- (its Gu row normaliser for those weights; 0 for a deleted
+// point).  Odd nWin only.
+struct TrajPoints {
+    std::vector<int> base;      // [nPt][3]
+    std::vector<double> rest;   // [nPt][3]
+    std::vector<double> inv_sum;
+};
+void traj_points(const TrajInfo& T, TrajPoints* out);
+std::shared_ptr<const TrajInfo> make_traj_info(int imDim, int nPt, const double* t, const int* N_u,
+                                               const double* dK_u, int kernelType, int nWin,
+                                               const double* kp, int nParam);
+
 struct SparseDev {
     int r_size = 0, l_size = 0, l_nJump = 0;
     int64_t nnz = 0;
@@ -185,6 +215,7 @@ struct SparseDev {
     std::vector<float> h_m_val;
     bool on_device = false;
     std::mutex mu;
+    std::shared_ptr<const TrajInfo> traj;  // set by the trajectory builders
 };
 
 // Validates a jump-encoded matrix like bmSparseMat.check and keeps the host

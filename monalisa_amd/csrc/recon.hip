@@ -451,6 +451,37 @@ void Plan::init(int imDim_, const int* N_, const int* n_, const double* dK_, int
         Nu *= N[d];
         nu *= n[d];
     }
+    // matrix-free gridding (nWin 3, 2-D / 3-D grids of at most 1024 cells
+    // an axis, a kernel whose series fits kMfTerms); MONA_MF=0 turns it off
+    mf_kernel_ok = nWin == 3 && (imDim == 2 || imDim == 3) && N[0] <= 1024 && N[1] <= 1024 && N[2] <= 1024 &&
+                   mf_kernel_params(ktype, nParam, kp, &mfk);
+    static const bool mf_env = [] {
+        const char* e = std::getenv("MONA_MF");
+        return !(e && std::atoi(e) == 0);
+    }();
+    use_mf = mf_env;
+}
+
+MfGeo Plan::mf_geo() const {
+    MfGeo g;
+    g.N0 = N[0]; g.N1 = N[1]; g.N2 = N[2];
+    if (imDim == 2) {
+        g.sy = N[0]; g.sz = 0;
+    } else if (grid_yz()) {
+        g.sy = N[0] * N[2]; g.sz = N[0];     // [N_y][N_z][N_x]
+    } else {
+        g.sy = N[0]; g.sz = N[0] * N[1];     // [N_z][N_y][N_x]
+    }
+    return g;
+}
+
+bool Plan::frame_mf(int f) const { return use_mf && frame(f).mf != nullptr; }
+
+bool Plan::batch_mf(int nA) const {
+    if (!use_mf || !mf_b.P) return false;
+    for (int f = 0; f < nA; f++)
+        if (!frame(f).mf) return false;
+    return true;
 }
 
 void Plan::ensure_grid() {
@@ -588,8 +619,31 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     const bool yzg = grid_yz();
     const int tx = (N[0] + 7) / 8, ty = (N[1] + 7) / 8, tz = (N[2] + 7) / 8;
     std::vector<int64_t> key(P);
+    // A Gu built from a trajectory of this plan's geometry and kernel grids
+    // matrix-free: the points are then ordered by the tile of their base
+    // cell and by its raster position inside the tile (neighbouring points
+    // gather the same cells), and each keeps its base cell, offsets and
+    // row normaliser.
+    TrajPoints tp;
+    bool mf = false;
+    if (mf_kernel_ok && Gu->traj) {
+        const TrajInfo& T = *Gu->traj;
+        mf = T.imDim == imDim && T.nPt == P && T.nWin == nWin && T.ktype == ktype_;
+        for (int d = 0; d < imDim && mf; d++)
+            mf = T.N[d] == N[d] && (double)(float)T.dK[d] == dK[d];
+        for (int i = 0; i < T.nParam && mf; i++) mf = (double)(float)T.kp[i] == kparam[i];
+        if (mf) traj_points(T, &tp);
+    }
     for (int64_t p = 0; p < P; p++) {
         if (rowptr[p + 1] == rowptr[p]) { key[p] = INT64_MAX; continue; }
+        if (mf) {
+            const int64_t x = tp.base[p * 3], y = tp.base[p * 3 + 1], z = imDim == 3 ? tp.base[p * 3 + 2] : 0;
+            const int64_t tile = yzg ? ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3)
+                                     : ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
+            const int64_t in = yzg ? ((y & 7) * 8 + (z & 7)) * 8 + (x & 7) : ((z & 7) * 8 + (y & 7)) * 8 + (x & 7);
+            key[p] = tile * 512 + in;
+            continue;
+        }
         int64_t cell = col[rowptr[p]];
         int64_t x = cell % N[0], y = (cell / N[0]) % N[1], z = cell / ((int64_t)N[0] * N[1]);
         key[p] = yzg ? ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3) : ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
@@ -772,6 +826,27 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     }
     auto dperm = std::make_shared<DevBuf>((size_t)std::max<int64_t>(P, 1) * sizeof(int));
     MONA_HIP(hipMemcpyAsync(dperm->p, perm.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice, s));
+    std::shared_ptr<MfPoints> dmf;
+    std::vector<uint32_t> hb;
+    std::vector<float4> hr;
+    if (mf) {
+        hb.assign(P, 0u);
+        hr.assign(P, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (int64_t s2 = 0; s2 < P; s2++) {
+            const int64_t p = perm[s2];
+            if (rowptr[p + 1] == rowptr[p]) continue;   // deleted point: zeros
+            const int* b = &tp.base[p * 3];
+            const double* r = &tp.rest[p * 3];
+            hb[s2] = (uint32_t)b[0] | ((uint32_t)b[1] << 10) | ((uint32_t)(imDim == 3 ? b[2] : 0) << 20);
+            hr[s2] = make_float4((float)r[0], (float)r[1], imDim == 3 ? (float)r[2] : 0.f, (float)tp.inv_sum[p]);
+        }
+        dmf = std::make_shared<MfPoints>();
+        dmf->P = (int)P;
+        dmf->base.alloc((size_t)std::max<int64_t>(P, 1) * sizeof(uint32_t));
+        dmf->rec.alloc((size_t)std::max<int64_t>(P, 1) * sizeof(float4));
+        MONA_HIP(hipMemcpyAsync(dmf->base.p, hb.data(), (size_t)P * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipMemcpyAsync(dmf->rec.p, hr.data(), (size_t)P * sizeof(float4), hipMemcpyHostToDevice, s));
+    }
     MONA_HIP(hipStreamSynchronize(s));
     if ((int)frames.size() <= f) frames.resize(f + 1);
     Frame& fr = frames[f];
@@ -787,6 +862,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     fr.ival = div;
     fr.ivaly = divy;
     fr.ivals = use_ival;
+    fr.mf = dmf;
     Gu_b.reset();  // the batched matrices are rebuilt on next use
     Gut_b.reset();
     koff_b.clear();
@@ -859,7 +935,10 @@ void Plan::forward(int f, const float2* x, float2* y, hipStream_t s) {
     }
     {
         TimedRegion t(0, s);
-        spmv_c2_rowmajor(*fr.Gu_s, gr, y, nCh, nullptr, s);
+        if (frame_mf(f))
+            u2n_mf(*fr.mf, nullptr, fr.mf->P, gr, false, Nu, y, false, nCh, imDim, mf_geo(), mfk, s);
+        else
+            spmv_c2_rowmajor(*fr.Gu_s, gr, y, nCh, nullptr, s);
     }
 }
 
@@ -1133,6 +1212,32 @@ void Plan::build_batch(hipStream_t s) {
             }
         }
     }
+    // matrix-free points of the frames back to back, with their frame
+    mf_b.P = 0;
+    mf_b.base.release();
+    mf_b.rec.release();
+    mf_b_frame.release();
+    bool all_mf = nFr <= 65535;
+    for (int f = 0; f < nFr; f++) all_mf = all_mf && frame(f).mf;
+    if (all_mf) {
+        mf_b.base.alloc((size_t)std::max<int64_t>(poff, 1) * sizeof(uint32_t));
+        mf_b.rec.alloc((size_t)std::max<int64_t>(poff, 1) * sizeof(float4));
+        mf_b_frame.alloc((size_t)std::max<int64_t>(poff, 1) * sizeof(uint16_t));
+        std::vector<uint16_t> fo((size_t)poff);
+        int64_t o = 0;
+        for (int f = 0; f < nFr; f++) {
+            const MfPoints& m = *frame(f).mf;
+            MONA_HIP(hipMemcpyAsync(mf_b.base.as<uint32_t>() + o, m.base.p, (size_t)m.P * sizeof(uint32_t),
+                                    hipMemcpyDeviceToDevice, s));
+            MONA_HIP(hipMemcpyAsync(mf_b.rec.as<float4>() + o, m.rec.p, (size_t)m.P * sizeof(float4),
+                                    hipMemcpyDeviceToDevice, s));
+            std::fill(fo.begin() + o, fo.begin() + o + m.P, (uint16_t)f);
+            o += m.P;
+        }
+        MONA_HIP(hipMemcpyAsync(mf_b_frame.p, fo.data(), (size_t)poff * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+        MONA_HIP(hipStreamSynchronize(s));
+        mf_b.P = (int)poff;
+    }
     auto gu = std::make_shared<SparseDev>();
     sparse_block_diag(*gu, us, ucol, urow, (int)(nFr * Nu), (int)poff, &view_u, s);
     auto gut = std::make_shared<SparseDev>();
@@ -1157,7 +1262,11 @@ void Plan::forward_batch(const float2* x, float2* y, hipStream_t s, int nA) {
     }
     {
         TimedRegion t(0, s);
-        spmv_c2_rowmajor(*Gu_b, grid_b.as<float2>(), y, nCh, nullptr, s, &view_u[nA]);
+        if (batch_mf(nA))
+            u2n_mf(mf_b, mf_b_frame.as<uint16_t>(), (int)(koff_b[nA] / nCh), grid_b.as<float2>(), false, Nu, y,
+                   false, nCh, imDim, mf_geo(), mfk, s);
+        else
+            spmv_c2_rowmajor(*Gu_b, grid_b.as<float2>(), y, nCh, nullptr, s, &view_u[nA]);
     }
 }
 
@@ -1335,7 +1444,9 @@ size_t Plan::device_bytes() const {
         if (once(fr.perm.get())) b += fr.perm->bytes;
         if (once(fr.fmask.get())) b += fr.fmask->bytes;
         if (once(fr.amask.get())) b += fr.amask->bytes;
+        if (fr.mf && once(fr.mf.get())) b += fr.mf->base.bytes + fr.mf->rec.bytes;
     }
+    b += mf_b.base.bytes + mf_b.rec.bytes + mf_b_frame.bytes;
     if (Gu_b) b += sparse_bytes(*Gu_b);
     if (Gut_b) b += sparse_bytes(*Gut_b);
     return b;
@@ -1456,7 +1567,10 @@ void Plan::forward_bf16(int f, const float2* x, bf16x2* y, hipStream_t s) {
     }
     {
         TimedRegion t(0, s);
-        spmv_c2_rowmajor_st(*fr.Gu_s, grid.p, true, y, true, nCh, nullptr, s);
+        if (frame_mf(f))
+            u2n_mf(*fr.mf, nullptr, fr.mf->P, grid.p, true, Nu, y, true, nCh, imDim, mf_geo(), mfk, s);
+        else
+            spmv_c2_rowmajor_st(*fr.Gu_s, grid.p, true, y, true, nCh, nullptr, s);
     }
 }
 
@@ -1763,6 +1877,7 @@ int mona_plan_set_option(mona_plan P, int option, long long value) {
                            "FFT lines per tile must be -1, 0, 16, 32 or 64");
                 P->P.fft_lines = (int)value;
                 break;
+            case MONA_OPT_MATRIX_FREE: P->P.use_mf = value != 0; break;
             default: MONA_CHECK(false, MONA_ERR_ARG, "unknown option");
         }
     });

@@ -3,6 +3,7 @@
 #include <memory>
 #include <vector>
 
+#include "gridmf.hpp"
 #include "mona_internal.hpp"
 
 namespace mona {
@@ -46,6 +47,8 @@ struct Frame {
     std::shared_ptr<DevBuf> ivaly;
     bool ivals = false;
     int64_t P = 0;
+    // matrix-free gridding data (frames whose Gu came with its trajectory)
+    std::shared_ptr<MfPoints> mf;
 };
 
 // Records HIP events around a region when kernel timing is enabled.
@@ -127,6 +130,16 @@ struct Plan {
     bool use_batch = false;            // teva_chain uses the batched operators
     bool use_graph = false;            // ADMM iterations replayed from a HIP graph
     int fft_tile = 0;                  // plain FFT pass tile (0 = default)
+    // matrix-free gridding: used for the frames that have its data (all
+    // frames of a batch); off = the stored matrices everywhere
+    bool use_mf = true;
+    bool mf_kernel_ok = false;
+    MfKernel mfk;
+    MfGeo mf_geo() const;
+    bool frame_mf(int f) const;        // frame f grids matrix-free
+    bool batch_mf(int nA) const;       // frames 0..nA-1 all do
+    MfPoints mf_b;                     // frames' points back to back
+    DevBuf mf_b_frame;                 // uint16 frame of each point
     int fft_lines = 0;                 // register FFT kernel lines per tile (16/32/64; 0 default, -1 off)
     std::vector<SpView> view_u, view_t;  // leading frames 0..nA-1 of Gu_b / Gut_b
     void build_batch(hipStream_t s);

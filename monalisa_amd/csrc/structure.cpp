@@ -394,6 +394,69 @@ void deform_to_sparse(int imDim, const int* N_in, const double* v, const double*
     assemble(nPt, Nu, nNb, cnt, cols, wts, wts, Dn, want_gn, want_gu, false, Gn, Gu, Gut);
 }
 
+void traj_points(const TrajInfo& T, TrajPoints* out) {
+    MONA_CHECK(T.nWin % 2 == 1, MONA_ERR_ARG, "matrix-free gridding needs an odd window width");
+    KernelSpec K;
+    K.type = T.ktype;
+    for (int i = 0; i < 3; i++) K.p[i] = i < T.nParam ? as_single(T.kp[i]) : 0.0;
+    K.I0a = T.ktype == MONA_KERNEL_KAISER ? bessel_i0(K.p[1]) : 1.0;
+    const int D = T.imDim, h = T.nWin / 2, nw = T.nWin;
+    double N[3] = {1, 1, 1}, dK[3] = {1, 1, 1}, shift[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+        N[d] = as_single((double)T.N[d]);
+        dK[d] = as_single(T.dK[d]);
+        shift[d] = std::trunc(N[d] / 2 + 1);
+    }
+    int nNb = 1;
+    for (int d = 0; d < D; d++) nNb *= nw;
+    out->base.assign((size_t)T.nPt * 3, 0);
+    out->rest.assign((size_t)T.nPt * 3, 0.0);
+    out->inv_sum.assign(T.nPt, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < T.nPt; p++) {
+        double tg[3], tf[3], tr[3];
+        bool del = false;
+        for (int d = 0; d < D; d++) {
+            tg[d] = T.t[(int64_t)p * D + d] / dK[d] + shift[d];
+            if (tg[d] < 1 || tg[d] > N[d]) del = true;
+            tf[d] = std::floor(tg[d] + 0.5);
+            tr[d] = tg[d] - tf[d];
+        }
+        if (del) continue;
+        double sum = 0.0;
+        for (int j = 0; j < nNb; j++) {
+            int c[3] = {j % nw - h, D > 1 ? (j / nw) % nw - h : 0, D > 2 ? j / (nw * nw) - h : 0};
+            double sq = 0.0;
+            for (int d = 0; d < D; d++) {
+                double dd = tr[d] - c[d];
+                sq = sq + dd * dd;
+            }
+            sum += K.weight(std::sqrt(sq));
+        }
+        // the device evaluates the kernel without its constant factor
+        // (I0(alpha) for Kaiser, sqrt(2 pi) sigma for Gauss)
+        sum *= T.ktype == MONA_KERNEL_KAISER ? K.I0a : std::sqrt(2.0 * M_PI) * K.p[0];
+        for (int d = 0; d < D; d++) {
+            double m = std::fmod(tf[d] - 1, N[d]);
+            if (m < 0) m += N[d];
+            out->base[(size_t)p * 3 + d] = (int)m;
+            out->rest[(size_t)p * 3 + d] = tr[d];
+        }
+        out->inv_sum[p] = sum > 0 ? 1.0 / sum : 0.0;
+    }
+}
+
+std::shared_ptr<const TrajInfo> make_traj_info(int imDim, int nPt, const double* t, const int* N_u,
+                                               const double* dK_u, int kernelType, int nWin,
+                                               const double* kp, int nParam) {
+    auto T = std::make_shared<TrajInfo>();
+    T->imDim = imDim; T->nPt = nPt; T->ktype = kernelType; T->nWin = nWin; T->nParam = nParam;
+    for (int d = 0; d < imDim; d++) { T->N[d] = N_u[d]; T->dK[d] = dK_u[d]; }
+    for (int i = 0; i < nParam && i < 3; i++) T->kp[i] = kp[i];
+    T->t.assign(t, t + (size_t)nPt * imDim);
+    return T;
+}
+
 }  // namespace mona
 
 using namespace mona;
@@ -467,6 +530,10 @@ extern "C" int mona_traj2sparsemat(int imDim, int nPt, const double* t, const do
                 *Gu = nullptr;
                 throw;
             }
+            // the trajectory travels with Gu / Gut: plans can grid from it
+            auto T = make_traj_info(imDim, nPt, t, N_u, dK_u, kernelType, nWin, kernelParam, nParam);
+            (*Gu)->S.traj = T;
+            (*Gut)->S.traj = T;
         }
     });
 }

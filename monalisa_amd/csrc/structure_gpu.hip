@@ -430,6 +430,9 @@ extern "C" int mona_traj2sparsemat_device(int imDim, int nPt, const double* t, c
                 *Gu = nullptr;
                 throw;
             }
+            auto T = make_traj_info(imDim, nPt, t, N_u, dK_u, kernelType, nWin, kernelParam, nParam);
+            (*Gu)->S.traj = T;
+            (*Gut)->S.traj = T;
         }
     });
 }
