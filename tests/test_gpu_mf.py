@@ -3,7 +3,7 @@ bmTraj2SparseMat's builders keep with Gu) against the stored-matrix product
 of the same plan and against the float64 oracle: within float rounding of
 the stored product, within the 1e-5 gridding tolerance of the oracle, in 2-D
 and 3-D, Kaiser and Gauss, odd and even coil counts, a sparse frame, deleted
-points, bf16 storage and the batched form."""
+points, bf16 storage, the batched form and CG-SENSE."""
 import numpy as np
 import pytest
 
@@ -112,3 +112,28 @@ def test_mf_batched_forward_equals_per_frame():
         rf = P.bmSensa(xf, ys[f], ves[f], 1, f=f)
         np.testing.assert_array_equal(xb[f].cpu().numpy(), xf.cpu().numpy())
         assert rb[f, 0] == rf[0]
+
+
+def test_mf_sensa_dense_centre():
+    """CG-SENSE with the matrix-free forward gridding (the adjoint keeps the
+    stored Gut) on a trajectory dense at the k-space centre (rows of more
+    than 64 entries) matches the float64 oracle."""
+    import torch
+    from monalisa_amd import synth
+    print("[gpu] matrix-free forward in CG-SENSE, dense centre", flush=True)
+    p = make_problem("3d32", seed=6, nCh=4)
+    t = synth.koosh_ball(20, 30, 32, k_max=8.0)
+    p["t"], p["ve"] = t, synth.ve_radial3(t, 32, 600)
+    P, G = _plan(p)
+    assert int(G["Gut"].r_nJump.max()) > 64
+    x = torch.from_numpy(p["x"]).cuda()
+    y = P.bmShanna(x)
+    ve = torch.from_numpy(p["ve"].astype(np.float32)).cuda()
+    x0 = torch.zeros_like(x)
+    P.bmSensa(x0, y, ve, 6)
+    R = oracle_mats(p)
+    Cn = nd(p["C"].astype(np.complex128), p["n_u"], 4)
+    xs, _ = O.sensa(np.zeros(p["n_u"], complex), y.cpu().numpy().astype(np.complex128),
+                    p["ve"].astype(np.float32).astype(np.float64), Cn, R["Gu"], R["Gut"], p["N_u"], p["n_u"],
+                    p["dK_u"], 6, None, 1, kernel=(p["kernel"], None, None))
+    assert rel(x0.cpu().numpy(), xs.reshape(-1, order="F")) < 1e-4
