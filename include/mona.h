@@ -403,6 +403,9 @@ int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream
 #define MONA_OPT_MATRIX_FREE 5  /* forward gridding computed from the trajectory that mona_traj2sparsemat[_device]
                                    keeps with Gu (frames whose Gu has one and whose geometry and kernel match the
                                    plan; nWin 3): 1 = on (default), 0 = the stored entries */
+#define MONA_OPT_STAGED_ADJOINT 6 /* adjoint gridding: the k-space rows of groups of neighbouring grid cells
+                                     staged in LDS once per group (1 = on, default) or gathered per entry (0);
+                                     the results are the same bit for bit */
 int mona_plan_set_option(mona_plan P, int option, long long value);
 /* Autotuner: times forward + adjoint of one frame for each FFT pass variant
  * and keeps the fastest; *chosen (may be NULL) receives the register kernel's

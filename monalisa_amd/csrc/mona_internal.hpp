@@ -209,6 +209,17 @@ struct SparseDev {
     DevBuf long_first;        // int32[n_long+1] first chunk of each long row
     DevBuf chunk_begin;       // int64[2*n_chunk] [begin, end) entry offsets of each chunk
     mutable DevBuf partial;   // float2[n_chunk * n_vec] scratch
+    // Staged form of the short rows (operator products over n_vec ==
+    // group_vec vectors): runs of consecutive short rows whose entries
+    // reference at most a group's capacity of distinct columns.  A group's
+    // columns (ascending) are loaded into LDS once and its rows are summed
+    // from there; slot[k] is entry k's position in its group's column list.
+    int n_group = 0, group_vec = 0;
+    int group_cap = 0, group_ecap = 0, group_rcap = 0;  // most columns, entries, rows of a group
+    DevBuf group;             // int4[n_group]: first row, end row, first column in group_col, column count
+    DevBuf group_col;         // int32[n_group_col]: the groups' columns back to back
+    int64_t n_group_col = 0;
+    DevBuf slot;              // uint16[nnz] (entries of long rows unused)
     // host copy of the reference encoding (kept for export; decoded onto the
     // device on first use)
     std::vector<int> h_r_jump, h_r_nJump, h_l_jump;
@@ -232,6 +243,11 @@ void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector
                          const std::vector<float>& val, const std::vector<int>& r_nJump,
                          const std::vector<int>* rowidx, hipStream_t s);
 void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s);
+// Builds S's staged groups for products over n_vec vectors from the decoded
+// host arrays (per-row counts, columns); no groups when n_vec is too large
+// for a group to hold the longest short row.
+void build_groups(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
+                  hipStream_t s);
 
 // Rows with more entries than this are summed in chunks by the operator path.
 constexpr int kLongRow = 64;
@@ -250,15 +266,19 @@ struct SpView {
     int64_t l_size = 0;
     int n_long = 0;
     int64_t n_chunk = 0;
+    int n_group = 0;
 };
 // zero_rest = false leaves rows of w that S does not store untouched (for a
 // consumer that never reads them).
+// staged = false: the short rows gather from global memory even when S has
+// staged groups (same results bit for bit).
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s, const SpView* view = nullptr, bool zero_rest = true);
+                      const float* yscale, hipStream_t s, const SpView* view = nullptr, bool zero_rest = true,
+                      bool staged = true);
 // The same product with v and / or w stored as bf16x2 (sums in float).
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
                          const float* yscale, hipStream_t s, const SpView* view = nullptr,
-                         bool zero_rest = true);
+                         bool zero_rest = true, bool staged = true);
 // Block-diagonal stacking on the device: part f's columns are shifted by
 // col_off[f] and its output rows by row_off[f] (out has r_size columns and
 // l_size output rows; its stored rows are the parts' in order).  views[f]

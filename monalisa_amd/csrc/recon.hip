@@ -793,6 +793,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     auto Gut_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
     Gut_s->covers_all = !sparse_frame && !use_ival;
+    build_groups(*Gut_s, dnj, dcol, nCh, s);
     // cell masks: cells the forward gridding reads (Gu columns) and cells the
     // adjoint gridding writes (Gut rows with entries); equal for a
     // transposed pair.  Layout of the pass along the last axis: for each
@@ -1109,7 +1110,7 @@ void Plan::adjoint(int f, const float2* y, float2* x, bool combine, const float*
     const bool limited = masked_grid(f) || ival_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s, nullptr, !limited);
+        spmv_c2_rowmajor(*fr.Gut_s, y, gr, nCh, hy, s, nullptr, !limited, use_staged);
     }
     {
         TimedRegion t(2, s);
@@ -1282,7 +1283,7 @@ void Plan::adjoint_batch(const float2* y, float2* x, const float* hy, float scal
     const bool limited = fwd_mask(-1) || ivals_of(-1);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA], !limited);
+        spmv_c2_rowmajor(*Gut_b, y, grid_b.as<float2>(), nCh, hy, s, &view_t[nA], !limited, use_staged);
     }
     {
         TimedRegion t(2, s);
@@ -1421,7 +1422,8 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
 namespace {
 size_t sparse_bytes(const SparseDev& S) {
     return S.col.bytes + S.val.bytes + S.rowptr.bytes + S.rowidx.bytes + S.cv.bytes + S.long_row.bytes +
-           S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes;
+           S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes + S.group.bytes + S.group_col.bytes +
+           S.slot.bytes;
 }
 }  // namespace
 
@@ -1583,7 +1585,7 @@ void Plan::adjoint_bf16(int f, const bf16x2* y, float2* x, const float* hy, floa
     const bool limited = masked_grid(f) || ival_grid(f);
     {
         TimedRegion t(1, s);
-        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s, nullptr, !limited);
+        spmv_c2_rowmajor_st(*fr.Gut_s, y, true, grid.p, true, nCh, hy, s, nullptr, !limited, use_staged);
     }
     {
         TimedRegion t(2, s);
@@ -1878,6 +1880,7 @@ int mona_plan_set_option(mona_plan P, int option, long long value) {
                 P->P.fft_lines = (int)value;
                 break;
             case MONA_OPT_MATRIX_FREE: P->P.use_mf = value != 0; break;
+            case MONA_OPT_STAGED_ADJOINT: P->P.use_staged = value != 0; break;
             default: MONA_CHECK(false, MONA_ERR_ARG, "unknown option");
         }
     });

@@ -133,6 +133,7 @@ struct Plan {
     // matrix-free gridding: used for the frames that have its data (all
     // frames of a batch); off = the stored matrices everywhere
     bool use_mf = true;
+    bool use_staged = true;            // adjoint gridding from LDS-staged groups (Gut_s groups)
     bool mf_kernel_ok = false;
     MfKernel mfk;
     MfGeo mf_geo() const;

@@ -352,6 +352,116 @@ spmv_cv_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const SpEn
     st_v<CPT>(partial + c * n_vec, k0, k1, ar, ai);
 }
 
+// Short rows of an operator product summed from LDS.  The block of a group
+// first stages in LDS the vectors v[c][0 .. n_vec) of the group's columns c
+// (scaled by scale[c] as row_sum_cv scales them), the group's entries
+// (value, slot) and its row offsets, then sums each of the group's rows with
+// n_vec / CPT work-items per row: entries added one by one in stored order
+// with row_sum_cv's operations, so every row equals spmv_cv's bit for bit.
+// A column's vector is read from global memory once per group instead of
+// once per entry (neighbouring grid cells gather mostly the same k-space
+// points), and the entries once per group instead of once per work-item of
+// their row.  A column takes n_vec + 1 complex values in LDS: with the odd
+// stride the rows of one wave spread over all banks.  Loads are issued in
+// branch-free batches (indices clamped into range, surplus results
+// dropped) so that a batch's gathers are in flight together.
+template <bool SCALE, int CPT, class TV, class TW>
+__global__ void __launch_bounds__(256)
+spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ rowptr, const SpEntry* __restrict__ cv,
+            const uint16_t* __restrict__ slot, const int* __restrict__ gcol, const int* __restrict__ rowidx,
+            int n_vec, const TV* __restrict__ v, TW* __restrict__ w, const float* __restrict__ scale, int xg,
+            int ccap, int ecap, int rcap) {
+    extern __shared__ float2 sv[];  // [ccap][n_vec + 1] vectors, [ecap] entries, row offsets, output rows
+    const int ls = n_vec + 1;
+    float2* se = sv + ccap * ls;                    // (value, slot bits)
+    int* ro = reinterpret_cast<int*>(se + ecap);    // [rows + 1]
+    int* rw = ro + rcap + 1;                        // [rows]
+    const int4 G = groups[xcd_block(blockIdx.x, gridDim.x, xg)];
+    const int64_t e0 = rowptr[G.x];
+    const int ne = (int)(rowptr[G.y] - e0), nr = G.y - G.x;
+    const int items = G.w * n_vec;
+    // vectors: item i = value i % n_vec of column i / n_vec
+    for (int i0 = threadIdx.x; i0 < items; i0 += 8 * 256) {
+        int c[8], dst[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = min(i0 + u * 256, items - 1);
+            const int k = i / n_vec;
+            c[u] = gcol[G.z + k];
+            dst[u] = k * ls + (i - k * n_vec);
+        }
+        float2 x[8];
+        float sc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = ld_c(v + (int64_t)c[u] * n_vec + (dst[u] % ls));
+            if (SCALE) sc[u] = scale[c[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (SCALE) {
+                x[u].x = sc[u] * x[u].x;
+                x[u].y = sc[u] * x[u].y;
+            }
+            sv[dst[u]] = x[u];  // a surplus item rewrites item items - 1's value
+        }
+    }
+    // entries and row offsets
+    for (int i0 = threadIdx.x; i0 < ne; i0 += 4 * 256) {
+        float m[4];
+        int sl[4], t[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            t[u] = min(i0 + u * 256, ne - 1);
+            m[u] = cv[e0 + t[u]].val;
+            sl[u] = slot[e0 + t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) se[t[u]] = make_float2(m[u], __int_as_float(sl[u]));
+    }
+    for (int i = threadIdx.x; i <= nr; i += 256) {
+        ro[i] = (int)(rowptr[G.x + i] - e0);
+        if (i < nr) rw[i] = rowidx ? rowidx[G.x + i] : G.x + i;
+    }
+    __syncthreads();
+    const int tpr = n_vec / CPT;
+    const int rpp = 256 / tpr;  // rows per pass
+    const int rr = threadIdx.x / tpr, q = threadIdx.x - rr * tpr;
+    if (rr >= rpp) return;
+    for (int r = rr; r < nr; r += rpp) {
+        const int jb = ro[r], je = ro[r + 1];
+        float ar[CPT], ai[CPT];
+#pragma unroll
+        for (int h = 0; h < CPT; h++) ar[h] = ai[h] = 0.0f;
+        for (int j = jb; j < je; j += 4) {
+            const int cnt = min(je - j, 4);
+            float2 e[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) e[u] = se[min(j + u, je - 1)];
+            float2 xv[4][CPT];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float2* xs = sv + __float_as_int(e[u].y) * ls + q * CPT;
+#pragma unroll
+                for (int h = 0; h < CPT; h++) xv[u][h] = xs[h];
+            }
+            // a surplus entry adds -0, which leaves every sum unchanged
+            // (+0, -0, infinities and NaN included)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool on = u < cnt;
+#pragma unroll
+                for (int h = 0; h < CPT; h++) {
+                    const float pr = e[u].x * xv[u][h].x, pi = e[u].x * xv[u][h].y;
+                    ar[h] = ar[h] + (on ? pr : -0.0f);
+                    ai[h] = ai[h] + (on ? pi : -0.0f);
+                }
+            }
+        }
+        st_n<CPT>(w + (int64_t)rw[r] * n_vec + q * CPT, ar, ai);
+    }
+}
+
 // Partial sums of the chunks of long rows: chunk c covers entries
 // [chunk_be[2c], chunk_be[2c+1]).
 template <bool SCALE, class TV>
@@ -425,6 +535,13 @@ __global__ void k_add_i64(int64_t* dst, const int64_t* src, int64_t n, int64_t a
 __global__ void k_add_entries(SpEntry* dst, const SpEntry* src, int64_t n, int64_t add) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = SpEntry{(int)(src[i].col + add), src[i].val};
+}
+__global__ void k_add_groups(int4* dst, const int4* src, int64_t n, int row_add, int col_add) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int4 g = src[i];
+        dst[i] = make_int4(g.x + row_add, g.y + row_add, g.z + col_add, g.w);
+    }
 }
 __global__ void k_iota_i32(int* dst, int64_t n, int64_t start) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -588,6 +705,108 @@ void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s) {
     MONA_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+// LDS budget of a group's staged vectors, and the most entries and rows a
+// group takes (MONA_STAGE_KB, MONA_GROUP_ENTRIES, MONA_GROUP_ROWS override
+// them; a budget of 0 turns the staged form off).  16 KB, 1024 entries and
+// 256 rows were the best of a sweep on the 3-D N=128 (256^3 grid) 20-coil
+// CG-SENSE: adjoint gridding 27.8 ms per recon against 32.4 ms for the
+// per-entry gather; 24-40 KB budgets, 2048 entries or 512-1024 rows were
+// 1-18% slower.
+int env_int(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+int64_t stage_bytes() {
+    static const int64_t b = (int64_t)env_int("MONA_STAGE_KB", 16) * 1024;
+    return b;
+}
+int group_entries() {
+    static const int n = std::max(env_int("MONA_GROUP_ENTRIES", 1024), kLongRow);
+    return n;
+}
+int group_rows() {
+    static const int n = std::max(env_int("MONA_GROUP_ROWS", 256), 1);
+    return n;
+}
+constexpr int kGroupMaxCols = 1024;
+}  // namespace
+
+void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<int>& col, int n_vec,
+                  hipStream_t s) {
+    S.n_group = S.group_vec = S.group_cap = S.group_ecap = S.group_rcap = 0;
+    S.n_group_col = 0;
+    S.group.release();
+    S.group_col.release();
+    S.slot.release();
+    if (n_vec <= 0 || S.nnz == 0) return;
+    const int cap = (int)std::min<int64_t>(kGroupMaxCols, stage_bytes() / ((int64_t)(n_vec + 1) * sizeof(float2)));
+    if (cap < kLongRow) return;  // a short row's columns must fit one group
+    const int rows = (int)nj.size();
+    MONA_CHECK((int64_t)col.size() == S.nnz, MONA_ERR_INTERNAL, "group build: entry count mismatch");
+    std::vector<int64_t> rp(rows + 1, 0);
+    for (int r = 0; r < rows; r++) rp[r + 1] = rp[r] + nj[r];
+    std::vector<int> mark(S.r_size, -1), seen(S.r_size, -1), pos(S.r_size, 0), cur, gcol;
+    std::vector<int4> groups;
+    std::vector<uint16_t> slot(S.nnz, 0);
+    const int ecap = group_entries(), rcap = group_rows();
+    int g = 0, r0 = 0;
+    // closes the group of rows [r0, r1): columns ascending, entry slots
+    auto close = [&](int r1) {
+        if (r1 > r0) {
+            std::sort(cur.begin(), cur.end());
+            for (int k = 0; k < (int)cur.size(); k++) pos[cur[k]] = k;
+            for (int64_t e = rp[r0]; e < rp[r1]; e++) slot[e] = (uint16_t)pos[col[e]];
+            groups.push_back(make_int4(r0, r1, (int)gcol.size(), (int)cur.size()));
+            gcol.insert(gcol.end(), cur.begin(), cur.end());
+            g++;
+        }
+        cur.clear();
+    };
+    for (int r = 0; r < rows; r++) {
+        if (nj[r] > kLongRow) {  // long rows stay with the chunked partial sums
+            close(r);
+            r0 = r + 1;
+            continue;
+        }
+        int add = 0;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int c = col[e];
+            if (mark[c] != g && seen[c] != r) {
+                seen[c] = r;
+                add++;
+            }
+        }
+        if ((int)cur.size() + add > cap || r - r0 >= rcap || rp[r + 1] - rp[r0] > ecap) {
+            close(r);
+            r0 = r;
+        }
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int c = col[e];
+            if (mark[c] != g) {
+                mark[c] = g;
+                cur.push_back(c);
+            }
+        }
+    }
+    close(rows);
+    if (groups.empty()) return;
+    S.group.alloc(groups.size() * sizeof(int4));
+    S.group_col.alloc(std::max<size_t>(gcol.size(), 1) * sizeof(int));
+    S.slot.alloc((size_t)S.nnz * sizeof(uint16_t));
+    MONA_HIP(hipMemcpyAsync(S.group.p, groups.data(), groups.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    if (!gcol.empty())
+        MONA_HIP(hipMemcpyAsync(S.group_col.p, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.slot.p, slot.data(), (size_t)S.nnz * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipStreamSynchronize(s));  // host arrays are temporaries
+    S.n_group = (int)groups.size();
+    S.group_vec = n_vec;
+    S.group_cap = cap;
+    S.group_ecap = ecap;
+    S.group_rcap = rcap;
+    S.n_group_col = (int64_t)gcol.size();
+}
+
 void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector<int>& col,
                          const std::vector<float>& val, const std::vector<int>& r_nJump,
                          const std::vector<int>* rowidx, hipStream_t s) {
@@ -645,11 +864,11 @@ int xcd_group(bool adjoint) {
 // 16-byte aligned, operator matrices only).
 template <int CPT, int U, class TV, class TW>
 void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
-                    hipStream_t s, float2* part_ext) {
+                    hipStream_t s, float2* part_ext, bool main_rows = true) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
     const int64_t total = (int64_t)V.rows * (n_vec / CPT);
     const SpEntry* cv = S.cv.as<SpEntry>();
-    if (total > 0) {
+    if (total > 0 && main_rows) {
         const int xg = xcd_group(yscale != nullptr);
         const unsigned nb = blocks_for(total, 256);
         if (yscale)
@@ -685,9 +904,30 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
     }
 }
 
+// The staged short rows of the first V.n_group groups (spmv_groups).
+template <int CPT, class TV, class TW>
+void spmv_groups_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
+                        hipStream_t s) {
+    const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
+    const size_t lds = (size_t)S.group_cap * (n_vec + 1) * sizeof(float2) + (size_t)S.group_ecap * sizeof(float2) +
+                       (size_t)(2 * S.group_rcap + 1) * sizeof(int);
+    const int xg = xcd_group(yscale != nullptr);
+    if (yscale)
+        spmv_groups<true, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.rowptr.as<int64_t>(),
+                                                           S.cv.as<SpEntry>(), S.slot.as<uint16_t>(),
+                                                           S.group_col.as<int>(), ri, n_vec, v, w, yscale, xg,
+                                                           S.group_cap, S.group_ecap, S.group_rcap);
+    else
+        spmv_groups<false, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.rowptr.as<int64_t>(),
+                                                            S.cv.as<SpEntry>(), S.slot.as<uint16_t>(),
+                                                            S.group_col.as<int>(), ri, n_vec, v, w, nullptr, xg,
+                                                            S.group_cap, S.group_ecap, S.group_rcap);
+    MONA_HIP(hipGetLastError());
+}
+
 template <class TV, class TW>
 void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const float* yscale, hipStream_t s,
-                   const SpView* view, bool zero_rest, float2* part_ext = nullptr) {
+                   const SpView* view, bool zero_rest, float2* part_ext = nullptr, bool staged = true) {
     MONA_CHECK(S.on_device, MONA_ERR_STATE, "sparse matrix not uploaded");
     if (S.cv.p && S.nnz > 0) {
         SpView V;
@@ -698,12 +938,25 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
             V.l_size = S.l_size;
             V.n_long = S.n_long;
             V.n_chunk = S.n_chunk;
+            V.n_group = S.n_group;
         }
         const bool al = ((uintptr_t)v % 16 == 0) && ((uintptr_t)w % 16 == 0);
         const int cpt = (al && n_vec % 4 == 0) ? 4 : (al && n_vec % 2 == 0) ? 2 : 1;
         if ((S.squeezed || S.l_nJump != S.l_size) && !S.covers_all && zero_rest)
             fill_f32(reinterpret_cast<float*>(w), 0.0f, V.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
         if (V.rows == 0) return;
+        // staged short rows, then the long rows' chunks as below
+        if (staged && V.n_group > 0 && S.group_vec == n_vec) {
+            if (cpt == 4) spmv_groups_launch<4>(S, V, v, w, n_vec, yscale, s);
+            else if (cpt == 2) spmv_groups_launch<2>(S, V, v, w, n_vec, yscale, s);
+            else spmv_groups_launch<1>(S, V, v, w, n_vec, yscale, s);
+            if (V.n_long > 0) {
+                if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s, part_ext, false);
+                else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s, part_ext, false);
+                else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s, part_ext, false);
+            }
+            return;
+        }
         // measured on 3D grids at 16 and 20 coils: the gridding gather
         // (~27 entries a row) runs best with two vectors per work-item and
         // 16 gathers in flight; the weighted spreading product (rows of ~8
@@ -758,20 +1011,24 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
 }  // namespace
 
 void spmv_c2_rowmajor(const SparseDev& S, const float2* v, float2* w, int n_vec,
-                      const float* yscale, hipStream_t s, const SpView* view, bool zero_rest) {
-    spmv_c2_typed(S, v, w, n_vec, yscale, s, view, zero_rest);
+                      const float* yscale, hipStream_t s, const SpView* view, bool zero_rest, bool staged) {
+    spmv_c2_typed(S, v, w, n_vec, yscale, s, view, zero_rest, nullptr, staged);
 }
 
 void spmv_c2_rowmajor_st(const SparseDev& S, const void* v, bool v_bf16, void* w, bool w_bf16, int n_vec,
-                         const float* yscale, hipStream_t s, const SpView* view, bool zero_rest) {
+                         const float* yscale, hipStream_t s, const SpView* view, bool zero_rest, bool staged) {
     if (v_bf16 && w_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest,
+                      nullptr, staged);
     else if (v_bf16)
-        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest);
+        spmv_c2_typed(S, static_cast<const bf16x2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest,
+                      nullptr, staged);
     else if (w_bf16)
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<bf16x2*>(w), n_vec, yscale, s, view, zero_rest,
+                      nullptr, staged);
     else
-        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest);
+        spmv_c2_typed(S, static_cast<const float2*>(v), static_cast<float2*>(w), n_vec, yscale, s, view, zero_rest,
+                      nullptr, staged);
 }
 
 void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& parts,
@@ -795,6 +1052,16 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
         nch += P.n_chunk;
         mx = std::max(mx, P.max_row);
     }
+    // staged groups stack when every part has them for the same vector count
+    bool groups = n > 0;
+    int64_t ng = 0, ngc = 0;
+    for (size_t f = 0; f < n && groups; f++) {
+        const SparseDev& P = *parts[f];
+        groups = P.n_group > 0 && P.group_vec == parts[0]->group_vec && P.group_cap == parts[0]->group_cap &&
+                 P.group_ecap == parts[0]->group_ecap && P.group_rcap == parts[0]->group_rcap;
+        ng += P.n_group;
+        ngc += P.n_group_col;
+    }
     MONA_CHECK(rows < 0x7fffffff && nnz < ((int64_t)1 << 40), MONA_ERR_ARG, "stacked matrix too large");
     out.r_size = r_size;
     out.l_size = l_size;
@@ -815,13 +1082,27 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
         out.long_first.alloc((size_t)(nl + 1) * sizeof(int));
         out.chunk_begin.alloc((size_t)nch * 2 * sizeof(int64_t));
     }
+    out.n_group = out.group_vec = out.group_cap = out.group_ecap = out.group_rcap = 0;
+    out.n_group_col = 0;
+    if (groups) {
+        MONA_CHECK(ngc < 0x7fffffff, MONA_ERR_ARG, "stacked matrix too large");
+        out.group.alloc((size_t)ng * sizeof(int4));
+        out.group_col.alloc((size_t)std::max<int64_t>(ngc, 1) * sizeof(int));
+        out.slot.alloc((size_t)std::max<int64_t>(nnz, 1) * sizeof(uint16_t));
+        out.n_group = (int)ng;
+        out.group_vec = parts[0]->group_vec;
+        out.group_cap = parts[0]->group_cap;
+        out.group_ecap = parts[0]->group_ecap;
+        out.group_rcap = parts[0]->group_rcap;
+        out.n_group_col = ngc;
+    }
     if (views) views->assign(n + 1, SpView{});
-    int64_t eo = 0, ro = 0, co = 0;
+    int64_t eo = 0, ro = 0, co = 0, go = 0, gco = 0;
     int lo = 0;
     auto nb = [](int64_t k) { return blocks_for(std::max<int64_t>(k, 1), 256); };
     for (size_t f = 0; f < n; f++) {
         const SparseDev& P = *parts[f];
-        if (views) (*views)[f] = SpView{(int)ro, row_off[f], lo, co};
+        if (views) (*views)[f] = SpView{(int)ro, row_off[f], lo, co, (int)go};
         if (P.nnz > 0) {
             k_add_i32<<<nb(P.nnz), 256, 0, s>>>(out.col.as<int>() + eo, P.col.as<int>(), P.nnz, col_off[f]);
             k_add_entries<<<nb(P.nnz), 256, 0, s>>>(out.cv.as<SpEntry>() + eo, P.cv.as<SpEntry>(), P.nnz,
@@ -846,13 +1127,25 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
             k_add_i64<<<nb(2 * P.n_chunk), 256, 0, s>>>(out.chunk_begin.as<int64_t>() + 2 * co,
                                                         P.chunk_begin.as<int64_t>(), 2 * P.n_chunk, eo);
         }
+        if (groups) {
+            const int64_t pgc = P.n_group_col;
+            k_add_groups<<<nb(P.n_group), 256, 0, s>>>(out.group.as<int4>() + go, P.group.as<int4>(), P.n_group,
+                                                      (int)ro, (int)gco);
+            k_add_i32<<<nb(pgc), 256, 0, s>>>(out.group_col.as<int>() + gco, P.group_col.as<int>(), pgc,
+                                              col_off[f]);
+            if (P.nnz > 0)
+                MONA_HIP(hipMemcpyAsync(out.slot.as<uint16_t>() + eo, P.slot.p, P.nnz * sizeof(uint16_t),
+                                        hipMemcpyDeviceToDevice, s));
+            go += P.n_group;
+            gco += pgc;
+        }
         MONA_HIP(hipGetLastError());
         eo += P.nnz;
         ro += P.l_nJump;
         lo += P.n_long;
         co += P.n_chunk;
     }
-    if (views) (*views)[n] = SpView{(int)ro, (int64_t)l_size, lo, co};
+    if (views) (*views)[n] = SpView{(int)ro, (int64_t)l_size, lo, co, (int)go};
     out.on_device = true;
 }
 

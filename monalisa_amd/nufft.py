@@ -114,9 +114,10 @@ class Plan:
             _arg(ve_list[f], f"ve[{f}]", self.nPt(f), real=True)
 
     # -- execution options (no reference counterpart) ----------------------
-    OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE, OPT_FFT_LINES, OPT_MATRIX_FREE = 1, 2, 3, 4, 5
+    OPT_BATCH_FRAMES, OPT_GRAPH, OPT_FFT_TILE, OPT_FFT_LINES, OPT_MATRIX_FREE, OPT_STAGED_ADJOINT = 1, 2, 3, 4, 5, 6
 
-    def set_options(self, batch_frames=None, graph=None, fft_tile=None, fft_lines=None, matrix_free=None):
+    def set_options(self, batch_frames=None, graph=None, fft_tile=None, fft_lines=None, matrix_free=None,
+                    staged_adjoint=None):
         """batch_frames: the temporal-TV solver applies its operators to all
         frames in one launch per gridding op; graph: its ADMM iterations are
         replayed from a HIP graph (constant delta/rho); fft_tile: plain FFT
@@ -124,10 +125,12 @@ class Plan:
         fft_lines: register-resident passes (N = 16 * N2) with 16 / 32 / 64
         lines per tile on strided axes, 0 = default, -1 = LDS kernel only;
         matrix_free: the forward gridding from the trajectory kept with Gu
-        (default on) instead of the stored entries."""
+        (default on) instead of the stored entries; staged_adjoint: the
+        adjoint gridding sums groups of neighbouring grid cells from k-space
+        rows staged in LDS (default on; the same results bit for bit)."""
         for key, v in ((self.OPT_BATCH_FRAMES, batch_frames), (self.OPT_GRAPH, graph),
                        (self.OPT_FFT_TILE, fft_tile), (self.OPT_FFT_LINES, fft_lines),
-                       (self.OPT_MATRIX_FREE, matrix_free)):
+                       (self.OPT_MATRIX_FREE, matrix_free), (self.OPT_STAGED_ADJOINT, staged_adjoint)):
             if v is not None:
                 check(lib.mona_plan_set_option(self._h, key, int(v)))
 
