@@ -407,6 +407,19 @@ int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream
                                      staged in LDS once per group (1 = on, default) or gathered per entry (0);
                                      the results are the same bit for bit */
 int mona_plan_set_option(mona_plan P, int option, long long value);
+/* Witness snapshots of the solvers (replaces the per-iteration
+ * witnessInfo.watch calls of bmSensa.m:85, bmSteva and the Morphosia solvers,
+ * bmWitnessInfo.m:92-190).  While set, mona_sensa, mona_steva,
+ * mona_teva_chain, mona_morphosia and mona_sensa_morphosia copy, after outer
+ * iteration c (0-based), frame 0's witness image -- the whole image in 1-D
+ * and 2-D, its middle z slice (z = n_z / 2, 0-based) in 3-D;
+ * mona_plan_witness_elems complex values -- into slices + c * elems while
+ * c < n_slots, and the whole x (all frames) into full + k * (x elements)
+ * after iteration full_ind[k] (1-based, as the reference numbers them).
+ * Device buffers owned by the caller; slices = full = NULL turns it off.
+ * Runs with a witness set replay no HIP graph. */
+int mona_plan_set_witness(mona_plan P, void* slices, int n_slots, void* full, const int* full_ind, int n_full);
+int mona_plan_witness_elems(mona_plan P, long long* elems);
 /* Autotuner: times forward + adjoint of one frame for each FFT pass variant
  * and keeps the fastest; *chosen (may be NULL) receives the register kernel's
  * lines per tile, or minus the LDS kernel's tile. */

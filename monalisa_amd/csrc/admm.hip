@@ -361,6 +361,7 @@ void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const
         vsub(y, Mp, res_y, ny, s);
         sqnorm(res_y, ny, hy, P.nCh, 1.0, hist + 2 * c, false, part, s);
         abs1(Fp, (int64_t)D * nu, (double)HZ, hist + 2 * c + 1, part, s);
+        P.witness(c, x, nu, s);
     }
     if (hist_out && nIter > 0) {
         MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -594,7 +595,9 @@ void morphosia(Plan& P, int solver, int nFr1, float2* x, float2* z, float2* u, b
     };
     bool constant = true;
     for (int c = 1; c < nIter; c++) constant = constant && delta[c] == delta[0] && rho[c] == rho[0];
-    if (P.use_graph && constant && nIter > 1) {
+    // witness snapshots differ per iteration: those runs stay eager
+    const bool watched = P.wit_slices || P.wit_full;
+    if (P.use_graph && constant && nIter > 1 && !watched) {
         // iteration 0 eagerly (allocates every work buffer), then one
         // captured iteration replayed nIter - 1 times, on a private stream
         // ordered after / before the caller's stream
@@ -640,7 +643,10 @@ void morphosia(Plan& P, int solver, int nFr1, float2* x, float2* z, float2* u, b
         MONA_HIP(hipEventRecord(g.e1, g.gs));
         MONA_HIP(hipStreamWaitEvent(s, g.e1, 0));
     } else {
-        for (int c = 0; c < nIter; c++) body(delta[c], rho[c]);
+        for (int c = 0; c < nIter; c++) {
+            body(delta[c], rho[c]);
+            P.witness(c, x, nx, s);
+        }
     }
     if (hist_out && nIter > 0) {
         MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -751,6 +757,7 @@ void sensa_morphosia(Plan& P, float2* x, const float2* y, const long long* y_off
         A(x, Ap);
         vsub(ys, Ap, res, ny, s);
         sqnorm(res, ny, hy, P.nCh, 1.0, hist + c, false, part, s);
+        P.witness(c, x, nu, s);
     }
     if (hist_out && nIter > 0) {
         MONA_HIP(hipMemcpyAsync(hist_out, hist, (size_t)nIter * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -1354,6 +1354,21 @@ float device_max(const float* v, int64_t n, hipStream_t s) {
     return m;
 }
 
+// ================================================================ witness
+int64_t Plan::witness_elems() const { return imDim == 3 ? (int64_t)n[0] * n[1] : nu; }
+
+void Plan::witness(int c, const float2* x, int64_t x_elems, hipStream_t s) {
+    if (wit_slices && c < wit_slots) {
+        // the image is x-fastest: the middle z slice (z = n_z / 2, the
+        // reference's fix(n_u(end)/2) + 1) is one contiguous block
+        const int64_t off = imDim == 3 ? (int64_t)(n[2] / 2) * n[0] * n[1] : 0;
+        vcopy(x + off, wit_slices + (int64_t)c * witness_elems(), witness_elems(), s);
+    }
+    if (wit_full)
+        for (size_t k = 0; k < wit_full_ind.size(); k++)
+            if (wit_full_ind[k] == c + 1) vcopy(x, wit_full + (int64_t)k * x_elems, x_elems, s);
+}
+
 // ================================================================ CG-SENSE
 void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_max, int nIter, int nCGD,
                  double* residual, hipStream_t s) {
@@ -1407,10 +1422,11 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
             cg_beta(sc, s);
             vxpby(p, dag, &sc->b, nu, s);                    // p = dag + b p
         }
-        // monitoring residual <r, HY r>, r = y - M x   (bmSensa.m:82-84)
+        // monitoring residual <r, HY r>, r = y - M x   (bmSensa.m:82-85)
         forward(f, x, Mp, s);
         vsub(y, Mp, res, ny, s);
         sqnorm(res, ny, hy, nCh, 1.0, w.hist.as<double>() + it, false, part, s);
+        witness(it, x, nu, s);
     }
     if (residual && nIter > 0) {
         MONA_HIP(hipMemcpyAsync(residual, w.hist.p, nIter * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1860,6 +1876,27 @@ int mona_nakatsha_bf16(mona_plan P, int frame, const void* y, void* x, hipStream
                                                                   t, fr.P, p.nCh);
         MONA_HIP(hipGetLastError());
         p.adjoint_bf16(frame, t, static_cast<float2*>(x), nullptr, 1.0f, stream);
+    });
+}
+
+int mona_plan_set_witness(mona_plan P, void* slices, int n_slots, void* full, const int* full_ind, int n_full) {
+    return guarded([&] {
+        MONA_CHECK(P, MONA_ERR_ARG, "NULL plan");
+        MONA_CHECK(n_slots >= 0 && n_full >= 0, MONA_ERR_ARG, "n_slots and n_full must be >= 0");
+        MONA_CHECK(!slices || n_slots > 0, MONA_ERR_ARG, "a witness slice buffer needs n_slots >= 1");
+        MONA_CHECK(!full || (full_ind && n_full > 0), MONA_ERR_ARG, "a full-image buffer needs its iteration list");
+        Plan& p = P->P;
+        p.wit_slices = static_cast<float2*>(slices);
+        p.wit_slots = slices ? n_slots : 0;
+        p.wit_full = static_cast<float2*>(full);
+        p.wit_full_ind.assign(full ? full_ind : nullptr, full ? full_ind + n_full : nullptr);
+    });
+}
+
+int mona_plan_witness_elems(mona_plan P, long long* elems) {
+    return guarded([&] {
+        MONA_CHECK(P && elems, MONA_ERR_ARG, "NULL argument");
+        *elems = (long long)P->P.witness_elems();
     });
 }
 

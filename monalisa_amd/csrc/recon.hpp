@@ -150,6 +150,18 @@ struct Plan {
 
     void set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream_t s);
 
+    // Witness snapshots of the solvers (bmWitnessInfo.m:92-190): after outer
+    // iteration c (0-based) a solver copies frame 0's witness image (the
+    // whole image in 1-D / 2-D, its middle z slice in 3-D) into
+    // wit_slices + c * witness_elems() while c < wit_slots, and the whole x
+    // into wit_full + k * x_elems when c + 1 == wit_full_ind[k].
+    float2* wit_slices = nullptr;
+    int wit_slots = 0;
+    float2* wit_full = nullptr;
+    std::vector<int> wit_full_ind;
+    int64_t witness_elems() const;
+    void witness(int c, const float2* x, int64_t x_elems, hipStream_t s);
+
     // y = M x  (x [nu], y [P][nCh] in the frame's tile order)
     void forward(int f, const float2* x, float2* y, hipStream_t s);
     // x = scale * sum_c KFC_conj .* crop(cifft(Gut (hy .* y)))   (hy may be

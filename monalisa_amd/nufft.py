@@ -134,6 +134,44 @@ class Plan:
             if v is not None:
                 check(lib.mona_plan_set_option(self._h, key, int(v)))
 
+    def _watched(self, witnessInfo, nIter, x, run, names, params):
+        """run() with the device witness snapshots on when witnessInfo has
+        witness indices; then the reference's watch calls (bmSensa.m:85,
+        :92): one 'loop' call per outer iteration with its snapshot, the
+        full x kept at the witness indices, and a 'final' call.  names:
+        the history columns run() returns (stored in witnessInfo.param with
+        the run's parameters)."""
+        if witnessInfo is None:
+            return run()
+        import torch
+        watch = bool(witnessInfo.witness_ind) and nIter >= 1
+        if watch:
+            e = C.c_longlong()
+            check(lib.mona_plan_witness_elems(self._h, C.byref(e)))
+            sl = torch.empty((nIter, e.value), dtype=torch.complex64, device=x.device)
+            ind = sorted(i for i in set(witnessInfo.witness_ind) if 1 <= i <= nIter)
+            full = torch.empty((len(ind), x.numel()), dtype=torch.complex64, device=x.device) if ind else None
+            ia = (C.c_int * max(len(ind), 1))(*ind)
+            check(lib.mona_plan_set_witness(self._h, ptr(sl), int(nIter), ptr(full) if ind else None,
+                                            ia if ind else None, len(ind)))
+        try:
+            out = run()
+        finally:
+            if watch:
+                check(lib.mona_plan_set_witness(self._h, None, 0, None, None, 0))
+        hist = out[-1] if isinstance(out, tuple) else out
+        witnessInfo.param.update(params)
+        witnessInfo.record(np.asarray(hist, dtype=np.float64).reshape(nIter, -1), names)
+        if watch:
+            torch.cuda.synchronize()
+            slh = sl.cpu().numpy()
+            fh = full.cpu().numpy() if ind else None
+            for c in range(1, nIter + 1):
+                xc = fh[ind.index(c)] if c in ind else None
+                witnessInfo.watch(c, xc, self.n_u, "loop", image=slh[c - 1])
+            witnessInfo.watch(nIter, x, self.n_u, "final")
+        return out
+
     def autotune(self, f=0, stream=None):
         """Times forward + adjoint of frame f for each FFT pass variant, keeps
         the fastest and returns the register kernel's lines per tile (or minus
@@ -178,18 +216,24 @@ class Plan:
         return out
 
     # -- solvers -----------------------------------------------------------
-    def bmSensa(self, x, y, ve, nCGD, ve_max=None, nIter=1, f=0, stream=None):
+    def bmSensa(self, x, y, ve, nCGD, ve_max=None, nIter=1, f=0, stream=None, witnessInfo=None):
         """CG-SENSE in place on x (bmSensa.m:6-96); returns the per-outer-
-        iteration residuals <y - Mx, HY (y - Mx)>."""
+        iteration residuals <y - Mx, HY (y - Mx)>.  witnessInfo: a
+        bmWitnessInfo that receives the residuals and the witness images."""
         nIter = n_iter(nIter)
         _arg(x, "x", self.nu)
         _arg(y, "y", self.nPt(f) * self.nCh)
         _arg(ve, "ve", self.nPt(f), real=True)
-        res = (C.c_double * max(nIter, 1))()
-        check(lib.mona_sensa(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
-                             float(ve_max) if ve_max else 0.0, int(nIter), int(nCGD), res,
-                             stream_of(stream)))
-        return list(res)[:nIter]
+
+        def run():
+            res = (C.c_double * max(nIter, 1))()
+            check(lib.mona_sensa(self._h, int(f), ptr(x), ptr(y), C.cast(ptr(ve), C.POINTER(C.c_float)),
+                                 float(ve_max) if ve_max else 0.0, int(nIter), int(nCGD), res,
+                                 stream_of(stream)))
+            return list(res)[:nIter]
+        return self._watched(witnessInfo, nIter, x, run, ["residu"],
+                             {"recon_name": "sensa", "dK_u": self.dK_u, "N_u": self.N_u, "n_u": self.n_u,
+                              "nCGD": int(nCGD), "ve_max": ve_max})
 
     def bmSensa_batch(self, x, y_list, ve_list, nCGD, ve_max=None, nIter=1, stream=None):
         """bmSensa on frames 0 .. nFr-1 at once (batched operators, per-frame
@@ -264,7 +308,7 @@ class Plan:
         check(lib.mona_nakatsha_bf16(self._h, int(f), ptr(y), ptr(out), stream_of(stream)))
         return out
 
-    def bmSteva(self, x, z, u, y, ve, delta, rho, nCGD, ve_max=None, nIter=1, f=0, stream=None):
+    def bmSteva(self, x, z, u, y, ve, delta, rho, nCGD, ve_max=None, nIter=1, f=0, stream=None, witnessInfo=None):
         """Spatial-TV ADMM in place on x, z, u (bmSteva.m:6-138).  z, u are
         [imDim, nu] complex64 (given) or None (z = grad x, u = 0)."""
         nIter = n_iter(nIter)
@@ -280,15 +324,19 @@ class Plan:
         _arg(u, "u", self.imDim * self.nu)
         d = (C.c_float * 2)(*_sched(delta))
         r = (C.c_float * 2)(*_sched(rho))
-        hist = (C.c_double * (2 * max(nIter, 1)))()
-        check(lib.mona_steva(self._h, int(f), ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(y),
-                             C.cast(ptr(ve), C.POINTER(C.c_float)), float(ve_max) if ve_max else 0.0,
-                             d, r, int(nIter), int(nCGD), hist, stream_of(stream)))
-        h = np.array(list(hist)[:2 * nIter]).reshape(nIter, 2)
-        return z, u, h
+
+        def run():
+            hist = (C.c_double * (2 * max(nIter, 1)))()
+            check(lib.mona_steva(self._h, int(f), ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(y),
+                                 C.cast(ptr(ve), C.POINTER(C.c_float)), float(ve_max) if ve_max else 0.0,
+                                 d, r, int(nIter), int(nCGD), hist, stream_of(stream)))
+            return z, u, np.array(list(hist)[:2 * nIter]).reshape(nIter, 2)
+        return self._watched(witnessInfo, nIter, x, run, ["residu", "total_variation"],
+                             {"recon_name": "steva", "dK_u": self.dK_u, "N_u": self.N_u, "n_u": self.n_u,
+                              "delta": delta, "rho": rho, "nCGD": int(nCGD), "ve_max": ve_max})
 
     def bmTevaMorphosia_chain(self, x, z, u, y_list, ve_list, delta, rho, nCGD, ve_max, nIter=1,
-                              stream=None, Tu=None, Tut=None):
+                              stream=None, Tu=None, Tut=None, witnessInfo=None):
         """Temporal-TV ADMM over all frames (bmTevaMorphosia_chain.m:6-294),
         regul_mode 'normal'.  x, z, u are [nFr, nu] complex64 (z, u None:
         z = F x, u = 0).  Tu, Tut: lists of deformation matrices per frame
@@ -314,18 +362,25 @@ class Plan:
         voff = (C.c_longlong * nFr)(*[int(v) for v in vo])
         d = (C.c_float * 2)(*_sched(delta))
         r = (C.c_float * 2)(*_sched(rho))
-        hist = (C.c_double * (3 * max(nIter, 1)))()
-        check(lib.mona_teva_chain(self._h, ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(ycat), yoff,
-                                  C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), d, r,
-                                  int(nIter), int(nCGD), hist, stream_of(stream)))
-        h = np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
-        return z, u, h
+
+        def run():
+            hist = (C.c_double * (3 * max(nIter, 1)))()
+            check(lib.mona_teva_chain(self._h, ptr(x), ptr(z), ptr(u), 1 if given else 0, ptr(ycat), yoff,
+                                      C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), d, r,
+                                      int(nIter), int(nCGD), hist, stream_of(stream)))
+            return z, u, np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
+        return self._watched(witnessInfo, nIter, x, run, ["objective", "dafi", "regul"],
+                             {"recon_name": "tevaMorphosia_chain", "dK_u": self.dK_u, "N_u": self.N_u,
+                              "n_u": self.n_u, "delta": delta, "rho": rho, "nCGD": int(nCGD), "ve_max": ve_max})
 
     # -- the other Morphosia solvers (mona_morphosia) ------------------------
     MORPH_TEVA_CHAIN, MORPH_TEVA_DUO_CHAIN, MORPH_TEVA_SHEET = 0, 1, 2
     MORPH_SENSITIVA_CHAIN, MORPH_SENSITIVA_SHEET, MORPH_SENSITIVA_DUO_CHAIN = 3, 4, 5
+    MORPH_NAMES = ("tevaMorphosia_chain", "tevaDuoMorphosia_chain", "tevaMorphosia_sheet",
+                   "sensitivaMorphosia_chain", "sensitivaMorphosia_sheet", "sensitivaDuoMorphosia_chain")
 
-    def _morph(self, solver, nFr1, x, zs, us, y_list, ve_list, delta, rho, nCGD, ve_max, nIter, stream, nT):
+    def _morph(self, solver, nFr1, x, zs, us, y_list, ve_list, delta, rho, nCGD, ve_max, nIter, stream, nT,
+               witnessInfo=None):
         nIter = n_iter(nIter)
         import torch
         nFr = x.shape[0]
@@ -351,17 +406,22 @@ class Plan:
         voff = (C.c_longlong * nFr)(*[int(v) for v in vo])
         d = (C.c_float * 2)(*_sched(delta))
         r = (C.c_float * 2)(*_sched(rho)) if l1 else None
-        hist = (C.c_double * (3 * max(nIter, 1)))()
-        check(lib.mona_morphosia(self._h, int(solver), int(nFr1), ptr(x), ptr(z), ptr(u), 1 if given else 0,
-                                 ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), d, r,
-                                 int(nIter), int(nCGD), hist, stream_of(stream)))
-        h = np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
+
+        def run():
+            hist = (C.c_double * (3 * max(nIter, 1)))()
+            check(lib.mona_morphosia(self._h, int(solver), int(nFr1), ptr(x), ptr(z), ptr(u), 1 if given else 0,
+                                     ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max),
+                                     d, r, int(nIter), int(nCGD), hist, stream_of(stream)))
+            return np.array(list(hist)[:3 * nIter]).reshape(nIter, 3)
+        h = self._watched(witnessInfo, nIter, x, run, ["objective", "dafi", "regul"],
+                          {"recon_name": self.MORPH_NAMES[solver], "dK_u": self.dK_u, "N_u": self.N_u,
+                           "n_u": self.n_u, "delta": delta, "rho": rho, "nCGD": int(nCGD), "ve_max": ve_max})
         if not l1:
             return h
         return [z[k] for k in range(nT)], [u[k] for k in range(nT)], h
 
     def bmTevaDuoMorphosia_chain(self, x, z1, z2, u1, u2, y_list, ve_list, delta, rho, nCGD, ve_max, nIter=1,
-                                 Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+                                 Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None, witnessInfo=None):
         """bmTevaDuoMorphosia_chain.m:6-242: two half-weighted temporal
         differences, 0.5 (Tu1_i x_i - x_{i-1}) and 0.5 (Tu2_i x_i - x_{i+1}).
         x [nFr, nu] complex64 is updated in place; z1..u2 None start from
@@ -371,11 +431,11 @@ class Plan:
         self._set_deforms(nFr, 1, Tu2, Tu2t)
         zs, us, h = self._morph(self.MORPH_TEVA_DUO_CHAIN, None, x, None if z1 is None else (z1, z2),
                                 None if u1 is None else (u1, u2), y_list, ve_list, delta, rho, nCGD, ve_max, nIter,
-                                stream, 2)
+                                stream, 2, witnessInfo)
         return zs[0], zs[1], us[0], us[1], h
 
     def bmTevaMorphosia_sheet(self, x, z1, z2, u1, u2, y_list, ve_list, nFr_1, delta, rho, nCGD, ve_max, nIter=1,
-                              Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+                              Tu1=None, Tu1t=None, Tu2=None, Tu2t=None, stream=None, witnessInfo=None):
         """bmTevaMorphosia_sheet.m:16-244: frames on an nFr_1 x nFr_2 cell
         array, frame k = i + nFr_1*j (the y(:) order of x, y_list, ve_list
         and the deformation lists); differences along i (Tu1) and j (Tu2).
@@ -385,38 +445,39 @@ class Plan:
         self._set_deforms(nFr, 1, Tu2, Tu2t)
         zs, us, h = self._morph(self.MORPH_TEVA_SHEET, nFr_1, x, None if z1 is None else (z1, z2),
                                 None if u1 is None else (u1, u2), y_list, ve_list, delta, rho, nCGD, ve_max, nIter,
-                                stream, 2)
+                                stream, 2, witnessInfo)
         return zs[0], zs[1], us[0], us[1], h
 
     def bmSensitivaMorphosia_chain(self, x, y_list, ve_list, delta, nCGD, ve_max, nIter=1, Tu=None, Tut=None,
-                                   stream=None):
+                                   stream=None, witnessInfo=None):
         """bmSensitivaMorphosia_chain.m:6-184: quadratic temporal penalty
         delta*HZ*||Tu_i x_i - x_{i-1}||^2; x updated in place; returns hist."""
         self._set_deforms(x.shape[0], 0, Tu, Tut)
         return self._morph(self.MORPH_SENSITIVA_CHAIN, None, x, None, None, y_list, ve_list, delta, None, nCGD,
-                           ve_max, nIter, stream, 1)
+                           ve_max, nIter, stream, 1, witnessInfo)
 
     def bmSensitivaMorphosia_sheet(self, x, y_list, ve_list, nFr_1, delta, nCGD, ve_max, nIter=1, Tu1=None,
-                                   Tu1t=None, Tu2=None, Tu2t=None, stream=None):
+                                   Tu1t=None, Tu2=None, Tu2t=None, stream=None, witnessInfo=None):
         """bmSensitivaMorphosia_sheet.m:6-187: the quadratic penalty along
         both cell-array axes (frame k = i + nFr_1*j); returns hist."""
         nFr = x.shape[0]
         self._set_deforms(nFr, 0, Tu1, Tu1t)
         self._set_deforms(nFr, 1, Tu2, Tu2t)
         return self._morph(self.MORPH_SENSITIVA_SHEET, nFr_1, x, None, None, y_list, ve_list, delta, None, nCGD,
-                           ve_max, nIter, stream, 2)
+                           ve_max, nIter, stream, 2, witnessInfo)
 
     def bmSensitivaDuoMorphosia_chain(self, x, y_list, ve_list, delta, nCGD, ve_max, nIter=1, Tu1=None, Tu1t=None,
-                                      Tu2=None, Tu2t=None, stream=None):
+                                      Tu2=None, Tu2t=None, stream=None, witnessInfo=None):
         """bmSensitivaDuoMorphosia_chain.m:6-108: the quadratic penalty on
         0.5 (Tu1_i x_i - x_{i-1}) and 0.5 (Tu2_i x_i - x_{i+1}); returns hist."""
         nFr = x.shape[0]
         self._set_deforms(nFr, 0, Tu1, Tu1t)
         self._set_deforms(nFr, 1, Tu2, Tu2t)
         return self._morph(self.MORPH_SENSITIVA_DUO_CHAIN, None, x, None, None, y_list, ve_list, delta, None, nCGD,
-                           ve_max, nIter, stream, 2)
+                           ve_max, nIter, stream, 2, witnessInfo)
 
-    def bmSensaMorphosia_chain(self, x, y_list, ve_list, nCGD, ve_max, nIter=1, Tu=None, Tut=None, stream=None):
+    def bmSensaMorphosia_chain(self, x, y_list, ve_list, nCGD, ve_max, nIter=1, Tu=None, Tut=None, stream=None,
+                               witnessInfo=None):
         """bmSensaMorphosia_chain.m:6-104: one image x [nu] (complex64,
         updated in place) seen by every frame through Tu_f; returns the
         residual R after each iteration."""
@@ -430,10 +491,15 @@ class Plan:
         vcat = torch.cat([vv.reshape(-1) for vv in ve_list])
         yoff = (C.c_longlong * nFr)(*[int(v) for v in np.cumsum([0] + [yy.numel() for yy in y_list])[:-1]])
         voff = (C.c_longlong * nFr)(*[int(v) for v in np.cumsum([0] + [vv.numel() for vv in ve_list])[:-1]])
-        hist = (C.c_double * max(nIter, 1))()
-        check(lib.mona_sensa_morphosia(self._h, ptr(x), ptr(ycat), yoff, C.cast(ptr(vcat), C.POINTER(C.c_float)),
-                                       voff, float(ve_max), int(nIter), int(nCGD), hist, stream_of(stream)))
-        return np.array(list(hist)[:nIter])
+        def run():
+            hist = (C.c_double * max(nIter, 1))()
+            check(lib.mona_sensa_morphosia(self._h, ptr(x), ptr(ycat), yoff,
+                                           C.cast(ptr(vcat), C.POINTER(C.c_float)), voff, float(ve_max), int(nIter),
+                                           int(nCGD), hist, stream_of(stream)))
+            return np.array(list(hist)[:nIter])
+        return self._watched(witnessInfo, nIter, x, run, ["residu"],
+                             {"recon_name": "sensaMorphosia_chain", "dK_u": self.dK_u, "N_u": self.N_u,
+                              "n_u": self.n_u, "nCGD": int(nCGD), "ve_max": ve_max})
 
     def bmNasha(self, y, Gn: BmSparseMat, Cmaps=None, stream=None):
         """Gridded recon (bmNasha.m:6-91)."""
