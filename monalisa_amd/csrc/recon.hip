@@ -952,7 +952,20 @@ int64_t Plan::fft_scratch_elems() const {
         for (int d = 0; d < imDim; d++) v *= (d <= a) ? N[d] : n[d];
         mx = std::max(mx, v);
     }
+    if (grid_yz()) mx = std::max(mx, (int64_t)N[1] * mid_rs());
     return mx;
+}
+
+// Row stride along y of the y-major middle intermediate [N_y][n_z][N_x nCh]:
+// its n_z N_x nCh values plus a pad (MONA_FFT_MID_PAD complex values,
+// default 32 = 256 bytes), so that the y pass's rows are not an exact
+// multiple of a large power of two apart.
+int64_t Plan::mid_rs() const {
+    static const int64_t pad = [] {
+        const char* e = std::getenv("MONA_FFT_MID_PAD");
+        return e ? std::max<int64_t>(0, std::atoll(e)) : (int64_t)32;
+    }();
+    return (int64_t)n[2] * N[0] * nCh + pad;
 }
 
 void Plan::ensure_fft_scratch() {
@@ -1007,14 +1020,14 @@ void Plan::forward_fft_pruned(const float2* x, float2* gr, int nb, DevBuf& ba, D
             p.out_ival = mid_ival;
             p.mask_nch = nCh;
         }
-        if (yz && a == 1) {  // out per frame [N_y][n_z][S]; line o = frame * n_z + z
+        if (yz && a == 1) {  // out per frame [N_y][n_z][S] (y rows mid_rs apart); line o = frame * n_z + z
             p.out_on = n[2];
-            p.out_os2 = (int64_t)N[1] * n[2] * S;
+            p.out_os2 = (int64_t)N[1] * mid_rs();
             p.out_os = S;
-            p.out_rs = (int64_t)n[2] * S;
+            p.out_rs = mid_rs();
         }
         if (yz && a == 2) {  // in per frame [N_y][n_z][S], out the grid [N_y][N_z][S]
-            p.in_os = (int64_t)n[2] * S;
+            p.in_os = mid_rs();
             p.in_rs = S;
             p.out_os = (int64_t)N[2] * S;
             p.out_rs = S;
@@ -1074,17 +1087,17 @@ void Plan::adjoint_fft_pruned(const float2* gr, float2* x, bool combine, float s
             p.in_ival = mid_ival;
             p.mask_nch = nCh;
         }
-        if (yz && a == 2) {  // in the grid [N_y][N_z][S], out per frame [N_y][n_z][S]
+        if (yz && a == 2) {  // in the grid [N_y][N_z][S], out per frame [N_y][n_z][S] (y rows mid_rs apart)
             p.in_os = (int64_t)N[2] * S;
             p.in_rs = S;
-            p.out_os = (int64_t)n[2] * S;
+            p.out_os = mid_rs();
             p.out_rs = S;
         }
         if (yz && a == 1) {  // in per frame [N_y][n_z][S]; line o = frame * n_z + z
             p.in_on = n[2];
-            p.in_os2 = (int64_t)N[1] * n[2] * S;
+            p.in_os2 = (int64_t)N[1] * mid_rs();
             p.in_os = S;
-            p.in_rs = (int64_t)n[2] * S;
+            p.in_rs = mid_rs();
         }
         if (a == 0) {
             p.mode = combine ? 2 : 3;

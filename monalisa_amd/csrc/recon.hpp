@@ -118,6 +118,7 @@ struct Plan {
     const int2* ivals_of(int f) const;
     const int2* mid_ivals_of(int f) const;
     int64_t fft_scratch_elems() const;
+    int64_t mid_rs() const;  // y row stride of the 3-D y-major middle intermediate
     void ensure_fft_scratch();
 
     // Batched form over all frames: one block-diagonal gridding launch and
