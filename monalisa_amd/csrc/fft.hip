@@ -480,13 +480,18 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int64_t obase = line_off(o, a.out_os, a.out_on, a.out_os2);
     const float sg = a.inverse ? 1.0f : -1.0f;
 
-    // masked passes: the bits of this work-item's column, in registers
+    // masked passes: the bits of this work-item's column, reduced at once to
+    // the bits of the rows it loads (j + N2 n1, bit n1 of lbits) and stores
+    // (j + N1 k2, bit k2 of sbits), with compile-time word indices only (an
+    // indexed mask array went to scratch memory, whose loads wait for every
+    // load in flight)
     constexpr int MW = (N + 31) / 32;
-    uint32_t mw[MW];
+    uint32_t lbits = ~0u, sbits = ~0u;
     const uint32_t* mask = MODE == 0 ? (a.in_mask ? a.in_mask : a.out_mask) : nullptr;
     if (mask && live) {
         // a column's words are MW * 4 bytes at a multiple of that: vector loads
         const uint32_t* mp = mask + (o * a.mask_cpl + ws / a.mask_nch) * MW;
+        uint32_t mw[MW];
         if constexpr (MW % 4 == 0) {
 #pragma unroll
             for (int i = 0; i < MW / 4; i++) {
@@ -507,16 +512,24 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
 #pragma unroll
             for (int i = 0; i < MW; i++) mw[i] = mp[i];
         }
-    } else {
+        lbits = sbits = 0u;
 #pragma unroll
-        for (int i = 0; i < MW; i++) mw[i] = ~0u;
+        for (int n1 = 0; n1 < N1; n1++) {
+            const int row = j + N2 * n1;
+            uint32_t wv = 0u;
+#pragma unroll
+            for (int i = 0; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
+            lbits |= ((wv >> (row & 31)) & 1u) << n1;
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+            const int row = j + N1 * k2;
+            uint32_t wv = 0u;
+#pragma unroll
+            for (int i = 0; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
+            sbits |= ((wv >> (row & 31)) & 1u) << k2;
+        }
     }
-    auto bit = [&](int row) {  // row < N; word picked by selects (no indexed registers)
-        uint32_t wv = mw[0];
-#pragma unroll
-        for (int i = 1; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
-        return (wv >> (row & 31)) & 1u;
-    };
     // interval passes: this work-item's column keeps rows [ilo, ihi); a dead
     // column (-1, -1) is neither loaded nor stored
     int ilo = 0, ihi = N;
@@ -543,7 +556,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
                 const int row = j + N2 * n1;
                 bool ok = take;
                 if (a.in_ival) ok = ok && row >= ilo && row < ihi;
-                if (a.in_mask) ok = ok && bit(row);
+                if (a.in_mask) ok = ok && ((lbits >> n1) & 1u);
                 if (ok) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
             } else if (take) {
                 if (MODE == 1) {
@@ -657,7 +670,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         const int k = j + N1 * k2;
         const int kl = k - a.out_off;
         bool put = kl >= 0 && kl < a.out_cnt;
-        if (MODE == 0 && a.out_mask) put = put && bit(k);
+        if (MODE == 0 && a.out_mask) put = put && ((sbits >> k2) & 1u);
         if (MODE == 0 && a.out_ival) put = put && k >= ilo && k < ihi;
         if (put) {
             float sc = a.scale;
