@@ -169,8 +169,9 @@ def bench_binned(args, cfg):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, x
 
-    # the tuned FFT variant serves both forms, so they compute the same sums
-    tile = P.autotune(0)
+    # the tuned FFT variant and adjoint-gridding group sizes serve both
+    # forms, so they compute the same sums
+    tile, groups = P.autotune(0, groups=True)
     P.set_options(batch_frames=False, graph=False)
     s_plain, x_plain = timed(max(1, args.steps // 2), args.warmup)
     P.set_options(batch_frames=True, graph=True)
@@ -205,6 +206,7 @@ def bench_binned(args, cfg):
         "image_rel_err_vs_phantom": img_err,
         "gridding_matrix_build_s": prob["build_s"],
         "fft_tile_autotuned": tile,
+        "adjoint_groups_autotuned": {"stage_kib": groups[0], "rows": groups[1]},
         "points_per_bin": [len(b) * cfg["nSample"] for b in prob["bins"]],
     }
     print(json.dumps(out), flush=True)

@@ -424,6 +424,12 @@ int mona_plan_witness_elems(mona_plan P, long long* elems);
  * and keeps the fastest; *chosen (may be NULL) receives the register kernel's
  * lines per tile, or minus the LDS kernel's tile. */
 int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* chosen);
+/* Autotuner of the staged adjoint gridding (no reference counterpart): times
+ * the weighted adjoint gridding of one frame for several group sizes (LDS
+ * budget of the staged k-space rows in KiB, rows per group), keeps the
+ * fastest, regroups every frame with it and reports it (either pointer may
+ * be NULL).  Results do not depend on the group sizes. */
+int mona_plan_autotune_groups(mona_plan P, int frame, hipStream_t stream, int* stage_kb, int* rows);
 
 /* Kernel timing: when enabled, every gridding launch (forward Gu product,
  * adjoint Gut product) records HIP events on its stream; the accumulated

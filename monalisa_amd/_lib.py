@@ -83,6 +83,7 @@ SIGNATURES = {
     "mona_nakatsha_bf16": (I, [P, I, P, P, P]),
     "mona_plan_set_option": (I, [P, I, LL]),
     "mona_plan_set_witness": (I, [P, P, I, P, C.POINTER(C.c_int), I]),
+    "mona_plan_autotune_groups": (I, [P, I, P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mona_plan_witness_elems": (I, [P, C.POINTER(C.c_longlong)]),
     "mona_plan_autotune": (I, [P, I, P, LLP]),
     "mona_timing_enable": (I, [I]),

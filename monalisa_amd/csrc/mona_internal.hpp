@@ -245,9 +245,15 @@ void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector
 void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s);
 // Builds S's staged groups for products over n_vec vectors from the decoded
 // host arrays (per-row counts, columns); no groups when n_vec is too large
-// for a group to hold the longest short row.
+// for a group to hold the longest short row.  stage_kb / entries / rows:
+// LDS budget of a group's vectors (0 = no groups), most entries and rows of
+// a group; negative = the defaults (MONA_STAGE_KB, MONA_GROUP_ENTRIES,
+// MONA_GROUP_ROWS or 16, 1024, 256).
 void build_groups(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
-                  hipStream_t s);
+                  hipStream_t s, int stage_kb = -1, int entries = -1, int rows = -1);
+// The same from S's own device arrays (rebuilds the groups of an uploaded
+// operator matrix with other sizes).
+void rebuild_groups(SparseDev& S, int n_vec, hipStream_t s, int stage_kb, int entries, int rows);
 
 // Rows with more entries than this are summed in chunks by the operator path.
 constexpr int kLongRow = 64;

@@ -793,7 +793,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     auto Gut_s = std::make_shared<SparseDev>();
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
     Gut_s->covers_all = !sparse_frame && !use_ival;
-    build_groups(*Gut_s, dnj, dcol, nCh, s);
+    build_groups(*Gut_s, dnj, dcol, nCh, s, grp_stage_kb, grp_entries, grp_rows);
     // cell masks: cells the forward gridding reads (Gu columns) and cells the
     // adjoint gridding writes (Gut rows with entries); equal for a
     // transposed pair.  Layout of the pass along the last axis: for each
@@ -1983,6 +1983,71 @@ int mona_plan_autotune(mona_plan P, int frame, hipStream_t stream, long long* ch
         p.fft_tile = win.tile;
         p.fft_lines = win.lines;
         if (chosen) *chosen = win.lines > 0 ? win.lines : -(long long)win.tile;
+    });
+}
+
+void Plan::regroup(hipStream_t s) {
+    std::vector<const SparseDev*> done;
+    for (Frame& fr : frames) {
+        if (!fr.Gut_s || std::find(done.begin(), done.end(), fr.Gut_s.get()) != done.end()) continue;
+        rebuild_groups(*fr.Gut_s, nCh, s, grp_stage_kb, grp_entries, grp_rows);
+        done.push_back(fr.Gut_s.get());
+    }
+    Gu_b.reset();  // the batched matrices stack the new groups on next use
+    Gut_b.reset();
+    koff_b.clear();
+}
+
+// Times the weighted adjoint gridding of frame `frame` for several group
+// sizes of its staged form (LDS budget of the k-space rows, rows per group)
+// and keeps the fastest for every frame; two timed repetitions after one
+// warm-up each.
+int mona_plan_autotune_groups(mona_plan P, int frame, hipStream_t stream, int* stage_kb, int* rows) {
+    return guarded([&] {
+        MONA_CHECK(P, MONA_ERR_ARG, "NULL plan");
+        Plan& p = P->P;
+        const Frame& fr = p.frame(frame);
+        DevBuf xo((size_t)p.nu * sizeof(float2)), yk((size_t)fr.P * p.nCh * sizeof(float2)),
+            hy((size_t)std::max<int64_t>(fr.P, 1) * sizeof(float));
+        fill_f32(yk.as<float>(), 1.0f, fr.P * p.nCh * 2, stream);
+        fill_f32(hy.as<float>(), 1.0f, fr.P, stream);
+        hipEvent_t a, b;
+        MONA_HIP(hipEventCreate(&a));
+        MONA_HIP(hipEventCreate(&b));
+        struct Variant { int kb, rows; };
+        const Variant vars[] = {{16, 256}, {12, 256}, {24, 256}, {16, 128}, {16, 512}};
+        float best = 1e30f;
+        Variant win = vars[0];
+        const bool staged = p.use_staged;
+        p.use_staged = true;
+        try {
+            for (const Variant& v : vars) {
+                rebuild_groups(*fr.Gut_s, p.nCh, stream, v.kb, p.grp_entries, v.rows);
+                if (fr.Gut_s->n_group == 0) continue;
+                p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, hy.as<float>(), 1.0f, stream);
+                MONA_HIP(hipEventRecord(a, stream));
+                for (int r = 0; r < 2; r++)
+                    p.adjoint(frame, yk.as<float2>(), xo.as<float2>(), true, hy.as<float>(), 1.0f, stream);
+                MONA_HIP(hipEventRecord(b, stream));
+                MONA_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                MONA_HIP(hipEventElapsedTime(&ms, a, b));
+                if (ms < best) { best = ms; win = v; }
+            }
+        } catch (...) {
+            p.use_staged = staged;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            throw;
+        }
+        p.use_staged = staged;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        p.grp_stage_kb = win.kb;
+        p.grp_rows = win.rows;
+        p.regroup(stream);
+        if (stage_kb) *stage_kb = win.kb;
+        if (rows) *rows = win.rows;
     });
 }
 

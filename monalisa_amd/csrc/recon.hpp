@@ -135,6 +135,10 @@ struct Plan {
     // frames of a batch); off = the stored matrices everywhere
     bool use_mf = true;
     bool use_staged = true;            // adjoint gridding from LDS-staged groups (Gut_s groups)
+    // group sizes of the staged adjoint gridding (-1: the defaults); set by
+    // the autotuner, used by set_frame
+    int grp_stage_kb = -1, grp_entries = -1, grp_rows = -1;
+    void regroup(hipStream_t s);       // every frame's Gut_s with the current sizes
     bool mf_kernel_ok = false;
     MfKernel mfk;
     MfGeo mf_geo() const;

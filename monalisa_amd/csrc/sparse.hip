@@ -717,30 +717,31 @@ int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
 }
-int64_t stage_bytes() {
-    static const int64_t b = (int64_t)env_int("MONA_STAGE_KB", 16) * 1024;
+int stage_kb_default() {
+    static const int b = env_int("MONA_STAGE_KB", 16);
     return b;
 }
-int group_entries() {
-    static const int n = std::max(env_int("MONA_GROUP_ENTRIES", 1024), kLongRow);
+int group_entries_default() {
+    static const int n = env_int("MONA_GROUP_ENTRIES", 1024);
     return n;
 }
-int group_rows() {
-    static const int n = std::max(env_int("MONA_GROUP_ROWS", 256), 1);
+int group_rows_default() {
+    static const int n = env_int("MONA_GROUP_ROWS", 256);
     return n;
 }
 constexpr int kGroupMaxCols = 1024;
 }  // namespace
 
 void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<int>& col, int n_vec,
-                  hipStream_t s) {
+                  hipStream_t s, int stage_kb, int entries, int max_rows) {
     S.n_group = S.group_vec = S.group_cap = S.group_ecap = S.group_rcap = 0;
     S.n_group_col = 0;
     S.group.release();
     S.group_col.release();
     S.slot.release();
     if (n_vec <= 0 || S.nnz == 0) return;
-    const int cap = (int)std::min<int64_t>(kGroupMaxCols, stage_bytes() / ((int64_t)(n_vec + 1) * sizeof(float2)));
+    const int64_t stage = (int64_t)(stage_kb < 0 ? stage_kb_default() : stage_kb) * 1024;
+    const int cap = (int)std::min<int64_t>(kGroupMaxCols, stage / ((int64_t)(n_vec + 1) * sizeof(float2)));
     if (cap < kLongRow) return;  // a short row's columns must fit one group
     const int rows = (int)nj.size();
     MONA_CHECK((int64_t)col.size() == S.nnz, MONA_ERR_INTERNAL, "group build: entry count mismatch");
@@ -749,7 +750,8 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     std::vector<int> mark(S.r_size, -1), seen(S.r_size, -1), pos(S.r_size, 0), cur, gcol;
     std::vector<int4> groups;
     std::vector<uint16_t> slot(S.nnz, 0);
-    const int ecap = group_entries(), rcap = group_rows();
+    const int ecap = std::max(entries < 0 ? group_entries_default() : entries, kLongRow);
+    const int rcap = std::max(max_rows < 0 ? group_rows_default() : max_rows, 1);
     int g = 0, r0 = 0;
     // closes the group of rows [r0, r1): columns ascending, entry slots
     auto close = [&](int r1) {
@@ -805,6 +807,20 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     S.group_ecap = ecap;
     S.group_rcap = rcap;
     S.n_group_col = (int64_t)gcol.size();
+}
+
+void rebuild_groups(SparseDev& S, int n_vec, hipStream_t s, int stage_kb, int entries, int rows) {
+    MONA_CHECK(S.on_device && (S.cv.p || S.nnz == 0), MONA_ERR_INTERNAL, "regrouping needs a packed device matrix");
+    std::vector<int64_t> rp((size_t)S.l_nJump + 1);
+    std::vector<SpEntry> cv((size_t)S.nnz);
+    MONA_HIP(hipMemcpyAsync(rp.data(), S.rowptr.p, rp.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (S.nnz > 0)
+        MONA_HIP(hipMemcpyAsync(cv.data(), S.cv.p, cv.size() * sizeof(SpEntry), hipMemcpyDeviceToHost, s));
+    MONA_HIP(hipStreamSynchronize(s));
+    std::vector<int> nj((size_t)S.l_nJump), col((size_t)S.nnz);
+    for (int r = 0; r < S.l_nJump; r++) nj[r] = (int)(rp[r + 1] - rp[r]);
+    for (int64_t k = 0; k < S.nnz; k++) col[k] = cv[k].col;
+    build_groups(S, nj, col, n_vec, s, stage_kb, entries, rows);
 }
 
 void sparse_from_decoded(SparseDev& S, int r_size, int l_size, const std::vector<int>& col,

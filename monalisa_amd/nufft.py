@@ -172,13 +172,24 @@ class Plan:
             witnessInfo.watch(nIter, x, self.n_u, "final")
         return out
 
-    def autotune(self, f=0, stream=None):
+    def autotune(self, f=0, stream=None, groups=False):
         """Times forward + adjoint of frame f for each FFT pass variant, keeps
         the fastest and returns the register kernel's lines per tile (or minus
-        the LDS kernel's tile)."""
+        the LDS kernel's tile).  groups: then also the group sizes of the
+        staged adjoint gridding; returns (lines, (stage KiB, rows))."""
         chosen = C.c_longlong()
         check(lib.mona_plan_autotune(self._h, int(f), stream_of(stream), C.byref(chosen)))
-        return chosen.value
+        if not groups:
+            return chosen.value
+        return chosen.value, self.autotune_groups(f, stream)
+
+    def autotune_groups(self, f=0, stream=None):
+        """Times the weighted adjoint gridding of frame f for several group
+        sizes of its staged form, regroups every frame with the fastest and
+        returns it as (stage KiB, rows per group); results are unchanged."""
+        kb, rows = C.c_int(), C.c_int()
+        check(lib.mona_plan_autotune_groups(self._h, int(f), stream_of(stream), C.byref(kb), C.byref(rows)))
+        return kb.value, rows.value
 
     def K(self):
         import torch

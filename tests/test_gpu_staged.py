@@ -123,3 +123,40 @@ def test_staged_adjoint_batched():
     np.testing.assert_array_equal(sb, sf)
     np.testing.assert_array_equal(sb, pb)
     np.testing.assert_array_equal(pb, pf)
+
+
+def test_group_autotune_keeps_results():
+    """The group-size autotuner regroups every frame's adjoint gridding; the
+    adjoint of each frame and the batched CG-SENSE keep their bits, and the
+    batched form restacks the new groups."""
+    import torch
+    from monalisa_amd import synth
+    from monalisa_amd.nufft import Plan
+    from monalisa_amd.sparse import bmTraj2SparseMat
+    print("[gpu] group autotune", flush=True)
+    rng = np.random.default_rng(12)
+    n_u, N_u, dK_u, nCh = [16, 16, 16], [32, 32, 32], [0.5] * 3, 20
+    P = Plan(N_u, n_u, dK_u, nCh, "kaiser")
+    P.set_coils(torch.from_numpy(synth.coil_maps(n_u, nCh, rng)).cuda())
+    t_all = synth.koosh_ball(10, 12, 32, k_max=8.0)
+    ve_all = synth.ve_radial3(t_all, 32, 120)
+    for f, (a, b) in enumerate(((0, 60), (60, 120))):
+        sl = slice(a * 32, b * 32)
+        G = bmTraj2SparseMat(t_all[:, sl], ve_all[sl], N_u, dK_u, "kaiser")
+        P.set_frame(f, G["Gu"], G["Gut"])
+    ys = [torch.from_numpy((rng.normal(size=(P.nPt(f), nCh)) + 1j * rng.normal(size=(P.nPt(f), nCh)))
+                           .astype(np.complex64)).cuda() for f in range(2)]
+    ves = [torch.from_numpy(np.abs(rng.normal(size=P.nPt(f))).astype(np.float32) + 0.1).cuda() for f in range(2)]
+
+    def run():
+        adj = [P.bmNakatsha(ys[f], f=f).cpu().numpy() for f in range(2)]
+        xb = torch.zeros((2, P.nu), dtype=torch.complex64, device="cuda")
+        P.bmSensa_batch(xb, ys, ves, 2)
+        return adj, xb.cpu().numpy()
+    adj0, xb0 = run()
+    kb, rows = P.autotune_groups(0)
+    assert kb in (12, 16, 24) and rows in (128, 256, 512)
+    adj1, xb1 = run()
+    for f in range(2):
+        np.testing.assert_array_equal(adj1[f], adj0[f])
+    np.testing.assert_array_equal(xb1, xb0)
