@@ -161,7 +161,11 @@ void launch_u2n(int cpl, const uint32_t* base, const float4* rec, const uint16_t
     const int ppb = std::min(kMfMaxPts, kMfBlock / tpp);
     MONA_CHECK(ppb >= 1, MONA_ERR_ARG, "too many coils for the matrix-free gridding");
     const unsigned nb = (unsigned)((P + ppb - 1) / ppb);
-    constexpr int xg = 16;  // runs of 16 blocks per XCD
+    // runs of xg consecutive blocks per XCD (MONA_XCD_GROUP_MF, default 16)
+    static const int xg = [] {
+        const char* e = std::getenv("MONA_XCD_GROUP_MF");
+        return e ? std::max(1, std::atoi(e)) : 16;
+    }();
     switch (cpl) {
         case 4:
             if constexpr (std::is_same<TG, bf16x2>::value)

@@ -2,6 +2,7 @@
 // of the trajectory instead of the stored entries of Gu.
 #pragma once
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "mona_internal.hpp"

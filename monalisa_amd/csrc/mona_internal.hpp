@@ -190,6 +190,14 @@ std::shared_ptr<const TrajInfo> make_traj_info(int imDim, int nPt, const double*
                                                const double* dK_u, int kernelType, int nWin,
                                                const double* kp, int nParam);
 
+// One segment of the range form: entries [eb, eb + len) of one long row,
+// summed into partial number part.
+struct SpSeg {
+    int64_t eb;
+    int len;
+    int part;
+};
+
 struct SparseDev {
     int r_size = 0, l_size = 0, l_nJump = 0;
     int64_t nnz = 0;
@@ -220,6 +228,20 @@ struct SparseDev {
     DevBuf group_col;         // int32[n_group_col]: the groups' columns back to back
     int64_t n_group_col = 0;
     DevBuf slot;              // uint16[nnz] (entries of long rows unused)
+    // Range form of the long rows (built with the groups; the operator
+    // products then use it instead of the chunks): the columns the long rows
+    // reference are cut into ranges of at most group_cap consecutive
+    // columns; within a range every long row that has entries there owns one
+    // segment (its entries are ascending in column, so a segment is a
+    // contiguous run of entries).  A block stages a range's vectors (one
+    // contiguous run of v) and sums each segment into its partial; each
+    // long row then adds its partials in range order (spmv_c2_combine with
+    // range_first in place of long_first).
+    int n_range = 0, range_vec = 0, range_cap = 0;  // ranges for products over range_vec vectors, columns per range
+    int64_t n_seg = 0, n_rpart = 0;
+    DevBuf range;             // int4[n_range]: first column, column count, first segment, segment count
+    DevBuf seg;               // SpSeg[n_seg], grouped by range
+    DevBuf range_first;       // int32[n_long+1]: first partial of each long row
     // host copy of the reference encoding (kept for export; decoded onto the
     // device on first use)
     std::vector<int> h_r_jump, h_r_nJump, h_l_jump;
@@ -251,6 +273,12 @@ void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s);
 // MONA_GROUP_ROWS or 16, 1024, 256).
 void build_groups(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
                   hipStream_t s, int stage_kb = -1, int entries = -1, int rows = -1);
+// Builds the range form of S's long rows for products over n_vec vectors
+// (a 16 KB stage of consecutive columns, MONA_RANGE_KB overrides it; 0 = no
+// ranges) from the decoded host arrays; none when a long row's columns are
+// not ascending.
+void build_ranges(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
+                  hipStream_t s);
 // The same from S's own device arrays (rebuilds the groups of an uploaded
 // operator matrix with other sizes).
 void rebuild_groups(SparseDev& S, int n_vec, hipStream_t s, int stage_kb, int entries, int rows);
@@ -273,6 +301,8 @@ struct SpView {
     int n_long = 0;
     int64_t n_chunk = 0;
     int n_group = 0;
+    int n_range = 0;
+    int64_t n_rpart = 0;
 };
 // zero_rest = false leaves rows of w that S does not store untouched (for a
 // consumer that never reads them).

@@ -575,6 +575,30 @@ bool yz_interleave() {
 
 bool Plan::grid_yz() const { return imDim == 3 && yz_interleave() && fft_pass_fusable(N[0], nCh); }
 
+// Order of the 8^3 cell tiles in the gridding structures (points of Gu_s,
+// rows of Gut_s): 0 = the grid's storage order (raster over tiles), 1 =
+// Morton order (bits of the three tile coordinates interleaved), so that
+// runs of consecutive tiles are compact in all three axes and neighbouring
+// tiles' shared cells meet in one L2.  MONA_TILE_ORDER overrides it.
+static int tile_order() {
+    static const int o = [] {
+        const char* e = std::getenv("MONA_TILE_ORDER");
+        return e ? std::atoi(e) : 0;
+    }();
+    return o;
+}
+
+// Morton code of tile coordinates (a slowest, then b, then x), 10 bits each
+static int64_t morton3(int64_t a, int64_t b, int64_t x) {
+    int64_t m = 0;
+    for (int i = 0; i < 10; i++) {
+        m |= ((x >> i) & 1) << (3 * i);
+        m |= ((b >> i) & 1) << (3 * i + 1);
+        m |= ((a >> i) & 1) << (3 * i + 2);
+    }
+    return m;
+}
+
 bool grid_intervals() {
     static const bool on = [] {
         const char* e = std::getenv("MONA_GRID_INTERVALS");
@@ -614,16 +638,22 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
             col[k] = (int)c;
         }
     }
-    // tile key of each point: the tile of its first (lowest) cell
-    // (tiles in the grid's storage order: y before z for y-major grids)
+    // tile key of each point: the tile of its first (lowest) cell, tiles
+    // with y slowest, then z, then x (the storage order of the y-major 3-D
+    // grids).  The point order is the same for every grid layout, so the
+    // long rows' range sums, which follow it, are too.
     const bool yzg = grid_yz();
-    const int tx = (N[0] + 7) / 8, ty = (N[1] + 7) / 8, tz = (N[2] + 7) / 8;
+    const int tx = (N[0] + 7) / 8, tz = (N[2] + 7) / 8;
+    const bool morton = tile_order() == 1;
+    // order key of the tile of cell (x, y, z)
+    auto tile_key = [&](int64_t x, int64_t y, int64_t z) -> int64_t {
+        if (morton) return morton3(y >> 3, z >> 3, x >> 3);
+        return ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3);
+    };
     std::vector<int64_t> key(P);
     // A Gu built from a trajectory of this plan's geometry and kernel grids
-    // matrix-free: the points are then ordered by the tile of their base
-    // cell and by its raster position inside the tile (neighbouring points
-    // gather the same cells), and each keeps its base cell, offsets and
-    // row normaliser.
+    // matrix-free: each point keeps its base cell, offsets and row
+    // normaliser.
     TrajPoints tp;
     bool mf = false;
     if (mf_kernel_ok && Gu->traj) {
@@ -634,19 +664,15 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         for (int i = 0; i < T.nParam && mf; i++) mf = (double)(float)T.kp[i] == kparam[i];
         if (mf) traj_points(T, &tp);
     }
+    // key: the tile of the point's lowest cell (its first column), then the
+    // cell's raster position inside the tile (neighbouring points gather the
+    // same cells); the order depends on Gu alone, whether or not the
+    // trajectory came with it
     for (int64_t p = 0; p < P; p++) {
         if (rowptr[p + 1] == rowptr[p]) { key[p] = INT64_MAX; continue; }
-        if (mf) {
-            const int64_t x = tp.base[p * 3], y = tp.base[p * 3 + 1], z = imDim == 3 ? tp.base[p * 3 + 2] : 0;
-            const int64_t tile = yzg ? ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3)
-                                     : ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
-            const int64_t in = yzg ? ((y & 7) * 8 + (z & 7)) * 8 + (x & 7) : ((z & 7) * 8 + (y & 7)) * 8 + (x & 7);
-            key[p] = tile * 512 + in;
-            continue;
-        }
-        int64_t cell = col[rowptr[p]];
-        int64_t x = cell % N[0], y = (cell / N[0]) % N[1], z = cell / ((int64_t)N[0] * N[1]);
-        key[p] = yzg ? ((y >> 3) * tz + (z >> 3)) * tx + (x >> 3) : ((z >> 3) * ty + (y >> 3)) * tx + (x >> 3);
+        const int64_t cell = col[rowptr[p]];
+        const int64_t x = cell % N[0], y = (cell / N[0]) % N[1], z = cell / ((int64_t)N[0] * N[1]);
+        key[p] = tile_key(x, y, z) * 512 + ((y & 7) * 8 + (z & 7)) * 8 + (x & 7);
     }
     std::vector<int> perm(P), inv(P);
     for (int64_t p = 0; p < P; p++) perm[p] = (int)p;
@@ -764,9 +790,22 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
         // slowest axis sa, then sb, then x (storage order of the grid)
         const int sa = yzg ? 1 : 2, sb = yzg ? 2 : 1;
         int c3[3] = {0, 0, 0};
+        // the tiles in the order of tile_key
+        std::vector<std::pair<int64_t, int3>> tiles;
         for (int a0 = 0; a0 < N[sa]; a0 += 8)
             for (int b0 = 0; b0 < N[sb]; b0 += 8)
-                for (int x0 = 0; x0 < N[0]; x0 += 8)
+                for (int x0 = 0; x0 < N[0]; x0 += 8) {
+                    c3[0] = x0;
+                    c3[sa] = a0;
+                    c3[sb] = b0;
+                    tiles.push_back({tile_key(c3[0], c3[1], c3[2]), make_int3(a0, b0, x0)});
+                }
+        std::stable_sort(tiles.begin(), tiles.end(),
+                         [](const std::pair<int64_t, int3>& u, const std::pair<int64_t, int3>& v) {
+                             return u.first < v.first;
+                         });
+        for (const auto& tl : tiles) {
+            const int a0 = tl.second.x, b0 = tl.second.y, x0 = tl.second.z;
                     for (int a = a0; a < std::min(a0 + 8, N[sa]); a++)
                         for (int b = b0; b < std::min(b0 + 8, N[sb]); b++)
                             for (int x = x0; x < std::min(x0 + 8, N[0]); x++) {
@@ -785,8 +824,28 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
                                 }
                                 q++;
                             }
+        }
         dnj.resize(q);
         drow.resize(q);
+    }
+    // rows longer than kLongRow (the k-space centre): entries by ascending
+    // point, so that a row's entries inside a range of consecutive points
+    // are one run (the range form of the long rows)
+    {
+        int64_t o = 0;
+        std::vector<std::pair<int, float>> ent;
+        for (size_t r = 0; r < dnj.size(); o += dnj[r], r++) {
+            const int nr = dnj[r];
+            if (nr <= kLongRow) continue;
+            ent.resize(nr);
+            for (int k = 0; k < nr; k++) ent[k] = {dcol[o + k], dval[o + k]};
+            std::sort(ent.begin(), ent.end(),
+                      [](const std::pair<int, float>& a, const std::pair<int, float>& b) { return a.first < b.first; });
+            for (int k = 0; k < nr; k++) {
+                dcol[o + k] = ent[k].first;
+                dval[o + k] = ent[k].second;
+            }
+        }
     }
     if (yzg)
         for (auto& c : drow) c = stored_cell(c);
@@ -794,6 +853,7 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
     sparse_from_decoded(*Gut_s, Gut->r_size, Gut->l_size, dcol, dval, dnj, &drow, s);
     Gut_s->covers_all = !sparse_frame && !use_ival;
     build_groups(*Gut_s, dnj, dcol, nCh, s, grp_stage_kb, grp_entries, grp_rows);
+    build_ranges(*Gut_s, dnj, dcol, nCh, s);
     // cell masks: cells the forward gridding reads (Gu columns) and cells the
     // adjoint gridding writes (Gut rows with entries); equal for a
     // transposed pair.  Layout of the pass along the last axis: for each
@@ -1452,7 +1512,7 @@ namespace {
 size_t sparse_bytes(const SparseDev& S) {
     return S.col.bytes + S.val.bytes + S.rowptr.bytes + S.rowidx.bytes + S.cv.bytes + S.long_row.bytes +
            S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes + S.group.bytes + S.group_col.bytes +
-           S.slot.bytes;
+           S.slot.bytes + S.range.bytes + S.seg.bytes + S.range_first.bytes;
 }
 }  // namespace
 

@@ -428,6 +428,11 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ rowptr,
     const int rpp = 256 / tpr;  // rows per pass
     const int rr = threadIdx.x / tpr, q = threadIdx.x - rr * tpr;
     if (rr >= rpp) return;
+    // coils of work-item q: k0, k0 + 1 and k1, k1 + 1 at CPT = 4 (VecSplit:
+    // with float2 output the first loads and stores of a row's work-items
+    // cover one contiguous half of the row, the second ones the other)
+    using VS = VecSplit<CPT, float2, TW>;
+    const int k0 = VS::k0(q, n_vec), k1 = VS::k1(q, n_vec);
     for (int r = rr; r < nr; r += rpp) {
         const int jb = ro[r], je = ro[r + 1];
         float ar[CPT], ai[CPT];
@@ -441,9 +446,16 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ rowptr,
             float2 xv[4][CPT];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const float2* xs = sv + __float_as_int(e[u].y) * ls + q * CPT;
+                const float2* xs = sv + __float_as_int(e[u].y) * ls;
+                if constexpr (CPT == 4) {
+                    xv[u][0] = xs[k0];
+                    xv[u][1] = xs[k0 + 1];
+                    xv[u][2] = xs[k1];
+                    xv[u][3] = xs[k1 + 1];
+                } else {
 #pragma unroll
-                for (int h = 0; h < CPT; h++) xv[u][h] = xs[h];
+                    for (int h = 0; h < CPT; h++) xv[u][h] = xs[k0 + h];
+                }
             }
             // a surplus entry adds -0, which leaves every sum unchanged
             // (+0, -0, infinities and NaN included)
@@ -458,7 +470,90 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ rowptr,
                 }
             }
         }
-        st_n<CPT>(w + (int64_t)rw[r] * n_vec + q * CPT, ar, ai);
+        st_v<CPT>(w + (int64_t)rw[r] * n_vec, k0, k1, ar, ai);
+    }
+}
+
+// Long rows in the range form: the block of range R first stages the
+// vectors of R's columns -- one contiguous run of v, scaled by scale[c] as
+// row_sum_cv scales them -- in LDS, then sums each of R's segments into its
+// partial with n_vec / CPT work-items per segment, entries in stored order
+// with row_sum_cv's operations.  Neighbouring long rows (the k-space centre)
+// reference mostly the same points, so a point's vector is read once per
+// range instead of once per entry.
+template <bool SCALE, int CPT, class TV>
+__global__ void __launch_bounds__(256)
+spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, const SpEntry* __restrict__ cv,
+            int n_vec, const TV* __restrict__ v, const float* __restrict__ scale, float2* __restrict__ partial) {
+    extern __shared__ float2 sv[];  // [column count][n_vec + 1]
+    const int ls = n_vec + 1;
+    const int4 R = ranges[blockIdx.x];
+    const int c0 = R.x, items = R.y * n_vec;
+    const TV* vb = v + (int64_t)c0 * n_vec;
+    for (int i0 = threadIdx.x; i0 < items; i0 += 8 * 256) {
+        int k[8], i[8];
+        float2 x[8];
+        float sc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            i[u] = min(i0 + u * 256, items - 1);
+            k[u] = i[u] / n_vec;
+            x[u] = ld_c(vb + i[u]);
+            if (SCALE) sc[u] = scale[c0 + k[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (SCALE) {
+                x[u].x = sc[u] * x[u].x;
+                x[u].y = sc[u] * x[u].y;
+            }
+            sv[k[u] * ls + (i[u] - k[u] * n_vec)] = x[u];  // a surplus item rewrites item items - 1's value
+        }
+    }
+    __syncthreads();
+    const int tpr = n_vec / CPT;
+    const int spp = 256 / tpr;  // segments per pass
+    const int rr = threadIdx.x / tpr, q = threadIdx.x - rr * tpr;
+    if (rr >= spp) return;
+    using VS = VecSplit<CPT, float2, float2>;
+    const int k0 = VS::k0(q, n_vec), k1 = VS::k1(q, n_vec);
+    for (int j = rr; j < R.w; j += spp) {
+        const SpSeg sg = segs[R.z + j];
+        float ar[CPT], ai[CPT];
+#pragma unroll
+        for (int h = 0; h < CPT; h++) ar[h] = ai[h] = 0.0f;
+        for (int e = 0; e < sg.len; e += 4) {
+            const int cnt = min(sg.len - e, 4);
+            SpEntry en[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) en[u] = cv[sg.eb + min(e + u, sg.len - 1)];
+            float2 xv[4][CPT];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float2* xs = sv + (en[u].col - c0) * ls;
+                if constexpr (CPT == 4) {
+                    xv[u][0] = xs[k0];
+                    xv[u][1] = xs[k0 + 1];
+                    xv[u][2] = xs[k1];
+                    xv[u][3] = xs[k1 + 1];
+                } else {
+#pragma unroll
+                    for (int h = 0; h < CPT; h++) xv[u][h] = xs[k0 + h];
+                }
+            }
+            // a surplus entry adds -0, which leaves every sum unchanged
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool on = u < cnt;
+#pragma unroll
+                for (int h = 0; h < CPT; h++) {
+                    const float pr = en[u].val * xv[u][h].x, pi = en[u].val * xv[u][h].y;
+                    ar[h] = ar[h] + (on ? pr : -0.0f);
+                    ai[h] = ai[h] + (on ? pi : -0.0f);
+                }
+            }
+        }
+        st_v<CPT>(partial + (int64_t)sg.part * n_vec, k0, k1, ar, ai);
     }
 }
 
@@ -541,6 +636,20 @@ __global__ void k_add_groups(int4* dst, const int4* src, int64_t n, int row_add,
     if (i < n) {
         const int4 g = src[i];
         dst[i] = make_int4(g.x + row_add, g.y + row_add, g.z + col_add, g.w);
+    }
+}
+__global__ void k_add_ranges(int4* dst, const int4* src, int64_t n, int col_add, int seg_add) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int4 g = src[i];
+        dst[i] = make_int4(g.x + col_add, g.y, g.z + seg_add, g.w);
+    }
+}
+__global__ void k_add_segs(SpSeg* dst, const SpSeg* src, int64_t n, int64_t entry_add, int part_add) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const SpSeg g = src[i];
+        dst[i] = SpSeg{g.eb + entry_add, g.len, g.part + part_add};
     }
 }
 __global__ void k_iota_i32(int* dst, int64_t n, int64_t start) {
@@ -732,6 +841,89 @@ int group_rows_default() {
 constexpr int kGroupMaxCols = 1024;
 }  // namespace
 
+void build_ranges(SparseDev& S, const std::vector<int>& nj, const std::vector<int>& col, int n_vec,
+                  hipStream_t s) {
+    S.n_range = S.range_vec = S.range_cap = 0;
+    S.n_seg = S.n_rpart = 0;
+    S.range.release();
+    S.seg.release();
+    S.range_first.release();
+    static const int kb = env_int("MONA_RANGE_KB", 16);
+    const int cap = n_vec > 0 ? (int)std::min<int64_t>(kGroupMaxCols, (int64_t)kb * 1024 / ((int64_t)(n_vec + 1) *
+                                                                                          sizeof(float2)))
+                              : 0;
+    if (S.n_long == 0 || cap <= 0) return;
+    const int rows = (int)nj.size();
+    MONA_CHECK((int64_t)col.size() == S.nnz, MONA_ERR_INTERNAL, "range build: entry count mismatch");
+    std::vector<int64_t> rp(rows + 1, 0);
+    for (int r = 0; r < rows; r++) rp[r + 1] = rp[r] + nj[r];
+    std::vector<int> lrow;
+    std::vector<char> used((size_t)S.r_size, 0);
+    for (int r = 0; r < rows; r++) {
+        if (rp[r + 1] - rp[r] <= kLongRow) continue;
+        lrow.push_back(r);
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            if (e > rp[r] && col[e] <= col[e - 1]) return;
+            used[col[e]] = 1;
+        }
+    }
+    if ((int)lrow.size() != S.n_long) return;
+    // ranges: from each first column not yet covered, cap columns on
+    std::vector<int> rid((size_t)S.r_size, -1), rstart;
+    for (int c = 0; c < S.r_size;) {
+        if (!used[c]) {
+            c++;
+            continue;
+        }
+        const int k = (int)rstart.size(), e = std::min(S.r_size, c + cap);
+        rstart.push_back(c);
+        for (int c2 = c; c2 < e; c2++) rid[c2] = k;
+        c = e;
+    }
+    const int nr = (int)rstart.size();
+    // segments in row order (each row's in range order), then grouped by range
+    std::vector<SpSeg> rowseg;
+    std::vector<int> segrange, first(lrow.size() + 1, 0);
+    for (size_t l = 0; l < lrow.size(); l++) {
+        const int r = lrow[l];
+        first[l] = (int)rowseg.size();
+        int cur = -1;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int k = rid[col[e]];
+            if (k != cur) {
+                rowseg.push_back(SpSeg{e, 0, (int)rowseg.size()});
+                segrange.push_back(k);
+                cur = k;
+            }
+            rowseg.back().len++;
+        }
+    }
+    first[lrow.size()] = (int)rowseg.size();
+    std::vector<int> cnt(nr + 1, 0);
+    for (int k : segrange) cnt[k + 1]++;
+    for (int k = 0; k < nr; k++) cnt[k + 1] += cnt[k];
+    std::vector<int4> ranges(nr);
+    for (int k = 0; k < nr; k++)
+        ranges[k] = make_int4(rstart[k], std::min(cap, S.r_size - rstart[k]), cnt[k], cnt[k + 1] - cnt[k]);
+    std::vector<SpSeg> segs(rowseg.size());
+    {
+        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+        for (size_t i = 0; i < rowseg.size(); i++) segs[pos[segrange[i]]++] = rowseg[i];
+    }
+    S.range.alloc((size_t)nr * sizeof(int4));
+    S.seg.alloc(std::max<size_t>(segs.size(), 1) * sizeof(SpSeg));
+    S.range_first.alloc(first.size() * sizeof(int));
+    MONA_HIP(hipMemcpyAsync(S.range.p, ranges.data(), (size_t)nr * sizeof(int4), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.seg.p, segs.data(), segs.size() * sizeof(SpSeg), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.range_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipStreamSynchronize(s));  // host arrays are temporaries
+    S.n_range = nr;
+    S.range_vec = n_vec;
+    S.range_cap = cap;
+    S.n_seg = (int64_t)segs.size();
+    S.n_rpart = (int64_t)rowseg.size();
+}
+
 void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<int>& col, int n_vec,
                   hipStream_t s, int stage_kb, int entries, int max_rows) {
     S.n_group = S.group_vec = S.group_cap = S.group_ecap = S.group_rcap = 0;
@@ -880,7 +1072,7 @@ int xcd_group(bool adjoint) {
 // 16-byte aligned, operator matrices only).
 template <int CPT, int U, class TV, class TW>
 void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
-                    hipStream_t s, float2* part_ext, bool main_rows = true) {
+                    hipStream_t s, float2* part_ext, bool main_rows = true, bool long_rows = true) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
     const int64_t total = (int64_t)V.rows * (n_vec / CPT);
     const SpEntry* cv = S.cv.as<SpEntry>();
@@ -894,7 +1086,7 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
                                                       xg);
         MONA_HIP(hipGetLastError());
     }
-    if (V.n_long > 0) {
+    if (V.n_long > 0 && long_rows) {
         float2* part = part_ext;
         if (!part) {
             S.partial.ensure((size_t)S.n_chunk * n_vec * sizeof(float2));
@@ -918,6 +1110,29 @@ void spmv_cv_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int
                                                             V.n_long, ri, n_vec, part, w);
         MONA_HIP(hipGetLastError());
     }
+}
+
+// The long rows of the first V.n_long long rows in the range form
+// (spmv_ranges, then the partials added per row in range order).
+template <int CPT, class TV, class TW>
+void spmv_ranges_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
+                        hipStream_t s) {
+    if (V.n_long == 0) return;
+    S.partial.ensure((size_t)S.n_rpart * n_vec * sizeof(float2));
+    float2* part = S.partial.as<float2>();
+    const size_t lds = (size_t)S.range_cap * (n_vec + 1) * sizeof(float2);
+    if (yscale)
+        spmv_ranges<true, CPT><<<V.n_range, 256, lds, s>>>(S.range.as<int4>(), S.seg.as<SpSeg>(), S.cv.as<SpEntry>(),
+                                                           n_vec, v, yscale, part);
+    else
+        spmv_ranges<false, CPT><<<V.n_range, 256, lds, s>>>(S.range.as<int4>(), S.seg.as<SpSeg>(),
+                                                            S.cv.as<SpEntry>(), n_vec, v, nullptr, part);
+    MONA_HIP(hipGetLastError());
+    const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
+    const int64_t tl = (int64_t)V.n_long * n_vec;
+    spmv_c2_combine<<<blocks_for(tl, 256), 256, 0, s>>>(S.long_row.as<int>(), S.range_first.as<int>(), V.n_long, ri,
+                                                        n_vec, part, w);
+    MONA_HIP(hipGetLastError());
 }
 
 // The staged short rows of the first V.n_group groups (spmv_groups).
@@ -955,22 +1170,42 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
             V.n_long = S.n_long;
             V.n_chunk = S.n_chunk;
             V.n_group = S.n_group;
+            V.n_range = S.n_range;
+            V.n_rpart = S.n_rpart;
         }
         const bool al = ((uintptr_t)v % 16 == 0) && ((uintptr_t)w % 16 == 0);
         const int cpt = (al && n_vec % 4 == 0) ? 4 : (al && n_vec % 2 == 0) ? 2 : 1;
         if ((S.squeezed || S.l_nJump != S.l_size) && !S.covers_all && zero_rest)
             fill_f32(reinterpret_cast<float*>(w), 0.0f, V.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
         if (V.rows == 0) return;
-        // staged short rows, then the long rows' chunks as below
+        // long rows in the range form (when built for n_vec vectors);
+        // otherwise their chunks
+        const bool ranges = S.n_range > 0 && S.range_vec == n_vec && V.n_long > 0;
+        auto long_ranges = [&] {
+            if (cpt == 4) spmv_ranges_launch<4>(S, V, v, w, n_vec, yscale, s);
+            else if (cpt == 2) spmv_ranges_launch<2>(S, V, v, w, n_vec, yscale, s);
+            else spmv_ranges_launch<1>(S, V, v, w, n_vec, yscale, s);
+        };
+        // staged short rows, then the long rows
         if (staged && V.n_group > 0 && S.group_vec == n_vec) {
             if (cpt == 4) spmv_groups_launch<4>(S, V, v, w, n_vec, yscale, s);
             else if (cpt == 2) spmv_groups_launch<2>(S, V, v, w, n_vec, yscale, s);
             else spmv_groups_launch<1>(S, V, v, w, n_vec, yscale, s);
-            if (V.n_long > 0) {
+            if (ranges) {
+                long_ranges();
+            } else if (V.n_long > 0) {
                 if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s, part_ext, false);
                 else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s, part_ext, false);
                 else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s, part_ext, false);
             }
+            return;
+        }
+        if (ranges) {
+            // short rows by the per-entry gather (same results as the groups)
+            if (cpt == 4) spmv_cv_launch<4, 4>(S, V, v, w, n_vec, yscale, s, part_ext, true, false);
+            else if (cpt == 2) spmv_cv_launch<2, 8>(S, V, v, w, n_vec, yscale, s, part_ext, true, false);
+            else spmv_cv_launch<1, 8>(S, V, v, w, n_vec, yscale, s, part_ext, true, false);
+            long_ranges();
             return;
         }
         // measured on 3D grids at 16 and 20 coils: the gridding gather
@@ -1078,6 +1313,22 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
         ng += P.n_group;
         ngc += P.n_group_col;
     }
+    // the range form of the long rows stacks when every part with long rows
+    // has it for the same vector count and range width
+    bool rng = n > 0;
+    int rvec = 0, rcap = 0;
+    int64_t nrg = 0, nsg = 0, nrp = 0;
+    for (size_t f = 0; f < n && rng; f++) {
+        const SparseDev& P = *parts[f];
+        if (P.n_long == 0) continue;
+        rng = P.n_range > 0 && (rvec == 0 || (P.range_vec == rvec && P.range_cap == rcap));
+        rvec = P.range_vec;
+        rcap = P.range_cap;
+        nrg += P.n_range;
+        nsg += P.n_seg;
+        nrp += P.n_rpart;
+    }
+    rng = rng && nrg > 0;
     MONA_CHECK(rows < 0x7fffffff && nnz < ((int64_t)1 << 40), MONA_ERR_ARG, "stacked matrix too large");
     out.r_size = r_size;
     out.l_size = l_size;
@@ -1112,13 +1363,26 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
         out.group_rcap = parts[0]->group_rcap;
         out.n_group_col = ngc;
     }
+    out.n_range = out.range_vec = out.range_cap = 0;
+    out.n_seg = out.n_rpart = 0;
+    if (rng) {
+        out.range_vec = rvec;
+        out.range_cap = rcap;
+        MONA_CHECK(nrg < 0x7fffffff && nrp < 0x7fffffff, MONA_ERR_ARG, "stacked matrix too large");
+        out.range.alloc((size_t)nrg * sizeof(int4));
+        out.seg.alloc((size_t)std::max<int64_t>(nsg, 1) * sizeof(SpSeg));
+        out.range_first.alloc((size_t)(nl + 1) * sizeof(int));
+        out.n_range = (int)nrg;
+        out.n_seg = nsg;
+        out.n_rpart = nrp;
+    }
     if (views) views->assign(n + 1, SpView{});
-    int64_t eo = 0, ro = 0, co = 0, go = 0, gco = 0;
+    int64_t eo = 0, ro = 0, co = 0, go = 0, gco = 0, rgo = 0, sgo = 0, rpo = 0;
     int lo = 0;
     auto nb = [](int64_t k) { return blocks_for(std::max<int64_t>(k, 1), 256); };
     for (size_t f = 0; f < n; f++) {
         const SparseDev& P = *parts[f];
-        if (views) (*views)[f] = SpView{(int)ro, row_off[f], lo, co, (int)go};
+        if (views) (*views)[f] = SpView{(int)ro, row_off[f], lo, co, (int)go, (int)rgo, rpo};
         if (P.nnz > 0) {
             k_add_i32<<<nb(P.nnz), 256, 0, s>>>(out.col.as<int>() + eo, P.col.as<int>(), P.nnz, col_off[f]);
             k_add_entries<<<nb(P.nnz), 256, 0, s>>>(out.cv.as<SpEntry>() + eo, P.cv.as<SpEntry>(), P.nnz,
@@ -1155,13 +1419,24 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
             go += P.n_group;
             gco += pgc;
         }
+        if (rng && P.n_long > 0) {
+            k_add_ranges<<<nb(P.n_range), 256, 0, s>>>(out.range.as<int4>() + rgo, P.range.as<int4>(), P.n_range,
+                                                      (int)col_off[f], (int)sgo);
+            k_add_segs<<<nb(P.n_seg), 256, 0, s>>>(out.seg.as<SpSeg>() + sgo, P.seg.as<SpSeg>(), P.n_seg, eo,
+                                                  (int)rpo);
+            k_add_i32<<<nb(P.n_long + 1), 256, 0, s>>>(out.range_first.as<int>() + lo, P.range_first.as<int>(),
+                                                       P.n_long + 1, rpo);
+            rgo += P.n_range;
+            sgo += P.n_seg;
+            rpo += P.n_rpart;
+        }
         MONA_HIP(hipGetLastError());
         eo += P.nnz;
         ro += P.l_nJump;
         lo += P.n_long;
         co += P.n_chunk;
     }
-    if (views) (*views)[n] = SpView{(int)ro, (int64_t)l_size, lo, co, (int)go};
+    if (views) (*views)[n] = SpView{(int)ro, (int64_t)l_size, lo, co, (int)go, (int)rgo, rpo};
     out.on_device = true;
 }
 
