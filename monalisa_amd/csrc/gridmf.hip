@@ -154,18 +154,138 @@ u2n_mf_kernel(const uint32_t* __restrict__ base, const float4* __restrict__ rec,
     st_cells<CPL>(y + p * nCh + q * CPL, ar, ai);
 }
 
+// x-run form of u2n_mf_kernel: the nCh / CPL lanes of a (point, x offset)
+// pair gather that offset's nine (3-D) or three (2-D) cells, so the lanes
+// of one point read its three x-neighbours of a (y, z) row together, one
+// contiguous run of 3 nCh values (fewer cache lines per load instruction
+// than one cell at a time).  The three x partial sums of a point are added
+// in x order by its first lane.  Steps: each wave takes ppw points a step.
+constexpr int kXrMaxPts = 96;
+template <int D, int CPL, class TG, class TY>
+__global__ void __launch_bounds__(kMfBlock)
+u2n_mf_xrun(const uint32_t* __restrict__ base, const float4* __restrict__ rec, const uint16_t* __restrict__ frame_of,
+            int P, const TG* __restrict__ grid, int64_t frame_cells, TY* __restrict__ y, int nCh, MfGeo g,
+            MfKernel k, int ppw, int steps, int xg) {
+    constexpr int NB = D == 3 ? 27 : 9;
+    constexpr int NR = NB / 3;
+    __shared__ float sw[kXrMaxPts * NB];
+    __shared__ int soff[kXrMaxPts * NB];
+    __shared__ int sfr[kXrMaxPts];
+    const int ppb = (kMfBlock / 64) * ppw * steps;
+    const int64_t p0 = xcd_block(blockIdx.x, gridDim.x, xg) * ppb;
+    for (int t = threadIdx.x; t < ppb * NB; t += kMfBlock) {
+        const int i = t / NB, j = t - i * NB;
+        const int64_t p = p0 + i;
+        float w = 0.0f;
+        int off = 0;
+        if (p < P) {
+            const uint32_t b = base[p];
+            const float4 r = rec[p];
+            const int cx = j % 3 - 1, cy = (j / 3) % 3 - 1, cz = D == 3 ? j / 9 - 1 : 0;
+            const float dx = r.x - cx, dy = r.y - cy, dz = r.z - cz;
+            w = mf_weight(dx * dx + dy * dy + dz * dz, k) * r.w;
+            const int x = wrap_axis((int)(b & 1023u) + cx, g.N0);
+            const int yy = wrap_axis((int)((b >> 10) & 1023u) + cy, g.N1);
+            const int z = D == 3 ? wrap_axis((int)(b >> 20) + cz, g.N2) : 0;
+            off = x + g.sy * yy + g.sz * z;
+            // a deleted point (no Gu row) reads nothing: -1 marks it
+            if (j == 0) sfr[i] = r.w == 0.0f ? -1 : (frame_of ? (int)frame_of[p] : 0);
+        } else if (j == 0) {
+            sfr[i] = -1;
+        }
+        sw[t] = w;
+        soff[t] = off;
+    }
+    __syncthreads();
+    const int tpp = nCh / CPL, lpp = 3 * tpp;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int iw = lane / lpp, rl = lane - iw * lpp, dx = rl / tpp, q = rl - dx * tpp;
+    for (int st = 0; st < steps; st++) {
+        const int i = (st * (kMfBlock / 64) + wv) * ppw + iw;
+        const int64_t p = p0 + i;
+        const bool on = iw < ppw && p < P;
+        float ar[CPL], ai[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) ar[c] = ai[c] = 0.0f;
+        const int fr = on ? sfr[i] : -1;
+        if (fr >= 0) {
+            const TG* gb = grid + (int64_t)fr * frame_cells * nCh + q * CPL;
+            float vr[NR][CPL], vi[NR][CPL];
+#pragma unroll
+            for (int jr = 0; jr < NR; jr++) ld_cells<CPL>(gb + (int64_t)soff[i * NB + jr * 3 + dx] * nCh, vr[jr], vi[jr]);
+#pragma unroll
+            for (int jr = 0; jr < NR; jr++) {
+                const float w = sw[i * NB + jr * 3 + dx];
+#pragma unroll
+                for (int c = 0; c < CPL; c++) {
+                    ar[c] = fmaf(w, vr[jr][c], ar[c]);
+                    ai[c] = fmaf(w, vi[jr][c], ai[c]);
+                }
+            }
+        }
+        // the x offsets 0 and +1 of the point to its first lanes (same wave)
+        float br[CPL], bi[CPL], cr[CPL], ci[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            br[c] = __shfl(ar[c], lane + tpp, 64);
+            bi[c] = __shfl(ai[c], lane + tpp, 64);
+            cr[c] = __shfl(ar[c], lane + 2 * tpp, 64);
+            ci[c] = __shfl(ai[c], lane + 2 * tpp, 64);
+        }
+        if (on && dx == 0) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                ar[c] = (ar[c] + br[c]) + cr[c];
+                ai[c] = (ai[c] + bi[c]) + ci[c];
+            }
+            st_cells<CPL>(y + p * nCh + q * CPL, ar, ai);
+        }
+    }
+}
+
 template <int D, class TG, class TY>
 void launch_u2n(int cpl, const uint32_t* base, const float4* rec, const uint16_t* frame_of, int P, const TG* grid,
                 int64_t frame_cells, TY* y, int nCh, const MfGeo& g, const MfKernel& k, hipStream_t s) {
     const int tpp = nCh / cpl;
-    const int ppb = std::min(kMfMaxPts, kMfBlock / tpp);
-    MONA_CHECK(ppb >= 1, MONA_ERR_ARG, "too many coils for the matrix-free gridding");
-    const unsigned nb = (unsigned)((P + ppb - 1) / ppb);
     // runs of xg consecutive blocks per XCD (MONA_XCD_GROUP_MF, default 16)
     static const int xg = [] {
         const char* e = std::getenv("MONA_XCD_GROUP_MF");
         return e ? std::max(1, std::atoi(e)) : 16;
     }();
+    // x-run form when a point's three x offsets fit one wave (MONA_MF_XRUN=0
+    // turns it off)
+    static const bool xrun = [] {
+        const char* e = std::getenv("MONA_MF_XRUN");
+        return !(e && std::atoi(e) == 0);
+    }();
+    if (xrun && 3 * tpp <= 64) {
+        const int ppw = 64 / (3 * tpp);
+        const int steps = std::max(1, 32 / ((kMfBlock / 64) * ppw));
+        const int ppb = (kMfBlock / 64) * ppw * steps;
+        MONA_CHECK(ppb <= kXrMaxPts, MONA_ERR_INTERNAL, "x-run block too large");
+        const unsigned nb = (unsigned)((P + ppb - 1) / ppb);
+        switch (cpl) {
+            case 4:
+                if constexpr (std::is_same<TG, bf16x2>::value)
+                    u2n_mf_xrun<D, 4, TG, TY><<<nb, kMfBlock, 0, s>>>(base, rec, frame_of, P, grid, frame_cells, y,
+                                                                     nCh, g, k, ppw, steps, xg);
+                else
+                    MONA_CHECK(false, MONA_ERR_INTERNAL, "four coils per lane is a bf16 layout");
+                break;
+            case 2:
+                u2n_mf_xrun<D, 2, TG, TY><<<nb, kMfBlock, 0, s>>>(base, rec, frame_of, P, grid, frame_cells, y, nCh,
+                                                                 g, k, ppw, steps, xg);
+                break;
+            default:
+                u2n_mf_xrun<D, 1, TG, TY><<<nb, kMfBlock, 0, s>>>(base, rec, frame_of, P, grid, frame_cells, y, nCh,
+                                                                 g, k, ppw, steps, xg);
+        }
+        MONA_HIP(hipGetLastError());
+        return;
+    }
+    const int ppb = std::min(kMfMaxPts, kMfBlock / tpp);
+    MONA_CHECK(ppb >= 1, MONA_ERR_ARG, "too many coils for the matrix-free gridding");
+    const unsigned nb = (unsigned)((P + ppb - 1) / ppb);
     switch (cpl) {
         case 4:
             if constexpr (std::is_same<TG, bf16x2>::value)

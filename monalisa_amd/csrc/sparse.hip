@@ -1188,8 +1188,11 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
         };
         // staged short rows, then the long rows
         if (staged && V.n_group > 0 && S.group_vec == n_vec) {
-            if (cpt == 4) spmv_groups_launch<4>(S, V, v, w, n_vec, yscale, s);
-            else if (cpt == 2) spmv_groups_launch<2>(S, V, v, w, n_vec, yscale, s);
+            // two vectors per work-item (measured on the 3-D N=128 20-coil
+            // CG-SENSE: 0.93 ms a launch against 0.99 ms at four, whose rows
+            // fill fewer lanes of a block's second pass; bigger blocks, padded
+            // LDS rows and entries read from global memory were slower)
+            if (cpt >= 2) spmv_groups_launch<2>(S, V, v, w, n_vec, yscale, s);
             else spmv_groups_launch<1>(S, V, v, w, n_vec, yscale, s);
             if (ranges) {
                 long_ranges();
