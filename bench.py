@@ -675,9 +675,10 @@ def main(argv=None):
     avg_u2n = ms_u2n / max(n_u2n, 1)
     avg_n2u = ms_n2u / max(n_n2u, 1)
     if ms_n2u >= ms_u2n:
-        dom, bytes_, avg = "n2u (adjoint gridding: LDS-staged groups of grid cells + long-row chunks)", n2u_b, avg_n2u
+        dom, bytes_, avg = ("n2u (adjoint gridding: LDS-staged groups of grid cells + long rows by ranges of "
+                            "consecutive samples)", n2u_b, avg_n2u)
     else:
-        dom, bytes_, avg = "u2n (matrix-free forward gridding)", u2n_b, avg_u2n
+        dom, bytes_, avg = "u2n (matrix-free forward gridding, x-runs)", u2n_b, avg_u2n
     achieved = bytes_ / (avg * 1e-3) / 1e9
     ms_per = elapsed / args.steps * 1e3
     # the FFT op (the largest share of a recon): each timed region is one
@@ -717,7 +718,7 @@ def main(argv=None):
                      "bytes_per_launch": bytes_, "avg_launch_ms": avg,
                      "u2n_GBps": u2n_b / (avg_u2n * 1e-3) / 1e9 if n_u2n else None,
                      "n2u_GBps": n2u_b / (avg_n2u * 1e-3) / 1e9 if n_n2u else None,
-                     "u2n": {"kernel": "u2n (matrix-free forward gridding)", "bytes_per_launch": u2n_b,
+                     "u2n": {"kernel": "u2n (matrix-free forward gridding, x-runs)", "bytes_per_launch": u2n_b,
                              "avg_launch_ms": avg_u2n,
                              "achieved": u2n_b / (avg_u2n * 1e-3) / 1e9 if n_u2n else None,
                              "frac": u2n_b / (avg_u2n * 1e-3) / 1e9 / 8000.0 if n_u2n else None, "traffic": None},
