@@ -225,7 +225,8 @@ struct SparseDev {
     int n_group = 0, group_vec = 0;
     int group_cap = 0, group_ecap = 0, group_rcap = 0;  // most columns, entries, rows of a group
     DevBuf group;             // int4[n_group]: first row, end row, first column in group_col, column count
-    DevBuf group_col;         // int32[n_group_col]: the groups' columns back to back
+    DevBuf group_e;           // int64[2 n_group]: first and end entry of each group
+    DevBuf group_col;         // int32[n_group_col]: group g's columns at g * group_cap (padded with its last)
     int64_t n_group_col = 0;
     DevBuf slot;              // uint16[nnz] (entries of long rows unused)
     // Range form of the long rows (built with the groups; the operator

@@ -1511,7 +1511,7 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
 namespace {
 size_t sparse_bytes(const SparseDev& S) {
     return S.col.bytes + S.val.bytes + S.rowptr.bytes + S.rowidx.bytes + S.cv.bytes + S.long_row.bytes +
-           S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes + S.group.bytes + S.group_col.bytes +
+           S.long_first.bytes + S.chunk_begin.bytes + S.partial.bytes + S.group.bytes + S.group_col.bytes + S.group_e.bytes +
            S.slot.bytes + S.range.bytes + S.seg.bytes + S.range_first.bytes;
 }
 }  // namespace

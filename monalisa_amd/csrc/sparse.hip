@@ -367,27 +367,32 @@ spmv_cv_chunks(const int64_t* __restrict__ chunk_be, int64_t n_chunk, const SpEn
 // dropped) so that a batch's gathers are in flight together.
 template <bool SCALE, int CPT, class TV, class TW>
 __global__ void __launch_bounds__(256)
-spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ rowptr, const SpEntry* __restrict__ cv,
-            const uint16_t* __restrict__ slot, const int* __restrict__ gcol, const int* __restrict__ rowidx,
-            int n_vec, const TV* __restrict__ v, TW* __restrict__ w, const float* __restrict__ scale, int xg,
-            int ccap, int ecap, int rcap) {
+spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, const int64_t* __restrict__ rowptr,
+            const SpEntry* __restrict__ cv, const uint16_t* __restrict__ slot, const int* __restrict__ gcol,
+            const int* __restrict__ rowidx, int n_vec, const TV* __restrict__ v, TW* __restrict__ w,
+            const float* __restrict__ scale, int xg, int ccap, int ecap, int rcap) {
     extern __shared__ float2 sv[];  // [ccap][n_vec + 1] vectors, [ecap] entries, row offsets, output rows
     const int ls = n_vec + 1;
     float2* se = sv + ccap * ls;                    // (value, slot bits)
     int* ro = reinterpret_cast<int*>(se + ecap);    // [rows + 1]
     int* rw = ro + rcap + 1;                        // [rows]
-    const int4 G = groups[xcd_block(blockIdx.x, gridDim.x, xg)];
-    const int64_t e0 = rowptr[G.x];
-    const int ne = (int)(rowptr[G.y] - e0), nr = G.y - G.x;
-    const int items = G.w * n_vec;
-    // vectors: item i = value i % n_vec of column i / n_vec
+    // the group's descriptor, entry bounds and (padded) columns are found
+    // from the group index alone: their loads go out together
+    const int64_t gi = xcd_block(blockIdx.x, gridDim.x, xg);
+    const int4 G = groups[gi];
+    const int64_t e0 = gent[2 * gi];
+    const int ne = (int)(gent[2 * gi + 1] - e0), nr = G.y - G.x;
+    const int* gc = gcol + gi * ccap;
+    // vectors: item i = value i % n_vec of column i / n_vec, for every
+    // column slot of the group (the padding repeats its last column)
+    const int items = ccap * n_vec;
     for (int i0 = threadIdx.x; i0 < items; i0 += 8 * 256) {
         int c[8], dst[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int i = min(i0 + u * 256, items - 1);
             const int k = i / n_vec;
-            c[u] = gcol[G.z + k];
+            c[u] = gc[k];
             dst[u] = k * ls + (i - k * n_vec);
         }
         float2 x[8];
@@ -817,11 +822,11 @@ void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s) {
 namespace {
 // LDS budget of a group's staged vectors, and the most entries and rows a
 // group takes (MONA_STAGE_KB, MONA_GROUP_ENTRIES, MONA_GROUP_ROWS override
-// them; a budget of 0 turns the staged form off).  16 KB, 1024 entries and
-// 256 rows were the best of a sweep on the 3-D N=128 (256^3 grid) 20-coil
-// CG-SENSE: adjoint gridding 27.8 ms per recon against 32.4 ms for the
-// per-entry gather; 24-40 KB budgets, 2048 entries or 512-1024 rows were
-// 1-18% slower.
+// them; a budget of 0 turns the staged form off).  16 KB, 512 entries and
+// 256 rows were the best of sweeps on the 3-D N=128 (256^3 grid) 20-coil
+// CG-SENSE: 24-40 KB budgets, 2048 entries or 512-1024 rows were 1-18%
+// slower; 512 entries (a group averages about 410) leave room for a seventh
+// group per CU against 1024 (916 -> 890 us a launch).
 int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
@@ -831,7 +836,7 @@ int stage_kb_default() {
     return b;
 }
 int group_entries_default() {
-    static const int n = env_int("MONA_GROUP_ENTRIES", 1024);
+    static const int n = env_int("MONA_GROUP_ENTRIES", 512);
     return n;
 }
 int group_rows_default() {
@@ -930,6 +935,7 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     S.n_group_col = 0;
     S.group.release();
     S.group_col.release();
+    S.group_e.release();
     S.slot.release();
     if (n_vec <= 0 || S.nnz == 0) return;
     const int64_t stage = (int64_t)(stage_kb < 0 ? stage_kb_default() : stage_kb) * 1024;
@@ -941,6 +947,7 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     for (int r = 0; r < rows; r++) rp[r + 1] = rp[r] + nj[r];
     std::vector<int> mark(S.r_size, -1), seen(S.r_size, -1), pos(S.r_size, 0), cur, gcol;
     std::vector<int4> groups;
+    std::vector<int64_t> gent;
     std::vector<uint16_t> slot(S.nnz, 0);
     const int ecap = std::max(entries < 0 ? group_entries_default() : entries, kLongRow);
     const int rcap = std::max(max_rows < 0 ? group_rows_default() : max_rows, 1);
@@ -951,8 +958,13 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
             std::sort(cur.begin(), cur.end());
             for (int k = 0; k < (int)cur.size(); k++) pos[cur[k]] = k;
             for (int64_t e = rp[r0]; e < rp[r1]; e++) slot[e] = (uint16_t)pos[col[e]];
+            // columns at g * cap, padded to cap with the last one, so that a
+            // block finds its group's columns without reading the group first
             groups.push_back(make_int4(r0, r1, (int)gcol.size(), (int)cur.size()));
             gcol.insert(gcol.end(), cur.begin(), cur.end());
+            gcol.resize((size_t)(g + 1) * cap, cur.back());
+            gent.push_back(rp[r0]);
+            gent.push_back(rp[r1]);
             g++;
         }
         cur.clear();
@@ -987,8 +999,10 @@ void build_groups(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     if (groups.empty()) return;
     S.group.alloc(groups.size() * sizeof(int4));
     S.group_col.alloc(std::max<size_t>(gcol.size(), 1) * sizeof(int));
+    S.group_e.alloc(gent.size() * sizeof(int64_t));
     S.slot.alloc((size_t)S.nnz * sizeof(uint16_t));
     MONA_HIP(hipMemcpyAsync(S.group.p, groups.data(), groups.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    MONA_HIP(hipMemcpyAsync(S.group_e.p, gent.data(), gent.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (!gcol.empty())
         MONA_HIP(hipMemcpyAsync(S.group_col.p, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice, s));
     MONA_HIP(hipMemcpyAsync(S.slot.p, slot.data(), (size_t)S.nnz * sizeof(uint16_t), hipMemcpyHostToDevice, s));
@@ -1144,12 +1158,14 @@ void spmv_groups_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w,
                        (size_t)(2 * S.group_rcap + 1) * sizeof(int);
     const int xg = xcd_group(yscale != nullptr);
     if (yscale)
-        spmv_groups<true, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.rowptr.as<int64_t>(),
+        spmv_groups<true, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.group_e.as<int64_t>(),
+                                                           S.rowptr.as<int64_t>(),
                                                            S.cv.as<SpEntry>(), S.slot.as<uint16_t>(),
                                                            S.group_col.as<int>(), ri, n_vec, v, w, yscale, xg,
                                                            S.group_cap, S.group_ecap, S.group_rcap);
     else
-        spmv_groups<false, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.rowptr.as<int64_t>(),
+        spmv_groups<false, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.group_e.as<int64_t>(),
+                                                            S.rowptr.as<int64_t>(),
                                                             S.cv.as<SpEntry>(), S.slot.as<uint16_t>(),
                                                             S.group_col.as<int>(), ri, n_vec, v, w, nullptr, xg,
                                                             S.group_cap, S.group_ecap, S.group_rcap);
@@ -1357,6 +1373,7 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
     if (groups) {
         MONA_CHECK(ngc < 0x7fffffff, MONA_ERR_ARG, "stacked matrix too large");
         out.group.alloc((size_t)ng * sizeof(int4));
+        out.group_e.alloc((size_t)ng * 2 * sizeof(int64_t));
         out.group_col.alloc((size_t)std::max<int64_t>(ngc, 1) * sizeof(int));
         out.slot.alloc((size_t)std::max<int64_t>(nnz, 1) * sizeof(uint16_t));
         out.n_group = (int)ng;
@@ -1412,6 +1429,8 @@ void sparse_block_diag(SparseDev& out, const std::vector<const SparseDev*>& part
         }
         if (groups) {
             const int64_t pgc = P.n_group_col;
+            k_add_i64<<<nb(2 * P.n_group), 256, 0, s>>>(out.group_e.as<int64_t>() + 2 * go, P.group_e.as<int64_t>(),
+                                                        2 * P.n_group, eo);
             k_add_groups<<<nb(P.n_group), 256, 0, s>>>(out.group.as<int4>() + go, P.group.as<int4>(), P.n_group,
                                                       (int)ro, (int)gco);
             k_add_i32<<<nb(pgc), 256, 0, s>>>(out.group_col.as<int>() + gco, P.group_col.as<int>(), pgc,
