@@ -543,6 +543,15 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             dead = q.x < 0;
         }
     }
+    // coil-map pixel of row r of this line: kin + r (MODE 1) / kout + r
+    // (MODE 2, 3), wrapped once into [0, kfc_period) -- one 64-bit modulo
+    // per work-item instead of one per row
+    const int64_t kin = MODE == 1 ? (o * a.in_cnt) % a.kfc_period : 0;
+    const int64_t kout = MODE >= 2 ? (o * a.out_cnt) % a.kfc_period : 0;
+    auto kpix = [&](int64_t base, int r) {
+        const int64_t i = base + r;
+        return i >= a.kfc_period ? i - a.kfc_period : i;
+    };
     float2 v[N1];
     if (j < N2) {
 #pragma unroll
@@ -561,7 +570,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             } else if (take) {
                 if (MODE == 1) {
                     const int64_t ii = o * a.in_cnt + r;
-                    x = cmul(a.img[ii], a.kfc[(ii % a.kfc_period) * a.S + ws]);
+                    x = cmul(a.img[ii], a.kfc[kpix(kin, r) * a.S + ws]);
                 } else {
                     x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
                 }
@@ -578,7 +587,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             const int kl = j + N1 * k2 - a.out_off;
             kf[k2] = make_float2(0.f, 0.f);
             if (live && j < N1 && kl >= 0 && kl < a.out_cnt)
-                kf[k2] = a.kfc[((o * a.out_cnt + kl) % a.kfc_period) * a.S + ws];
+                kf[k2] = a.kfc[kpix(kout, kl) * a.S + ws];
         }
     }
     for (int i = t; i < N; i += blockDim.x) tw[i] = twg[i];
@@ -678,7 +687,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
             if (MODE == 3) {
                 const int64_t i = o * a.out_cnt + kl;
-                const float2 p = cmul(y, a.kfc[(i % a.kfc_period) * a.S + ws]);
+                const float2 p = cmul(y, a.kfc[kpix(kout, kl) * a.S + ws]);
                 st_c(out + i * a.S + ws, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
             } else {
                 st_c(out + obase + (int64_t)kl * a.out_rs + ws, y);
