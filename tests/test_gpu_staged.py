@@ -67,8 +67,9 @@ def test_staged_adjoint_bit_exact(kind, nCh):
 
 
 def test_staged_adjoint_long_rows_and_bf16():
-    """A trajectory dense at the k-space centre: rows over 64 entries keep
-    their chunked partial sums beside the staged groups."""
+    """A trajectory dense at the k-space centre: rows over 64 entries are
+    summed by ranges of consecutive samples beside the staged groups, the
+    same in both forms."""
     import torch
     from monalisa_amd import synth
     print("[gpu] staged adjoint, long rows, bf16", flush=True)
@@ -163,8 +164,9 @@ def test_group_autotune_keeps_results():
 
 
 def test_staged_long_rows_batched():
-    """Frames dense at the k-space centre (rows over 64 entries, summed in
-    chunks beside the staged groups): the batched CG-SENSE equals the
+    """Frames dense at the k-space centre (rows over 64 entries, summed by
+    ranges of consecutive samples beside the staged groups): the batched
+    CG-SENSE, whose stacked matrices stack the frames' ranges, equals the
     per-frame one bit for bit, staged or not."""
     import torch
     from monalisa_amd import synth
