@@ -1194,9 +1194,10 @@ void spmv_c2_typed(const SparseDev& S, const TV* v, TW* w, int n_vec, const floa
         if ((S.squeezed || S.l_nJump != S.l_size) && !S.covers_all && zero_rest)
             fill_f32(reinterpret_cast<float*>(w), 0.0f, V.l_size * n_vec * (int64_t)(sizeof(TW) / 4), s);
         if (V.rows == 0) return;
-        // long rows in the range form (when built for n_vec vectors);
+        // long rows in the range form (when built for n_vec vectors; its
+        // partials live in S.partial, so not with caller scratch);
         // otherwise their chunks
-        const bool ranges = S.n_range > 0 && S.range_vec == n_vec && V.n_long > 0;
+        const bool ranges = !part_ext && S.n_range > 0 && S.range_vec == n_vec && V.n_long > 0;
         auto long_ranges = [&] {
             if (cpt == 4) spmv_ranges_launch<4>(S, V, v, w, n_vec, yscale, s);
             else if (cpt == 2) spmv_ranges_launch<2>(S, V, v, w, n_vec, yscale, s);
