@@ -274,10 +274,10 @@ void build_chunks(SparseDev& S, const int* r_nJump, hipStream_t s);
 // MONA_GROUP_ROWS or 16, 1024, 256).
 void build_groups(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
                   hipStream_t s, int stage_kb = -1, int entries = -1, int rows = -1);
-// Builds the range form of S's long rows for products over n_vec vectors
-// (a 16 KB stage of consecutive columns, MONA_RANGE_KB overrides it; 0 = no
-// ranges) from the decoded host arrays; none when a long row's columns are
-// not ascending.
+// Builds the range form of S's long rows for products over n_vec <= 256
+// vectors (a 16 KB stage of consecutive columns, MONA_RANGE_KB overrides it;
+// 0 = no ranges) from the decoded host arrays; none when a long row's
+// columns are not ascending.
 void build_ranges(SparseDev& S, const std::vector<int>& r_nJump, const std::vector<int>& col, int n_vec,
                   hipStream_t s);
 // The same from S's own device arrays (rebuilds the groups of an uploaded

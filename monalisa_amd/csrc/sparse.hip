@@ -857,7 +857,8 @@ void build_ranges(SparseDev& S, const std::vector<int>& nj, const std::vector<in
     const int cap = n_vec > 0 ? (int)std::min<int64_t>(kGroupMaxCols, (int64_t)kb * 1024 / ((int64_t)(n_vec + 1) *
                                                                                           sizeof(float2)))
                               : 0;
-    if (S.n_long == 0 || cap <= 0) return;
+    // a segment is summed by n_vec / CPT work-items of one 256-wide block
+    if (S.n_long == 0 || cap <= 0 || n_vec > 256) return;
     const int rows = (int)nj.size();
     MONA_CHECK((int64_t)col.size() == S.nnz, MONA_ERR_INTERNAL, "range build: entry count mismatch");
     std::vector<int64_t> rp(rows + 1, 0);
