@@ -588,6 +588,46 @@ static int tile_order() {
     return o;
 }
 
+// Order of the cells of an 8^3 tile in the adjoint matrix: cubes of c cells
+// a side in raster order (slowest axis first), raster order inside a cube;
+// c = 8 is the tile's plain raster order, c = 0 the Morton order (cubes of
+// 2 in cubes of 4 in the tile).  Cubes of 2 (the default) keep a group of
+// consecutive rows compact in all three axes, so it stages fewer samples
+// than a slab of whole x-rows would.  MONA_CELL_CUBE overrides it.
+static int cell_cube() {
+    static const int c = [] {
+        const char* e = std::getenv("MONA_CELL_CUBE");
+        const int v = e ? std::atoi(e) : 2;
+        return (v == 0 || v == 4 || v == 8) ? v : 2;
+    }();
+    return c;
+}
+
+// (slowest, middle, x) offsets of the 512 cells of a tile in that order
+static std::vector<int3> tile_cell_order(int c) {
+    std::vector<int3> o;
+    o.reserve(512);
+    if (c == 0) {
+        for (int m = 0; m < 512; m++) {
+            int a = 0, b = 0, x = 0;
+            for (int l = 0; l < 3; l++) {
+                a |= ((m >> (3 * l + 2)) & 1) << l;
+                b |= ((m >> (3 * l + 1)) & 1) << l;
+                x |= ((m >> (3 * l)) & 1) << l;
+            }
+            o.push_back(make_int3(a, b, x));
+        }
+        return o;
+    }
+    for (int a1 = 0; a1 < 8; a1 += c)
+        for (int b1 = 0; b1 < 8; b1 += c)
+            for (int x1 = 0; x1 < 8; x1 += c)
+                for (int a = a1; a < a1 + c; a++)
+                    for (int b = b1; b < b1 + c; b++)
+                        for (int x = x1; x < x1 + c; x++) o.push_back(make_int3(a, b, x));
+    return o;
+}
+
 // Morton code of tile coordinates (a slowest, then b, then x), 10 bits each
 static int64_t morton3(int64_t a, int64_t b, int64_t x) {
     int64_t m = 0;
@@ -804,26 +844,28 @@ void Plan::set_frame(int f, const SparseDev* Gu, const SparseDev* Gut, hipStream
                          [](const std::pair<int64_t, int3>& u, const std::pair<int64_t, int3>& v) {
                              return u.first < v.first;
                          });
+        // the cells of a tile in the order of cell_cube()
+        const std::vector<int3> in_tile = tile_cell_order(cell_cube());
         for (const auto& tl : tiles) {
             const int a0 = tl.second.x, b0 = tl.second.y, x0 = tl.second.z;
-                    for (int a = a0; a < std::min(a0 + 8, N[sa]); a++)
-                        for (int b = b0; b < std::min(b0 + 8, N[sb]); b++)
-                            for (int x = x0; x < std::min(x0 + 8, N[0]); x++) {
-                                c3[0] = x;
-                                c3[sa] = a;
-                                c3[sb] = b;
-                                const int64_t cell = ((int64_t)c3[2] * N[1] + c3[1]) * N[0] + c3[0];
-                                const int64_t r = cell_row[cell];
-                                const int n = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
-                                if (n == 0 && (sparse_frame || (use_ival && !inside(cell)))) continue;
-                                drow[q] = (int)cell;
-                                dnj[q] = n;
-                                for (int64_t k = n ? trp[r] : 0; k < (n ? trp[r + 1] : 0); k++, o++) {
-                                    dcol[o] = tcol[k];
-                                    dval[o] = Gut->h_m_val[k];
-                                }
-                                q++;
-                            }
+            for (const int3& d : in_tile) {
+                const int a = a0 + d.x, b = b0 + d.y, x = x0 + d.z;
+                if (a >= N[sa] || b >= N[sb] || x >= N[0]) continue;
+                c3[0] = x;
+                c3[sa] = a;
+                c3[sb] = b;
+                const int64_t cell = ((int64_t)c3[2] * N[1] + c3[1]) * N[0] + c3[0];
+                const int64_t r = cell_row[cell];
+                const int n = r < 0 ? 0 : (int)(trp[r + 1] - trp[r]);
+                if (n == 0 && (sparse_frame || (use_ival && !inside(cell)))) continue;
+                drow[q] = (int)cell;
+                dnj[q] = n;
+                for (int64_t k = n ? trp[r] : 0; k < (n ? trp[r + 1] : 0); k++, o++) {
+                    dcol[o] = tcol[k];
+                    dval[o] = Gut->h_m_val[k];
+                }
+                q++;
+            }
         }
         dnj.resize(q);
         drow.resize(q);
