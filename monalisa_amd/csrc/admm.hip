@@ -280,7 +280,6 @@ void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const
     const Frame& fr = P.frame(f);
     int64_t Pn = fr.Gu->l_size, ny = Pn * P.nCh, nu = P.nu;
     int D = P.imDim;
-    if (!(ve_max > 0)) ve_max = device_max(ve, Pn, s);
     std::vector<float> delta = schedule(delta_in, nIter), rho = schedule(rho_in, nIter);
     Dims d{};
     for (int a = 0; a < 3; a++) {
@@ -310,7 +309,7 @@ void steva(Plan& P, int f, float2* x, float2* z, float2* u, bool zu_given, const
     // data and weights in the frame's tile order for the whole solve
     w.hy0.ensure((size_t)Pn * sizeof(float));
     w.ys.ensure((size_t)ny * sizeof(float2));
-    compute_hy(ve, ve_max, w.hy0.as<float>(), Pn, s);
+    compute_hy(ve, ve_max, w.hy0.as<float>(), Pn, hy_scratch(w), s);
     P.weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
     P.to_tile_order(f, y, w.ys.as<float2>(), s);
     y = w.ys.as<float2>();
@@ -481,7 +480,7 @@ void morphosia(Plan& P, int solver, int nFr1, float2* x, float2* z, float2* u, b
     w.hy0.ensure((size_t)(ny / P.nCh) * sizeof(float));
     w.ys.ensure((size_t)ny * sizeof(float2));
     for (int f = 0; f < nFr; f++) {
-        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], s);
+        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], hy_scratch(w), s);
         P.weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / P.nCh, hy + koff[f] / P.nCh, s);
         P.to_tile_order(f, y + y_off[f], w.ys.as<float2>() + koff[f], s);
     }
@@ -694,7 +693,7 @@ void sensa_morphosia(Plan& P, float2* x, const float2* y, const long long* y_off
     double* hist = w.hist.as<double>();
     const float invHX = (float)(1.0 / P.HX);
     for (int f = 0; f < nFr; f++) {
-        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], s);
+        compute_hy(ve + ve_off[f], ve_max, w.hy0.as<float>() + koff[f] / P.nCh, Pf[f], hy_scratch(w), s);
         P.weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / P.nCh, hy + koff[f] / P.nCh, s);
         P.to_tile_order(f, y + y_off[f], w.ys.as<float2>() + koff[f], s);
     }

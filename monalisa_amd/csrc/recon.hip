@@ -284,6 +284,11 @@ __global__ void k_hy(const float* ve, float vmax, float* hy, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) hy[i] = fminf(ve[i], vmax);
 }
+// the same with the bound read from device memory
+__global__ void k_hy_dev(const float* ve, const float* vmax, float* hy, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hy[i] = fminf(ve[i], *vmax);
+}
 
 // FP64 weighted squared norm sum_i w(i) |a_i|^2 (w = per-point weight for
 // [n/nvec][nvec] data, or a constant).  Fixed launch shape and fixed
@@ -380,6 +385,35 @@ k_reduce_final(const double* partial, int np, double* out, int accumulate) {
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = accumulate ? (*out + sh[0]) : sh[0];
+}
+
+// Maximum of v[0, n) (NaN values ignored, -inf for none): partial maxima
+// per block, then one block folds them into out.  A maximum is exact, so the
+// launch shape does not change the result.
+__global__ void __launch_bounds__(kRedThreads) k_max_partial(const float* v, int64_t n, float* partial) {
+    __shared__ float sh[kRedThreads];
+    float m = -INFINITY;
+    const int64_t G = (int64_t)gridDim.x * kRedThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += G) m = fmaxf(m, v[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+__global__ void __launch_bounds__(kRedThreads) k_max_final(const float* partial, int nb, float* out) {
+    __shared__ float sh[kRedThreads];
+    float m = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) m = fmaxf(m, partial[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 // CG scalar updates (bmSensa.m:54-75):  at the top of an iteration the
@@ -1438,8 +1472,19 @@ void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, dou
     k_reduce_final<<<1, kRedThreads, 0, s>>>(partial, nb, out, acc ? 1 : 0);
     MONA_HIP(hipGetLastError());
 }
-void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s) {
-    k_hy<<<nb256(n), 256, 0, s>>>(ve, vmax, hy, n);
+void compute_hy(const float* ve, float vmax, float* hy, int64_t n, float* scratch, hipStream_t s) {
+    if (vmax > 0) {
+        k_hy<<<nb256(n), 256, 0, s>>>(ve, vmax, hy, n);
+    } else {
+        // bmSensa.m:29-30: no bound given, the maximum of ve (on the device,
+        // so the solve keeps running ahead of the host)
+        const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+        k_max_partial<<<nb, kRedThreads, 0, s>>>(ve, n, scratch);
+        MONA_HIP(hipGetLastError());
+        k_max_final<<<1, kRedThreads, 0, s>>>(scratch, nb, scratch + kRedBlocks);
+        MONA_HIP(hipGetLastError());
+        k_hy_dev<<<nb256(n), 256, 0, s>>>(ve, scratch + kRedBlocks, hy, n);
+    }
     MONA_HIP(hipGetLastError());
 }
 void cg_alpha(CgScalars* sc, hipStream_t s, double eps) {
@@ -1460,15 +1505,6 @@ void dstore(double* dst, const double* src, hipStream_t s) {
 }
 
 // max of a float array on the host side (ve_max default = max(ve))
-float device_max(const float* v, int64_t n, hipStream_t s) {
-    std::vector<float> h(n);
-    MONA_HIP(hipMemcpyAsync(h.data(), v, n * sizeof(float), hipMemcpyDeviceToHost, s));
-    MONA_HIP(hipStreamSynchronize(s));
-    float m = -INFINITY;
-    for (float x : h) m = std::max(m, x);
-    return m;
-}
-
 // ================================================================ witness
 int64_t Plan::witness_elems() const { return imDim == 3 ? (int64_t)n[0] * n[1] : nu; }
 
@@ -1492,7 +1528,6 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
     int64_t P = fr.Gu->l_size;
     int64_t ny = P * nCh;
     MONA_CHECK(fr.Gut->r_size == P, MONA_ERR_ARG, "Gut does not match Gu");
-    if (!(ve_max > 0)) ve_max = device_max(ve, P, s);
     Work& w = work;
     w.hy.ensure((size_t)P * sizeof(float));
     w.k1.ensure((size_t)ny * sizeof(float2));
@@ -1512,7 +1547,7 @@ void Plan::sensa(int f, float2* x, const float2* y, const float* ve, float ve_ma
     // k-space data and weights in the frame's tile order for the whole solve
     w.hy0.ensure((size_t)P * sizeof(float));
     w.ys.ensure((size_t)ny * sizeof(float2));
-    compute_hy(ve, ve_max, w.hy0.as<float>(), P, s);
+    compute_hy(ve, ve_max, w.hy0.as<float>(), P, hy_scratch(w), s);
     weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
     to_tile_order(f, y, w.ys.as<float2>(), s);
     y = w.ys.as<float2>();
@@ -1630,8 +1665,7 @@ void Plan::sensa_batch(int nA, float2* x, const float2* y, const long long* y_of
     double* hist = w.hist.as<double>();
     for (int f = 0; f < nFr; f++) {
         float vmax = ve_max_in ? ve_max_in[f] : 0.0f;
-        if (!(vmax > 0)) vmax = device_max(ve + ve_off[f], Pf[f], s);
-        compute_hy(ve + ve_off[f], vmax, w.hy0.as<float>() + koff[f] / nCh, Pf[f], s);
+        compute_hy(ve + ve_off[f], vmax, w.hy0.as<float>() + koff[f] / nCh, Pf[f], hy_scratch(w), s);
         weights_to_tile_order(f, w.hy0.as<float>() + koff[f] / nCh, hy + koff[f] / nCh, s);
         to_tile_order(f, y + y_off[f], ys + koff[f], s);
     }
@@ -1754,7 +1788,6 @@ void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve
     const Frame& fr = frame(f);
     int64_t P = fr.Gu->l_size;
     int64_t ny = P * nCh;
-    if (!(ve_max > 0)) ve_max = device_max(ve, P, s);
     Work& w = work;
     w.hy.ensure((size_t)P * sizeof(float));
     w.k1.ensure((size_t)ny * sizeof(float2));
@@ -1775,7 +1808,7 @@ void Plan::sensa_ir(int f, float2* x, const float2* y, const float* ve, float ve
     double* hist = w.hist.as<double>();
     w.hy0.ensure((size_t)P * sizeof(float));
     w.ys.ensure((size_t)ny * sizeof(float2));
-    compute_hy(ve, ve_max, w.hy0.as<float>(), P, s);
+    compute_hy(ve, ve_max, w.hy0.as<float>(), P, hy_scratch(w), s);
     weights_to_tile_order(f, w.hy0.as<float>(), hy, s);
     to_tile_order(f, y, w.ys.as<float2>(), s);
     y = w.ys.as<float2>();

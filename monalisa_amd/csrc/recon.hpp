@@ -69,6 +69,11 @@ struct Work {
     DevBuf kpub;                // staging of the public operator calls
     DevBuf z1, z2, z3, z4, z5, z6;  // ADMM image-space buffers (z1, z2: one block per term)
 };
+// the reduction scratch of compute_hy (the partial sums' buffer, free then)
+inline float* hy_scratch(Work& w) {
+    w.partial.ensure(1025 * sizeof(float));
+    return w.partial.as<float>();
+}
 
 struct Plan {
     int imDim = 0, nCh = 0, nWin = 3, ktype_ = 0;
@@ -203,14 +208,15 @@ void vxpby(float2* p, const float2* d, const double* b, int64_t n, hipStream_t s
 void sqnorm(const float2* a, int64_t n, const float* w, int nvec, double cw, double* out, bool acc,
             double* partial, hipStream_t s);
 void dotre(const float2* a, const float2* b, int64_t n, double cw, double* out, double* partial, hipStream_t s);
-void compute_hy(const float* ve, float vmax, float* hy, int64_t n, hipStream_t s);
+// HY = min(ve, vmax) (bmSensa.m:29-32); vmax <= 0 takes the maximum of ve,
+// reduced on the device in scratch (at least 1025 floats: hy_scratch)
+void compute_hy(const float* ve, float vmax, float* hy, int64_t n, float* scratch, hipStream_t s);
 // a = sqn / den; the sticky done flag is raised when sqn < eps (bmSensa.m:54
 // uses 10 * eps('single'))
 void cg_alpha(CgScalars* sc, hipStream_t s, double eps = 10.0 * 1.1920928955078125e-07);
 void cg_beta(CgScalars* sc, hipStream_t s);
 void cg_reset(CgScalars* sc, hipStream_t s);
 void dstore(double* dst, const double* src, hipStream_t s);
-float device_max(const float* v, int64_t n, hipStream_t s);
 
 }  // namespace mona
 
