@@ -445,6 +445,14 @@ __global__ void __launch_bounds__(NT) fft_pass_kernel(PassArgs a, const float2* 
     }
 }
 
+// Bits b in [0, NB) such that row j + STRIDE b lies in [lo, hi).
+template <int STRIDE, int NB>
+__device__ __forceinline__ uint32_t row_bits(int lo, int hi, int j) {
+    const int b0 = lo > j ? (lo - j + STRIDE - 1) / STRIDE : 0;
+    const int b1 = hi > j ? min((hi - j + STRIDE - 1) / STRIDE, NB) : 0;
+    return b1 > b0 ? ((1u << b1) - 1u) & ~((1u << b0) - 1u) : 0u;
+}
+
 // Register-resident pass for N = 16 * N2 (N2 <= 16): the four-step split
 // n = n2 + N2 n1, k = k1 + 16 k2.  A tile holds L = Bo * W lines, 16
 // work-items per line.  Work-item (line, j): step 1 (j < N2) loads rows
@@ -543,37 +551,47 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             dead = q.x < 0;
         }
     }
+    // Rows this work-item loads (j + N2 n1, bit n1 of lmask) and stores
+    // (j + N1 k2, bit k2 of smask): the input / output window, the column's
+    // interval and its cell mask, intersected once here so that the row
+    // loops below only test a bit.
+    uint32_t lmask = 0u, smask = 0u;
+    if (live && !dead) {
+        int lo = a.in_off, hi = a.in_off + a.in_cnt;
+        if (MODE == 0 && a.in_ival) {
+            lo = max(lo, ilo);
+            hi = min(hi, ihi);
+        }
+        if (j < N2) lmask = row_bits<N2, N1>(lo, hi, j);
+        if (MODE == 0 && a.in_mask) lmask &= lbits;
+        lo = a.out_off;
+        hi = a.out_off + a.out_cnt;
+        if (MODE == 0 && a.out_ival) {
+            lo = max(lo, ilo);
+            hi = min(hi, ihi);
+        }
+        if (j < N1) smask = row_bits<N1, N2>(lo, hi, j);
+        if (MODE == 0 && a.out_mask) smask &= sbits;
+    }
     // coil-map pixel of row r of this line: kin + r (MODE 1) / kout + r
-    // (MODE 2, 3), wrapped once into [0, kfc_period) -- one 64-bit modulo
-    // per work-item instead of one per row
+    // (MODE 2, 3).  The maps repeat every kfc_period pixels, a whole number
+    // of lines (checked on the host), so kin + r stays below kfc_period.
     const int64_t kin = MODE == 1 ? (o * a.in_cnt) % a.kfc_period : 0;
     const int64_t kout = MODE >= 2 ? (o * a.out_cnt) % a.kfc_period : 0;
-    auto kpix = [&](int64_t base, int r) {
-        const int64_t i = base + r;
-        return i >= a.kfc_period ? i - a.kfc_period : i;
-    };
     float2 v[N1];
     if (j < N2) {
+        // element offsets of row j - in_off (r = 0 at n1 = 0 would be row
+        // in_off); row j + N2 n1 is n1 steps of N2 rows further
+        const int64_t r0 = j - a.in_off;
+        const int64_t ioff = ibase + ws + r0 * a.in_rs, istep = (int64_t)N2 * a.in_rs;
+        const int64_t koff = (kin + r0) * a.S + ws, kstep = (int64_t)N2 * a.S;
+        const int64_t goff = o * a.in_cnt + r0;
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) {
-            const int r = j + N2 * n1 - a.in_off;
             float2 x = make_float2(0.f, 0.f);
-            const bool take = live && !dead && r >= 0 && r < a.in_cnt;
-            if (MODE == 0 && (a.in_ival || a.in_mask)) {
-                // rows outside the column's interval, or masked off (not
-                // written by the producer), are zero and not loaded
-                const int row = j + N2 * n1;
-                bool ok = take;
-                if (a.in_ival) ok = ok && row >= ilo && row < ihi;
-                if (a.in_mask) ok = ok && ((lbits >> n1) & 1u);
-                if (ok) x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
-            } else if (take) {
-                if (MODE == 1) {
-                    const int64_t ii = o * a.in_cnt + r;
-                    x = cmul(a.img[ii], a.kfc[kpix(kin, r) * a.S + ws]);
-                } else {
-                    x = ld_c(in + ibase + (int64_t)r * a.in_rs + ws);
-                }
+            if ((lmask >> n1) & 1u) {
+                if (MODE == 1) x = cmul(a.img[goff + N2 * n1], a.kfc[koff + n1 * kstep]);
+                else x = ld_c(in + ioff + n1 * istep);
             }
             v[n1] = x;
         }
@@ -582,12 +600,11 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     // loaded now, in flight with the input, instead of after the transform
     float2 kf[MODE == 2 ? N2 : 1];
     if constexpr (MODE == 2) {
+        const int64_t koff = (kout + j - a.out_off) * a.S + ws, kstep = (int64_t)N1 * a.S;
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
-            const int kl = j + N1 * k2 - a.out_off;
             kf[k2] = make_float2(0.f, 0.f);
-            if (live && j < N1 && kl >= 0 && kl < a.out_cnt)
-                kf[k2] = a.kfc[kpix(kout, kl) * a.S + ws];
+            if ((smask >> k2) & 1u) kf[k2] = a.kfc[koff + k2 * kstep];
         }
     }
     for (int i = t; i < N; i += blockDim.x) tw[i] = twg[i];
@@ -644,13 +661,11 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         const int Wp = W | 1;
         __syncthreads();
         if (live) {
+            const float sc = (a.postsign && ((j + half) & 1)) ? -a.scale : a.scale;
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++) {
-                const int k = j + N1 * k2;
-                const int kl = k - a.out_off;
-                if (kl >= 0 && kl < a.out_cnt) {
-                    float sc = a.scale;
-                    if (a.postsign && ((k + half) & 1)) sc = -sc;
+                if ((smask >> k2) & 1u) {
+                    const int kl = j + N1 * k2 - a.out_off;
                     const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
                     ex[(ob * a.out_cnt + kl) * Wp + w] = cmul(y, kf[k2]);
                 }
@@ -674,23 +689,22 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         return;
     }
     if (!live || dead) return;
+    // postsign: (k + N/2) is odd for every k = j + 16 k2 of this work-item
+    // or for none (16 is even)
+    const float sc = (a.postsign && ((j + half) & 1)) ? -a.scale : a.scale;
+    const int64_t r0 = j - a.out_off;
+    const int64_t ooff = obase + ws + r0 * a.out_rs, ostep = (int64_t)N1 * a.out_rs;
+    const int64_t koff = (kout + r0) * a.S + ws, kstep = (int64_t)N1 * a.S;
+    const int64_t goff = (o * a.out_cnt + r0) * a.S + ws;
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
-        const int k = j + N1 * k2;
-        const int kl = k - a.out_off;
-        bool put = kl >= 0 && kl < a.out_cnt;
-        if (MODE == 0 && a.out_mask) put = put && ((sbits >> k2) & 1u);
-        if (MODE == 0 && a.out_ival) put = put && k >= ilo && k < ihi;
-        if (put) {
-            float sc = a.scale;
-            if (a.postsign && ((k + half) & 1)) sc = -sc;
+        if ((smask >> k2) & 1u) {
             const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
             if (MODE == 3) {
-                const int64_t i = o * a.out_cnt + kl;
-                const float2 p = cmul(y, a.kfc[kpix(kout, kl) * a.S + ws]);
-                st_c(out + i * a.S + ws, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
+                const float2 p = cmul(y, a.kfc[koff + k2 * kstep]);
+                st_c(out + goff + k2 * kstep, make_float2(p.x * a.epi_scale, p.y * a.epi_scale));
             } else {
-                st_c(out + obase + (int64_t)kl * a.out_rs + ws, y);
+                st_c(out + ooff + k2 * ostep, y);
             }
         }
     }
@@ -895,6 +909,10 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     a.img = p.img;
     a.kfc = p.kfc;
     a.kfc_period = p.kfc_period > 0 ? p.kfc_period : INT64_MAX;
+    // the register kernel adds the row to a line's first coil-map pixel
+    // without wrapping: a period must hold whole lines
+    MONA_CHECK(p.kfc_period <= 0 || (p.mode == 1 ? p.kfc_period % p.in_cnt : p.kfc_period % p.out_cnt) == 0,
+               MONA_ERR_INTERNAL, "coil-map period is not a whole number of lines");
     a.img_out = p.img_out;
     a.in_mask = p.in_mask;
     a.out_mask = p.out_mask;
