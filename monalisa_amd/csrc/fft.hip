@@ -63,6 +63,10 @@ struct PassArgs {
     int64_t in_rs, out_rs;     // row strides of in / out (S unless the layout interleaves lines)
     int64_t in_on, in_os2;     // two-level line offsets (line_off); in_on = 0: o * in_os
     int64_t out_on, out_os2;
+    // register kernel: ceil(2^20 / L) and ceil(2^20 / W) (exact quotients of
+    // work-item indices below 1024), ceil(2^32 / mask_nch) (exact for inner
+    // positions below 2^32 / mask_nch; 0 when mask_nch is 1)
+    uint32_t lmag, wmag, nchmag;
 };
 
 // Offset of line o: o * os, or with on > 0 (o / on) * os2 + (o % on) * os
@@ -478,8 +482,9 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     float2* ex = smem + N;  // exchange planes
     const int W = a.W, L = a.W * a.Bo, XS = a.xs;
     const int t = threadIdx.x;
-    const int line = t % L, j = t / L;
-    const int ob = line / W, w = line - ob * W;
+    const int j = (int)(__umul24((unsigned)t, a.lmag) >> 20);  // t / L
+    const int line = t - j * L;
+    const int ob = (int)(__umul24((unsigned)line, a.wmag) >> 20), w = line - ob * W;
     const int64_t tile_w = blockIdx.x % a.ntile_w, to = blockIdx.x / a.ntile_w;
     const int64_t o = to * a.Bo + ob;
     const int64_t ws = tile_w * W + w;
@@ -487,6 +492,9 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int64_t ibase = line_off(o, a.in_os, a.in_on, a.in_os2);
     const int64_t obase = line_off(o, a.out_os, a.out_on, a.out_os2);
     const float sg = a.inverse ? 1.0f : -1.0f;
+    // column of the masks and intervals: o * mask_cpl + ws / mask_nch
+    const int64_t mcol =
+        o * a.mask_cpl + (a.nchmag ? (int64_t)__umulhi((uint32_t)ws, a.nchmag) : ws);
 
     // masked passes: the bits of this work-item's column, reduced at once to
     // the bits of the rows it loads (j + N2 n1, bit n1 of lbits) and stores
@@ -498,7 +506,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const uint32_t* mask = MODE == 0 ? (a.in_mask ? a.in_mask : a.out_mask) : nullptr;
     if (mask && live) {
         // a column's words are MW * 4 bytes at a multiple of that: vector loads
-        const uint32_t* mp = mask + (o * a.mask_cpl + ws / a.mask_nch) * MW;
+        const uint32_t* mp = mask + mcol * MW;
         uint32_t mw[MW];
         if constexpr (MW % 4 == 0) {
 #pragma unroll
@@ -521,21 +529,20 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             for (int i = 0; i < MW; i++) mw[i] = mp[i];
         }
         lbits = sbits = 0u;
+        // row j + N2 n1 (j < N2 <= 16; rows of j >= N2 are never loaded)
+        // lies in word (N2 n1) / 32 or the next one, at most word MW - 1;
+        // row j + 16 k2 (j < 16) in word k2 / 2
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) {
             const int row = j + N2 * n1;
-            uint32_t wv = 0u;
-#pragma unroll
-            for (int i = 0; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
+            const int w0 = (N2 * n1) >> 5, w1 = (N2 * n1 + N2 - 1) >> 5;
+            const uint32_t wv = (w1 != w0 && row >= 32 * w1) ? mw[w1] : mw[w0];
             lbits |= ((wv >> (row & 31)) & 1u) << n1;
         }
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
             const int row = j + N1 * k2;
-            uint32_t wv = 0u;
-#pragma unroll
-            for (int i = 0; i < MW; i++) wv = (row >> 5) == i ? mw[i] : wv;
-            sbits |= ((wv >> (row & 31)) & 1u) << k2;
+            sbits |= ((mw[k2 >> 1] >> (row & 31)) & 1u) << k2;
         }
     }
     // interval passes: this work-item's column keeps rows [ilo, ihi); a dead
@@ -545,7 +552,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     {
         const int2* iv = MODE == 0 ? (a.in_ival ? a.in_ival : a.out_ival) : nullptr;
         if (iv && live) {
-            const int2 q = iv[o * a.mask_cpl + ws / a.mask_nch];
+            const int2 q = iv[mcol];
             ilo = q.x;
             ihi = q.y;
             dead = q.x < 0;
@@ -923,6 +930,16 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     const bool limited = p.in_mask || p.out_mask || p.in_ival || p.out_ival;
     MONA_CHECK(!limited || (p.mode == 0 && p.mask_nch >= 1 && p.S % p.mask_nch == 0), MONA_ERR_INTERNAL,
                "cell masks / intervals need a plain pass and S divisible by mask_nch");
+    // exact quotients in the kernel: t / L and line / W for t < 1024
+    // (2^20 scale), inner position / mask_nch below 2^32 / mask_nch
+    a.lmag = (uint32_t)(((1u << 20) + L - 1) / L);
+    a.wmag = (uint32_t)(((1u << 20) + a.W - 1) / a.W);
+    a.nchmag = 0;
+    if (limited && p.mask_nch > 1) {
+        MONA_CHECK((uint64_t)p.S * (uint64_t)p.mask_nch < (1ull << 32), MONA_ERR_INTERNAL,
+                   "mask columns exceed the 32-bit quotient");
+        a.nchmag = (uint32_t)(((1ull << 32) + p.mask_nch - 1) / p.mask_nch);
+    }
     const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     const int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
