@@ -471,7 +471,10 @@ __device__ __forceinline__ uint32_t row_bits(int lo, int hi, int j) {
 // SPLIT: the exchange goes through LDS in two halves of eight planes
 // (k1 < 8, then k1 >= 8), so a tile needs half the exchange space and more
 // tiles fit on a CU.
-template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
+// DIR: 0 direction from a.inverse; 1 forward, 2 inverse as a compile-time
+// constant (the plain passes), which folds the signs of the DFT butterflies
+// and of the twiddles into the arithmetic.
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0>
 __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
     const TI* __restrict__ in = reinterpret_cast<const TI*>(a.in);
     TO* __restrict__ out = reinterpret_cast<TO*>(a.out);
@@ -491,7 +494,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const bool live = o < a.outer && ws < a.S;
     const int64_t ibase = line_off(o, a.in_os, a.in_on, a.in_os2);
     const int64_t obase = line_off(o, a.out_os, a.out_on, a.out_os2);
-    const float sg = a.inverse ? 1.0f : -1.0f;
+    const float sg = DIR == 1 ? -1.0f : DIR == 2 ? 1.0f : (a.inverse ? 1.0f : -1.0f);
     // column of the masks and intervals: o * mask_cpl + ws / mask_nch
     const int64_t mcol =
         o * a.mask_cpl + (a.nchmag ? (int64_t)__umulhi((uint32_t)ws, a.nchmag) : ws);
@@ -798,9 +801,9 @@ int reg_n2(int N) {
     }
 }
 
-template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
-void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
-    auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT>;
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0>
+void launch_reg_dir(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+    auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT, DIR>;
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
@@ -809,6 +812,17 @@ void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, co
         });
     }
     kern<<<nblocks, nt, lds, s>>>(a, tw);
+}
+
+// plain passes with the direction as a template argument
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
+void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+    if constexpr (MODE == 0) {
+        if (a.inverse) launch_reg_dir<N2, 0, TI, TO, SPLIT, 2>(a, nblocks, nt, lds, tw, s);
+        else launch_reg_dir<N2, 0, TI, TO, SPLIT, 1>(a, nblocks, nt, lds, tw, s);
+    } else {
+        launch_reg_dir<N2, MODE, TI, TO, SPLIT, 0>(a, nblocks, nt, lds, tw, s);
+    }
 }
 
 // Storage variants: float2 throughout, or bf16x2 for the grid-side arrays
