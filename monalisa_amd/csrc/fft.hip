@@ -499,6 +499,14 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int64_t mcol =
         o * a.mask_cpl + (a.nchmag ? (int64_t)__umulhi((uint32_t)ws, a.nchmag) : ws);
 
+    // interval passes: this work-item's column keeps rows [ilo, ihi); a dead
+    // column (-1, -1) is neither loaded nor stored.  The interval is loaded
+    // here, in flight with the mask words below.
+    int2 civ = make_int2(0, N);
+    {
+        const int2* iv = MODE == 0 ? (a.in_ival ? a.in_ival : a.out_ival) : nullptr;
+        if (iv && live) civ = iv[mcol];
+    }
     // masked passes: the bits of this work-item's column, reduced at once to
     // the bits of the rows it loads (j + N2 n1, bit n1 of lbits) and stores
     // (j + N1 k2, bit k2 of sbits), with compile-time word indices only (an
@@ -548,19 +556,8 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             sbits |= ((mw[k2 >> 1] >> (row & 31)) & 1u) << k2;
         }
     }
-    // interval passes: this work-item's column keeps rows [ilo, ihi); a dead
-    // column (-1, -1) is neither loaded nor stored
-    int ilo = 0, ihi = N;
-    bool dead = false;
-    {
-        const int2* iv = MODE == 0 ? (a.in_ival ? a.in_ival : a.out_ival) : nullptr;
-        if (iv && live) {
-            const int2 q = iv[mcol];
-            ilo = q.x;
-            ihi = q.y;
-            dead = q.x < 0;
-        }
-    }
+    const int ilo = civ.x, ihi = civ.y;
+    const bool dead = civ.x < 0;
     // Rows this work-item loads (j + N2 n1, bit n1 of lmask) and stores
     // (j + N1 k2, bit k2 of smask): the input / output window, the column's
     // interval and its cell mask, intersected once here so that the row
