@@ -227,6 +227,16 @@ __device__ __forceinline__ void st_n(bf16x2* p, const float* ar, const float* ai
     else p->v = u[0];
 }
 
+// Staged item i of a group or range is value i % n_vec of column
+// i / n_vec.  The quotient as one multiply-high: with m = ceil(2^32 / n_vec)
+// (m = 0 stands for n_vec = 1), i / n_vec = umulhi(i, m) exactly for
+// i < 2^32 / n_vec (a stage holds at most 1024 columns of at most 256
+// values, so i n_vec < 2^26).
+__device__ __forceinline__ uint32_t item_magic(int n_vec) {
+    return n_vec == 1 ? 0u : 0xFFFFFFFFu / (uint32_t)n_vec + 1u;
+}
+__device__ __forceinline__ int item_col(int i, uint32_t m) { return m ? (int)__umulhi((uint32_t)i, m) : i; }
+
 // Vector offsets of work-item kq (of n_vec / CPT per row) for float2
 // vectors at CPT = 4: values k0, k0 + 1 and k1, k1 + 1.  With halves
 // (k0 = 2 kq, k1 = 2 kq + n_vec / 2) the first 16-byte loads of a row's
@@ -386,20 +396,22 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
     // vectors: item i = value i % n_vec of column i / n_vec, for every
     // column slot of the group (the padding repeats its last column)
     const int items = ccap * n_vec;
+    const uint32_t nmag = item_magic(n_vec);
     for (int i0 = threadIdx.x; i0 < items; i0 += 8 * 256) {
-        int c[8], dst[8];
+        int c[8], dst[8], r[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int i = min(i0 + u * 256, items - 1);
-            const int k = i / n_vec;
+            const int k = item_col(i, nmag);
+            r[u] = i - k * n_vec;
             c[u] = gc[k];
-            dst[u] = k * ls + (i - k * n_vec);
+            dst[u] = k * ls + r[u];
         }
         float2 x[8];
         float sc[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            x[u] = ld_c(v + (int64_t)c[u] * n_vec + (dst[u] % ls));
+            x[u] = ld_c(v + (int64_t)c[u] * n_vec + r[u]);
             if (SCALE) sc[u] = scale[c[u]];
         }
 #pragma unroll
@@ -495,6 +507,7 @@ spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, con
     const int4 R = ranges[blockIdx.x];
     const int c0 = R.x, items = R.y * n_vec;
     const TV* vb = v + (int64_t)c0 * n_vec;
+    const uint32_t nmag = item_magic(n_vec);
     for (int i0 = threadIdx.x; i0 < items; i0 += 8 * 256) {
         int k[8], i[8];
         float2 x[8];
@@ -502,7 +515,7 @@ spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, con
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             i[u] = min(i0 + u * 256, items - 1);
-            k[u] = i[u] / n_vec;
+            k[u] = item_col(i[u], nmag);
             x[u] = ld_c(vb + i[u]);
             if (SCALE) sc[u] = scale[c0 + k[u]];
         }
