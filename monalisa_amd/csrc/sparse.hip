@@ -381,10 +381,13 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             const SpEntry* __restrict__ cv, const uint16_t* __restrict__ slot, const int* __restrict__ gcol,
             const int* __restrict__ rowidx, int n_vec, const TV* __restrict__ v, TW* __restrict__ w,
             const float* __restrict__ scale, int xg, int ccap, int ecap, int rcap) {
-    extern __shared__ float2 sv[];  // [ccap][n_vec + 1] vectors, [ecap] entries, row offsets, output rows
+    // [ccap + 1][n_vec + 1] vectors (the last one -0), [ecap + 1] entries
+    // (the last one (1, slot ccap): a batch's surplus entries read it and
+    // add 1 * -0 = -0, which leaves every sum unchanged), row offsets, output rows
+    extern __shared__ float2 sv[];
     const int ls = n_vec + 1;
-    float2* se = sv + ccap * ls;                    // (value, slot bits)
-    int* ro = reinterpret_cast<int*>(se + ecap);    // [rows + 1]
+    float2* se = sv + (ccap + 1) * ls;              // (value, slot bits)
+    int* ro = reinterpret_cast<int*>(se + ecap + 1);  // [rows + 1]
     int* rw = ro + rcap + 1;                        // [rows]
     // the group's descriptor, entry bounds and (padded) columns are found
     // from the group index alone: their loads go out together
@@ -440,6 +443,8 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
         ro[i] = (int)(rowptr[G.x + i] - e0);
         if (i < nr) rw[i] = rowidx ? rowidx[G.x + i] : G.x + i;
     }
+    for (int i = threadIdx.x; i < n_vec; i += 256) sv[ccap * ls + i] = make_float2(-0.0f, -0.0f);
+    if (threadIdx.x == 0) se[ecap] = make_float2(1.0f, __int_as_float(ccap));
     __syncthreads();
     const int tpr = n_vec / CPT;
     const int rpp = 256 / tpr;  // rows per pass
@@ -456,10 +461,9 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
 #pragma unroll
         for (int h = 0; h < CPT; h++) ar[h] = ai[h] = 0.0f;
         for (int j = jb; j < je; j += 4) {
-            const int cnt = min(je - j, 4);
             float2 e[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) e[u] = se[min(j + u, je - 1)];
+            for (int u = 0; u < 4; u++) e[u] = se[j + u < je ? j + u : ecap];
             float2 xv[4][CPT];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -474,16 +478,15 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
                     for (int h = 0; h < CPT; h++) xv[u][h] = xs[k0 + h];
                 }
             }
-            // a surplus entry adds -0, which leaves every sum unchanged
-            // (+0, -0, infinities and NaN included)
+            // a surplus entry adds 1 * -0 = -0, which leaves every sum
+            // unchanged (+0, -0, infinities and NaN included)
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const bool on = u < cnt;
 #pragma unroll
                 for (int h = 0; h < CPT; h++) {
                     const float pr = e[u].x * xv[u][h].x, pi = e[u].x * xv[u][h].y;
-                    ar[h] = ar[h] + (on ? pr : -0.0f);
-                    ai[h] = ai[h] + (on ? pi : -0.0f);
+                    ar[h] = ar[h] + pr;
+                    ai[h] = ai[h] + pi;
                 }
             }
         }
@@ -502,7 +505,9 @@ template <bool SCALE, int CPT, class TV>
 __global__ void __launch_bounds__(256)
 spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, const SpEntry* __restrict__ cv,
             int n_vec, const TV* __restrict__ v, const float* __restrict__ scale, float2* __restrict__ partial) {
-    extern __shared__ float2 sv[];  // [column count][n_vec + 1]
+    // [column count + 1][n_vec + 1]: the range's columns, then -0 values
+    // that a batch's surplus entries (value 1) add, leaving the sums unchanged
+    extern __shared__ float2 sv[];
     const int ls = n_vec + 1;
     const int4 R = ranges[blockIdx.x];
     const int c0 = R.x, items = R.y * n_vec;
@@ -528,7 +533,9 @@ spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, con
             sv[k[u] * ls + (i[u] - k[u] * n_vec)] = x[u];  // a surplus item rewrites item items - 1's value
         }
     }
+    for (int i = threadIdx.x; i < n_vec; i += 256) sv[R.y * ls + i] = make_float2(-0.0f, -0.0f);
     __syncthreads();
+    const SpEntry null_entry{c0 + R.y, 1.0f};
     const int tpr = n_vec / CPT;
     const int spp = 256 / tpr;  // segments per pass
     const int rr = threadIdx.x / tpr, q = threadIdx.x - rr * tpr;
@@ -541,10 +548,12 @@ spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, con
 #pragma unroll
         for (int h = 0; h < CPT; h++) ar[h] = ai[h] = 0.0f;
         for (int e = 0; e < sg.len; e += 4) {
-            const int cnt = min(sg.len - e, 4);
             SpEntry en[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) en[u] = cv[sg.eb + min(e + u, sg.len - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (e + u >= sg.len) en[u] = null_entry;
             float2 xv[4][CPT];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -559,15 +568,14 @@ spmv_ranges(const int4* __restrict__ ranges, const SpSeg* __restrict__ segs, con
                     for (int h = 0; h < CPT; h++) xv[u][h] = xs[k0 + h];
                 }
             }
-            // a surplus entry adds -0, which leaves every sum unchanged
+            // a surplus entry adds 1 * -0 = -0, which leaves every sum unchanged
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const bool on = u < cnt;
 #pragma unroll
                 for (int h = 0; h < CPT; h++) {
                     const float pr = en[u].val * xv[u][h].x, pi = en[u].val * xv[u][h].y;
-                    ar[h] = ar[h] + (on ? pr : -0.0f);
-                    ai[h] = ai[h] + (on ? pi : -0.0f);
+                    ar[h] = ar[h] + pr;
+                    ai[h] = ai[h] + pi;
                 }
             }
         }
@@ -1148,7 +1156,7 @@ void spmv_ranges_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w,
     if (V.n_long == 0) return;
     S.partial.ensure((size_t)S.n_rpart * n_vec * sizeof(float2));
     float2* part = S.partial.as<float2>();
-    const size_t lds = (size_t)S.range_cap * (n_vec + 1) * sizeof(float2);
+    const size_t lds = (size_t)(S.range_cap + 1) * (n_vec + 1) * sizeof(float2);
     if (yscale)
         spmv_ranges<true, CPT><<<V.n_range, 256, lds, s>>>(S.range.as<int4>(), S.seg.as<SpSeg>(), S.cv.as<SpEntry>(),
                                                            n_vec, v, yscale, part);
@@ -1168,8 +1176,8 @@ template <int CPT, class TV, class TW>
 void spmv_groups_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
                         hipStream_t s) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
-    const size_t lds = (size_t)S.group_cap * (n_vec + 1) * sizeof(float2) + (size_t)S.group_ecap * sizeof(float2) +
-                       (size_t)(2 * S.group_rcap + 1) * sizeof(int);
+    const size_t lds = (size_t)(S.group_cap + 1) * (n_vec + 1) * sizeof(float2) +
+                       (size_t)(S.group_ecap + 1) * sizeof(float2) + (size_t)(2 * S.group_rcap + 1) * sizeof(int);
     const int xg = xcd_group(yscale != nullptr);
     if (yscale)
         spmv_groups<true, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.group_e.as<int64_t>(),
