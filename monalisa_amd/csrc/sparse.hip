@@ -382,11 +382,12 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             const int* __restrict__ rowidx, int n_vec, const TV* __restrict__ v, TW* __restrict__ w,
             const float* __restrict__ scale, int xg, int ccap, int ecap, int rcap) {
     // [ccap + 1][n_vec + 1] vectors (the last one -0), [ecap + 1] entries
-    // (the last one (1, slot ccap): a batch's surplus entries read it and
-    // add 1 * -0 = -0, which leaves every sum unchanged), row offsets, output rows
+    // (value, offset of the entry's vector; the last one (1, the -0
+    // vector): a batch's surplus entries read it and add 1 * -0 = -0,
+    // which leaves every sum unchanged), row offsets, output rows
     extern __shared__ float2 sv[];
     const int ls = n_vec + 1;
-    float2* se = sv + (ccap + 1) * ls;              // (value, slot bits)
+    float2* se = sv + (ccap + 1) * ls;              // (value, vector offset bits)
     int* ro = reinterpret_cast<int*>(se + ecap + 1);  // [rows + 1]
     int* rw = ro + rcap + 1;                        // [rows]
     // the group's descriptor, entry bounds and (padded) columns are found
@@ -437,14 +438,14 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             sl[u] = slot[e0 + t[u]];
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) se[t[u]] = make_float2(m[u], __int_as_float(sl[u]));
+        for (int u = 0; u < 4; u++) se[t[u]] = make_float2(m[u], __int_as_float(sl[u] * ls));
     }
     for (int i = threadIdx.x; i <= nr; i += 256) {
         ro[i] = (int)(rowptr[G.x + i] - e0);
         if (i < nr) rw[i] = rowidx ? rowidx[G.x + i] : G.x + i;
     }
     for (int i = threadIdx.x; i < n_vec; i += 256) sv[ccap * ls + i] = make_float2(-0.0f, -0.0f);
-    if (threadIdx.x == 0) se[ecap] = make_float2(1.0f, __int_as_float(ccap));
+    if (threadIdx.x == 0) se[ecap] = make_float2(1.0f, __int_as_float(ccap * ls));
     __syncthreads();
     const int tpr = n_vec / CPT;
     const int rpp = 256 / tpr;  // rows per pass
@@ -467,7 +468,7 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             float2 xv[4][CPT];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const float2* xs = sv + __float_as_int(e[u].y) * ls;
+                const float2* xs = sv + __float_as_int(e[u].y);
                 if constexpr (CPT == 4) {
                     xv[u][0] = xs[k0];
                     xv[u][1] = xs[k0 + 1];
