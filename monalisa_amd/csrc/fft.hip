@@ -959,12 +959,14 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     const size_t coil_sum = p.mode == 2 ? (size_t)a.Bo * p.out_cnt * (a.W | 1) : 0;
     size_t ex = std::max<size_t>((size_t)16 * xs, coil_sum);
     // passes whose exchange needs more than 20 KiB (fewer than eight tiles
-    // on a CU) exchange in two halves (MONA_FFT_SPLIT_KB sets the bound)
+    // on a CU) exchange in two halves (MONA_FFT_SPLIT_KB sets the bound);
+    // the coil-sum epilogue (mode 2) exchanges in one piece: its two extra
+    // barriers cost more than the added tiles per CU gain
     static const size_t split_bytes = [] {
         const char* e = std::getenv("MONA_FFT_SPLIT_KB");
         return (size_t)(e ? std::atoi(e) : 20) * 1024;
     }();
-    const bool split = split_exchange() && ((size_t)N + ex) * sizeof(float2) > split_bytes;
+    const bool split = split_exchange() && p.mode != 2 && ((size_t)N + ex) * sizeof(float2) > split_bytes;
     if (split) ex = std::max<size_t>((size_t)8 * xs, coil_sum);
     size_t lds = ((size_t)N + ex) * sizeof(float2);
     MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
