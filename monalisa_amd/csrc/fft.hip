@@ -67,6 +67,9 @@ struct PassArgs {
     // work-item indices below 1024), ceil(2^32 / mask_nch) (exact for inner
     // positions below 2^32 / mask_nch; 0 when mask_nch is 1)
     uint32_t lmag, wmag, nchmag;
+    // register kernel: ceil(2^32 / d) for the divisors ntile_w, in_on and
+    // out_on (exact for the quotients the launch takes; 0: divide)
+    uint32_t twmag, inmag, outmag;
 };
 
 // Offset of line o: o * os, or with on > 0 (o / on) * os2 + (o % on) * os
@@ -75,6 +78,17 @@ __device__ __forceinline__ int64_t line_off(int64_t o, int64_t os, int64_t on, i
     if (on <= 0) return o * os;
     const int64_t hi = o / on;
     return hi * os2 + (o - hi * on) * os;
+}
+// The same with o / on as a multiply-high when mag = ceil(2^32 / on) is
+// exact for every line of the launch (o on < 2^32, checked on the host)
+__device__ __forceinline__ int64_t line_off_m(int64_t o, int64_t os, int64_t on, int64_t os2, uint32_t mag) {
+    if (on <= 0) return o * os;
+    const int64_t hi = mag ? (int64_t)__umulhi((uint32_t)o, mag) : o / on;
+    return hi * os2 + (o - hi * on) * os;
+}
+// n / d for the host-computed magic of d (mag = 0: divide)
+__device__ __forceinline__ int64_t div_m(int64_t n, int64_t d, uint32_t mag) {
+    return mag ? (int64_t)__umulhi((uint32_t)n, mag) : n / d;
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -488,12 +502,12 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int j = (int)(__umul24((unsigned)t, a.lmag) >> 20);  // t / L
     const int line = t - j * L;
     const int ob = (int)(__umul24((unsigned)line, a.wmag) >> 20), w = line - ob * W;
-    const int64_t tile_w = blockIdx.x % a.ntile_w, to = blockIdx.x / a.ntile_w;
+    const int64_t to = div_m(blockIdx.x, a.ntile_w, a.twmag), tile_w = blockIdx.x - to * a.ntile_w;
     const int64_t o = to * a.Bo + ob;
     const int64_t ws = tile_w * W + w;
     const bool live = o < a.outer && ws < a.S;
-    const int64_t ibase = line_off(o, a.in_os, a.in_on, a.in_os2);
-    const int64_t obase = line_off(o, a.out_os, a.out_on, a.out_os2);
+    const int64_t ibase = line_off_m(o, a.in_os, a.in_on, a.in_os2, a.inmag);
+    const int64_t obase = line_off_m(o, a.out_os, a.out_on, a.out_os2, a.outmag);
     const float sg = DIR == 1 ? -1.0f : DIR == 2 ? 1.0f : (a.inverse ? 1.0f : -1.0f);
     // column of the masks and intervals: o * mask_cpl + ws / mask_nch
     const int64_t mcol =
@@ -954,6 +968,15 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     const int64_t ntile_o = (p.outer + a.Bo - 1) / a.Bo;
     const int64_t nblocks = a.ntile_w * ntile_o;
     MONA_CHECK(nblocks < 0x7fffffff, MONA_ERR_ARG, "FFT grid too large");
+    // multiply-high quotients where exact: n / d with n d < 2^32 for every
+    // numerator of the launch (blocks, or lines up to ntile_o Bo)
+    auto magic = [](int64_t d, int64_t nmax) -> uint32_t {
+        if (d <= 1 || nmax * d >= ((int64_t)1 << 32)) return 0u;
+        return (uint32_t)((((int64_t)1 << 32) + d - 1) / d);
+    };
+    a.twmag = magic(a.ntile_w, nblocks);
+    a.inmag = magic(a.in_on, ntile_o * a.Bo);
+    a.outmag = magic(a.out_on, ntile_o * a.Bo);
     const int nt = 16 * L;
     MONA_CHECK(nt <= 1024 && (p.mode != 2 || a.W == p.S), MONA_ERR_INTERNAL, "bad register FFT tile");
     const size_t coil_sum = p.mode == 2 ? (size_t)a.Bo * p.out_cnt * (a.W | 1) : 0;
