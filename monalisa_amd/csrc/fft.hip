@@ -1027,9 +1027,10 @@ void fft_pass(const AxisPass& p, hipStream_t s) {
     if (p.bf_in || p.bf_out) {
         MONA_CHECK(n2 && (p.mode == 0 || p.S <= 64), MONA_ERR_ARG,
                    "bf16 storage needs FFT lengths 16 * {2,3,4,5,6,8,10,12,16} and <= 64 coils");
-        // 4-byte elements: 32 lines (128-byte row segments) measured faster
-        // than 64 (256-byte segments, half the tiles per CU)
-        fft_pass_reg(p, n2, p.lines > 0 ? p.lines : 32, s);
+        // 4-byte elements: a pass writing bf16 takes 32 lines (128-byte row
+        // segments; 64 lines: 27% slower at C1, 16 lines: 26% slower),
+        // a pass reading bf16 16 lines (32: 12% slower, 64: 49% slower)
+        fft_pass_reg(p, n2, p.lines > 0 ? p.lines : (p.bf_in ? 16 : 32), s);
         return;
     }
     if (n2 && ((p.mode == 0 && lines > 0) || (p.mode != 0 && p.S <= 64 && (lines > 0 || !old_fits)))) {
