@@ -385,9 +385,16 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
     // (value, offset of the entry's vector; the last one (1, the -0
     // vector): a batch's surplus entries read it and add 1 * -0 = -0,
     // which leaves every sum unchanged), row offsets, output rows
+    // bf16 vectors are staged as they are stored (4 bytes a value, half the
+    // LDS, so more groups fit a CU), with the column's scale in the padding
+    // slot n_vec of its row; the sums widen and scale them as they read
+    // them, the same float operations as staging float values.
+    constexpr bool BF = std::is_same<TV, bf16x2>::value;
     extern __shared__ float2 sv[];
+    uint32_t* sb = reinterpret_cast<uint32_t*>(sv);
     const int ls = n_vec + 1;
-    float2* se = sv + (ccap + 1) * ls;              // (value, vector offset bits)
+    const int vwords = BF ? ((ccap + 1) * ls + 1) / 2 : (ccap + 1) * ls;  // in float2 units
+    float2* se = sv + vwords;                       // (value, vector offset bits)
     int* ro = reinterpret_cast<int*>(se + ecap + 1);  // [rows + 1]
     int* rw = ro + rcap + 1;                        // [rows]
     // the group's descriptor, entry bounds and (padded) columns are found
@@ -411,21 +418,33 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             c[u] = gc[k];
             dst[u] = k * ls + r[u];
         }
-        float2 x[8];
-        float sc[8];
+        if constexpr (BF) {
+            uint32_t x[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            x[u] = ld_c(v + (int64_t)c[u] * n_vec + r[u]);
-            if (SCALE) sc[u] = scale[c[u]];
-        }
+            for (int u = 0; u < 8; u++) x[u] = reinterpret_cast<const uint32_t*>(v)[(int64_t)c[u] * n_vec + r[u]];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (SCALE) {
-                x[u].x = sc[u] * x[u].x;
-                x[u].y = sc[u] * x[u].y;
+            for (int u = 0; u < 8; u++) sb[dst[u]] = x[u];  // a surplus item rewrites item items - 1's value
+        } else {
+            float2 x[8];
+            float sc[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                x[u] = ld_c(v + (int64_t)c[u] * n_vec + r[u]);
+                if (SCALE) sc[u] = scale[c[u]];
             }
-            sv[dst[u]] = x[u];  // a surplus item rewrites item items - 1's value
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (SCALE) {
+                    x[u].x = sc[u] * x[u].x;
+                    x[u].y = sc[u] * x[u].y;
+                }
+                sv[dst[u]] = x[u];  // a surplus item rewrites item items - 1's value
+            }
         }
+    }
+    if constexpr (BF) {
+        for (int k = threadIdx.x; k < ccap; k += 256)
+            sb[k * ls + n_vec] = __float_as_uint(SCALE ? scale[gc[k]] : 1.0f);
     }
     // entries and row offsets
     for (int i0 = threadIdx.x; i0 < ne; i0 += 4 * 256) {
@@ -444,7 +463,12 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
         ro[i] = (int)(rowptr[G.x + i] - e0);
         if (i < nr) rw[i] = rowidx ? rowidx[G.x + i] : G.x + i;
     }
-    for (int i = threadIdx.x; i < n_vec; i += 256) sv[ccap * ls + i] = make_float2(-0.0f, -0.0f);
+    if constexpr (BF) {
+        for (int i = threadIdx.x; i < n_vec; i += 256) sb[ccap * ls + i] = 0x80008000u;  // (-0, -0)
+        if (threadIdx.x == 0) sb[ccap * ls + n_vec] = __float_as_uint(1.0f);
+    } else {
+        for (int i = threadIdx.x; i < n_vec; i += 256) sv[ccap * ls + i] = make_float2(-0.0f, -0.0f);
+    }
     if (threadIdx.x == 0) se[ecap] = make_float2(1.0f, __int_as_float(ccap * ls));
     __syncthreads();
     const int tpr = n_vec / CPT;
@@ -468,6 +492,17 @@ spmv_groups(const int4* __restrict__ groups, const int64_t* __restrict__ gent, c
             float2 xv[4][CPT];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
+                if constexpr (BF) {
+                    // widened and scaled as the float stage would hold them
+                    const uint32_t* xs = sb + __float_as_int(e[u].y);
+                    const float sc = __uint_as_float(xs[n_vec]);
+#pragma unroll
+                    for (int h = 0; h < CPT; h++) {
+                        const uint32_t b = xs[k0 + h];
+                        xv[u][h] = make_float2(sc * __uint_as_float(b << 16), sc * __uint_as_float(b & 0xffff0000u));
+                    }
+                    continue;
+                }
                 const float2* xs = sv + __float_as_int(e[u].y);
                 if constexpr (CPT == 4) {
                     xv[u][0] = xs[k0];
@@ -1177,8 +1212,10 @@ template <int CPT, class TV, class TW>
 void spmv_groups_launch(const SparseDev& S, const SpView& V, const TV* v, TW* w, int n_vec, const float* yscale,
                         hipStream_t s) {
     const int* ri = S.squeezed ? S.rowidx.as<int>() : nullptr;
-    const size_t lds = (size_t)(S.group_cap + 1) * (n_vec + 1) * sizeof(float2) +
-                       (size_t)(S.group_ecap + 1) * sizeof(float2) + (size_t)(2 * S.group_rcap + 1) * sizeof(int);
+    constexpr bool BF = std::is_same<TV, bf16x2>::value;
+    const size_t vwords = BF ? ((size_t)(S.group_cap + 1) * (n_vec + 1) + 1) / 2 : (size_t)(S.group_cap + 1) * (n_vec + 1);
+    const size_t lds = vwords * sizeof(float2) + (size_t)(S.group_ecap + 1) * sizeof(float2) +
+                       (size_t)(2 * S.group_rcap + 1) * sizeof(int);
     const int xg = xcd_group(yscale != nullptr);
     if (yscale)
         spmv_groups<true, CPT><<<V.n_group, 256, lds, s>>>(S.group.as<int4>(), S.group_e.as<int64_t>(),
