@@ -70,6 +70,7 @@ struct PassArgs {
     // register kernel: ceil(2^32 / d) for the divisors ntile_w, in_on and
     // out_on (exact for the quotients the launch takes; 0: divide)
     uint32_t twmag, inmag, outmag;
+    int xg;  // register kernel: runs of consecutive tiles per XCD (1 = round-robin)
 };
 
 // Offset of line o: o * os, or with on > 0 (o / on) * os2 + (o % on) * os
@@ -502,7 +503,8 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int j = (int)(__umul24((unsigned)t, a.lmag) >> 20);  // t / L
     const int line = t - j * L;
     const int ob = (int)(__umul24((unsigned)line, a.wmag) >> 20), w = line - ob * W;
-    const int64_t to = div_m(blockIdx.x, a.ntile_w, a.twmag), tile_w = blockIdx.x - to * a.ntile_w;
+    const int64_t bid = xcd_block(blockIdx.x, gridDim.x, a.xg);
+    const int64_t to = div_m(bid, a.ntile_w, a.twmag), tile_w = bid - to * a.ntile_w;
     const int64_t o = to * a.Bo + ob;
     const int64_t ws = tile_w * W + w;
     const bool live = o < a.outer && ws < a.S;
@@ -975,6 +977,14 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
         return (uint32_t)((((int64_t)1 << 32) + d - 1) / d);
     };
     a.twmag = magic(a.ntile_w, nblocks);
+    // plain passes: runs of 4 consecutive tiles per XCD (neighbouring row
+    // segments share an L2; C1 z passes 2-3% faster, the fused x passes
+    // not), MONA_XCD_GROUP_FFT overrides it
+    static const int fft_xg = [] {
+        const char* e = std::getenv("MONA_XCD_GROUP_FFT");
+        return e ? std::max(1, std::atoi(e)) : 0;
+    }();
+    a.xg = fft_xg ? fft_xg : (p.mode == 0 ? 4 : 1);
     a.inmag = magic(a.in_on, ntile_o * a.Bo);
     a.outmag = magic(a.out_on, ntile_o * a.Bo);
     const int nt = 16 * L;
