@@ -630,8 +630,23 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             if ((smask >> k2) & 1u) kf[k2] = a.kfc[koff + k2 * kstep];
         }
     }
+    // plain passes: a tile none of whose work-items stores a row (every line
+    // dead, out of range or with an empty interval) leaves after the first
+    // barrier.  Each wave records whether any of its work-items stores; all
+    // work-items read the same records, so the whole tile leaves together.
+    __shared__ uint32_t wstore[16];
+    if constexpr (MODE == 0) {
+        const int any = __any(smask != 0u);
+        if ((t & 63) == 0) wstore[t >> 6] = any ? 1u : 0u;
+    }
     for (int i = t; i < N; i += blockDim.x) tw[i] = twg[i];
     __syncthreads();
+    if constexpr (MODE == 0) {
+        uint32_t st = 0u;
+        const int nw = ((int)blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; i++) st |= wstore[i];
+        if (st == 0u) return;
+    }
     if (j < N2) {
         if (a.presign) {
 #pragma unroll
@@ -817,11 +832,13 @@ int reg_n2(int N) {
 template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0>
 void launch_reg_dir(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
     auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT, DIR>;
-    if (lds > 65536) {
+    // the kernel's static LDS (16 words of wave records) counts too
+    if (lds + 16 * sizeof(uint32_t) > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024 - (int)(16 * sizeof(uint32_t)));
         });
     }
     kern<<<nblocks, nt, lds, s>>>(a, tw);
@@ -1002,7 +1019,7 @@ void fft_pass_reg(const AxisPass& p, int n2, int lines, hipStream_t s) {
     const bool split = split_exchange() && p.mode != 2 && ((size_t)N + ex) * sizeof(float2) > split_bytes;
     if (split) ex = std::max<size_t>((size_t)8 * xs, coil_sum);
     size_t lds = ((size_t)N + ex) * sizeof(float2);
-    MONA_CHECK(lds <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
+    MONA_CHECK(lds + 16 * sizeof(uint32_t) <= 160 * 1024, MONA_ERR_INTERNAL, "register FFT tile exceeds the LDS");
     const float2* tw = twiddles().get(N);
     switch (n2) {
         case 2: launch_reg<2>(a, p.mode, p.bf_in, p.bf_out, (unsigned)nblocks, nt, lds, tw, s, split); break;
