@@ -204,6 +204,43 @@ template <> struct Dft<16> {
             for (int k2 = 0; k2 < 4; k2++) a[k1 + 4 * k2] = t[k2];
         }
     }
+    // The same DFT when only a[4..11] can be non-zero (a line whose input is
+    // the central half): each first-layer radix-4 butterfly then has inputs
+    // (0, p, q, 0), giving (q + p, -q + s p, q - p, -q - s p) with s p = p
+    // times sg i, the values the full butterfly computes (up to the sign of
+    // zero) in half the additions.  a[0..3] and a[12..15] are not read.
+    __device__ static void run_mid(float2* a, float sg) {
+        const float c[10] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                             0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+                             -1.0f, -0.92387953251128675613f};
+        const float sn[10] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+                              1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                              0.0f, -0.38268343236508977173f};
+        float2 b[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            const float2 p = a[4 + n2], q = a[8 + n2];
+            const float2 sp = mul_si(p, sg);
+            float2 t[4];
+            t[0] = cadd(q, p);
+            t[1] = make_float2(sp.x - q.x, sp.y - q.y);
+            t[2] = csub(q, p);
+            t[3] = make_float2(-q.x - sp.x, -q.y - sp.y);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) {
+                int m = n2 * k1;
+                float2 w = make_float2(c[m], sg * sn[m]);
+                b[n2][k1] = (m == 0) ? t[k1] : cmul(t[k1], w);
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            float2 t[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+            Dft<4>::run(t, sg);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) a[k1 + 4 * k2] = t[k2];
+        }
+    }
 };
 
 // Composite register DFTs R = A * B (Cooley-Tukey, n = B n1 + n2,
@@ -489,8 +526,19 @@ __device__ __forceinline__ uint32_t row_bits(int lo, int hi, int j) {
 // DIR: 0 direction from a.inverse; 1 forward, 2 inverse as a compile-time
 // constant (the plain passes), which folds the signs of the DFT butterflies
 // and of the twiddles into the arithmetic.
-template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0>
+// WIN: 1 when the input window is exactly the central half, rows [N/4,
+// 3N/4) (the zero-filled forward passes): a work-item's rows j + N2 n1 are
+// then inside only for n1 in [4, 12), the others are zero at compile time
+// and step 1 runs the half-input DFT.  2 when the output window is exactly
+// the central half (the cropping adjoint passes, N2 a multiple of 4): rows
+// j + 16 k2 are kept only for k2 in [N2/4, 3 N2/4), and the outputs of the
+// other k2 are never computed.  0: windows from the arguments alone.
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0, int WIN = 0>
 __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2* __restrict__ twg) {
+    static_assert(WIN != 2 || N2 % 4 == 0, "central output window needs N2 % 4 == 0");
+    // compile-time window of the rows of step 1 (n1) and step 2 (k2)
+    constexpr auto in_win = [](int n1) { return WIN != 1 || (n1 >= 4 && n1 < 12); };
+    constexpr auto out_win = [](int k2) { return WIN != 2 || (k2 >= N2 / 4 && k2 < 3 * N2 / 4); };
     const TI* __restrict__ in = reinterpret_cast<const TI*>(a.in);
     TO* __restrict__ out = reinterpret_cast<TO*>(a.out);
     constexpr int N1 = 16;
@@ -555,21 +603,29 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
 #pragma unroll
             for (int i = 0; i < MW; i++) mw[i] = mp[i];
         }
-        lbits = sbits = 0u;
         // row j + N2 n1 (j < N2 <= 16; rows of j >= N2 are never loaded)
         // lies in word (N2 n1) / 32 or the next one, at most word MW - 1;
-        // row j + 16 k2 (j < 16) in word k2 / 2
+        // row j + 16 k2 (j < 16) in word k2 / 2.  Only the side that has
+        // the mask takes its bits.
+        if (a.in_mask) {
+            lbits = 0u;
 #pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) {
-            const int row = j + N2 * n1;
-            const int w0 = (N2 * n1) >> 5, w1 = (N2 * n1 + N2 - 1) >> 5;
-            const uint32_t wv = (w1 != w0 && row >= 32 * w1) ? mw[w1] : mw[w0];
-            lbits |= ((wv >> (row & 31)) & 1u) << n1;
+            for (int n1 = 0; n1 < N1; n1++) {
+                if (!in_win(n1)) continue;
+                const int row = j + N2 * n1;
+                const int w0 = (N2 * n1) >> 5, w1 = (N2 * n1 + N2 - 1) >> 5;
+                const uint32_t wv = (w1 != w0 && row >= 32 * w1) ? mw[w1] : mw[w0];
+                lbits |= ((wv >> (row & 31)) & 1u) << n1;
+            }
         }
+        if (a.out_mask) {
+            sbits = 0u;
 #pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) {
-            const int row = j + N1 * k2;
-            sbits |= ((mw[k2 >> 1] >> (row & 31)) & 1u) << k2;
+            for (int k2 = 0; k2 < N2; k2++) {
+                if (!out_win(k2)) continue;
+                const int row = j + N1 * k2;
+                sbits |= ((mw[k2 >> 1] >> (row & 31)) & 1u) << k2;
+            }
         }
     }
     const int ilo = civ.x, ihi = civ.y;
@@ -612,7 +668,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) {
             float2 x = make_float2(0.f, 0.f);
-            if ((lmask >> n1) & 1u) {
+            if (in_win(n1) && ((lmask >> n1) & 1u)) {
                 if (MODE == 1) x = cmul(a.img[goff + N2 * n1], a.kfc[koff + n1 * kstep]);
                 else x = ld_c(in + ioff + n1 * istep);
             }
@@ -627,7 +683,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
             kf[k2] = make_float2(0.f, 0.f);
-            if ((smask >> k2) & 1u) kf[k2] = a.kfc[koff + k2 * kstep];
+            if (out_win(k2) && ((smask >> k2) & 1u)) kf[k2] = a.kfc[koff + k2 * kstep];
         }
     }
     // plain passes: a tile none of whose work-items stores a row (every line
@@ -651,9 +707,10 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         if (a.presign) {
 #pragma unroll
             for (int n1 = 0; n1 < N1; n1++)
-                if ((j + N2 * n1) & 1) v[n1] = make_float2(-v[n1].x, -v[n1].y);
+                if (in_win(n1) && ((j + N2 * n1) & 1)) v[n1] = make_float2(-v[n1].x, -v[n1].y);
         }
-        Dft<N1>::run(v, sg);
+        if constexpr (WIN == 1) Dft<N1>::run_mid(v, sg);
+        else Dft<N1>::run(v, sg);
 #pragma unroll
         for (int k1 = 1; k1 < N1; k1++) {
             float2 wv = tw[j * k1];
@@ -702,7 +759,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             const float sc = (a.postsign && ((j + half) & 1)) ? -a.scale : a.scale;
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++) {
-                if ((smask >> k2) & 1u) {
+                if (out_win(k2) && ((smask >> k2) & 1u)) {
                     const int kl = j + N1 * k2 - a.out_off;
                     const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
                     ex[(ob * a.out_cnt + kl) * Wp + w] = cmul(y, kf[k2]);
@@ -736,7 +793,7 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
     const int64_t goff = (o * a.out_cnt + r0) * a.S + ws;
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
-        if ((smask >> k2) & 1u) {
+        if (out_win(k2) && ((smask >> k2) & 1u)) {
             const float2 y = make_float2(u[k2].x * sc, u[k2].y * sc);
             if (MODE == 3) {
                 const float2 p = cmul(y, a.kfc[koff + k2 * kstep]);
@@ -829,9 +886,9 @@ int reg_n2(int N) {
     }
 }
 
-template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0>
+template <int N2, int MODE, class TI, class TO, bool SPLIT = false, int DIR = 0, int WIN = 0>
 void launch_reg_dir(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
-    auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT, DIR>;
+    auto kern = fft_reg_kernel<N2, MODE, TI, TO, SPLIT, DIR, WIN>;
     // the kernel's static LDS (16 words of wave records) counts too
     if (lds + 16 * sizeof(uint32_t) > 65536) {
         static std::once_flag once;
@@ -844,12 +901,45 @@ void launch_reg_dir(const PassArgs& a, unsigned nblocks, int nt, size_t lds, con
     kern<<<nblocks, nt, lds, s>>>(a, tw);
 }
 
-// plain passes with the direction as a template argument
+// plain passes with the direction as a template argument; passes whose
+// input (forward, x .* KFC) or output (adjoint, coil sum) window is exactly
+// the central half with that window at compile time (N2 a multiple of 4)
+// (MONA_FFT_WIN=0 keeps the windows at run time)
+bool fft_win_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("MONA_FFT_WIN");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
 template <int N2, int MODE, class TI, class TO, bool SPLIT = false>
 void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, const float2* tw, hipStream_t s) {
+    constexpr int N = 16 * N2;
+    constexpr bool wins = N2 % 4 == 0;
+    const bool on = wins && fft_win_on();
+    const bool cin = on && a.in_off == N / 4 && a.in_cnt == N / 2;
+    const bool cout = on && a.out_off == N / 4 && a.out_cnt == N / 2;
     if constexpr (MODE == 0) {
-        if (a.inverse) launch_reg_dir<N2, 0, TI, TO, SPLIT, 2>(a, nblocks, nt, lds, tw, s);
-        else launch_reg_dir<N2, 0, TI, TO, SPLIT, 1>(a, nblocks, nt, lds, tw, s);
+        if (a.inverse) {
+            if constexpr (wins)
+                if (cout) return launch_reg_dir<N2, 0, TI, TO, SPLIT, 2, 2>(a, nblocks, nt, lds, tw, s);
+            launch_reg_dir<N2, 0, TI, TO, SPLIT, 2>(a, nblocks, nt, lds, tw, s);
+        } else {
+            if constexpr (wins)
+                if (cin) return launch_reg_dir<N2, 0, TI, TO, SPLIT, 1, 1>(a, nblocks, nt, lds, tw, s);
+            launch_reg_dir<N2, 0, TI, TO, SPLIT, 1>(a, nblocks, nt, lds, tw, s);
+        }
+    } else if constexpr (MODE == 1) {
+        // not at N2 = 16: there the half-input x pass takes 76 VGPRs instead
+        // of 72 (six waves a SIMD instead of seven) and ran 6% slower
+        if constexpr (wins && N2 < 16)
+            if (cin) return launch_reg_dir<N2, 1, TI, TO, SPLIT, 0, 1>(a, nblocks, nt, lds, tw, s);
+        launch_reg_dir<N2, 1, TI, TO, SPLIT, 0>(a, nblocks, nt, lds, tw, s);
+    } else if constexpr (MODE == 2) {
+        if constexpr (wins)
+            if (cout) return launch_reg_dir<N2, 2, TI, TO, SPLIT, 0, 2>(a, nblocks, nt, lds, tw, s);
+        launch_reg_dir<N2, 2, TI, TO, SPLIT, 0>(a, nblocks, nt, lds, tw, s);
     } else {
         launch_reg_dir<N2, MODE, TI, TO, SPLIT, 0>(a, nblocks, nt, lds, tw, s);
     }
