@@ -207,8 +207,10 @@ template <> struct Dft<16> {
     // The same DFT when only a[4..11] can be non-zero (a line whose input is
     // the central half): each first-layer radix-4 butterfly then has inputs
     // (0, p, q, 0), giving (q + p, -q + s p, q - p, -q - s p) with s p = p
-    // times sg i, the values the full butterfly computes (up to the sign of
-    // zero) in half the additions.  a[0..3] and a[12..15] are not read.
+    // times sg i, in half the additions.  a[0..3] and a[12..15] are not
+    // read.  Results agree with run() on such input to float rounding (the
+    // compiler may fuse the surrounding products into multiply-adds
+    // differently).
     __device__ static void run_mid(float2* a, float sg) {
         const float c[10] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
                              0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
