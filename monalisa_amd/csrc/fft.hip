@@ -706,18 +706,23 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
         if (st == 0u) return;
     }
     if (j < N2) {
-        if (a.presign) {
+        // a work-item that loaded no row holds zeros, which it passes on
+        // untransformed (a wave all of whose work-items did so skips the
+        // arithmetic)
+        if (lmask != 0u) {
+            if (a.presign) {
 #pragma unroll
-            for (int n1 = 0; n1 < N1; n1++)
-                if (in_win(n1) && ((j + N2 * n1) & 1)) v[n1] = make_float2(-v[n1].x, -v[n1].y);
-        }
-        if constexpr (WIN == 1) Dft<N1>::run_mid(v, sg);
-        else Dft<N1>::run(v, sg);
+                for (int n1 = 0; n1 < N1; n1++)
+                    if (in_win(n1) && ((j + N2 * n1) & 1)) v[n1] = make_float2(-v[n1].x, -v[n1].y);
+            }
+            if constexpr (WIN == 1) Dft<N1>::run_mid(v, sg);
+            else Dft<N1>::run(v, sg);
 #pragma unroll
-        for (int k1 = 1; k1 < N1; k1++) {
-            float2 wv = tw[j * k1];
-            wv.y *= -sg;  // table holds exp(-2 pi i t / N); conj for the inverse
-            v[k1] = cmul(v[k1], wv);
+            for (int k1 = 1; k1 < N1; k1++) {
+                float2 wv = tw[j * k1];
+                wv.y *= -sg;  // table holds exp(-2 pi i t / N); conj for the inverse
+                v[k1] = cmul(v[k1], wv);
+            }
         }
         if constexpr (!SPLIT) {
 #pragma unroll
@@ -750,7 +755,8 @@ __global__ void __launch_bounds__(1024) fft_reg_kernel(PassArgs a, const float2*
             for (int n2 = 0; n2 < N2; n2++) u[n2] = ex[(j - N1 / 2) * XS + n2 * L + line];
         }
     }
-    Dft<N2>::run(u, sg);
+    // only work-items that store a row need step 2's outputs
+    if (smask != 0u) Dft<N2>::run(u, sg);
     const int half = N / 2;
     if (MODE == 2) {
         // products with KFC_conj into LDS as [ob][kept row][coil] (odd
