@@ -939,9 +939,7 @@ void launch_reg_mode(const PassArgs& a, unsigned nblocks, int nt, size_t lds, co
             launch_reg_dir<N2, 0, TI, TO, SPLIT, 1>(a, nblocks, nt, lds, tw, s);
         }
     } else if constexpr (MODE == 1) {
-        // not at N2 = 16: there the half-input x pass takes 76 VGPRs instead
-        // of 72 (six waves a SIMD instead of seven) and ran 6% slower
-        if constexpr (wins && N2 < 16)
+        if constexpr (wins)
             if (cin) return launch_reg_dir<N2, 1, TI, TO, SPLIT, 0, 1>(a, nblocks, nt, lds, tw, s);
         launch_reg_dir<N2, 1, TI, TO, SPLIT, 0>(a, nblocks, nt, lds, tw, s);
     } else if constexpr (MODE == 2) {
